@@ -1,0 +1,157 @@
+#!/usr/bin/env python3
+"""Benchmark of the raster join on one MI355X.
+
+Workload: raster join at eps = 10 m, count() and avg(fare), over 100M points
+(70% Gaussian mixture / 30% uniform in a 50 km box) and 260 Voronoi polygons
+with jittered shared edges; the points are resident in HBM (a spatially
+organized table) before the timed region starts. One step = one complete
+join: the polygon label map is rebuilt from the polygons (no caching across
+steps) and the points are streamed once.
+
+Prints one JSON line with the throughput, the roofline of the point pass
+(HIP events over the timed region) and a CPU baseline (the oracle's CPUJoin
+restatement on the host cores, on a bounded sample of the same points).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "raster-join_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+import rasterjoin as rj  # noqa: E402
+
+METRIC = "G points joined/s at ε=10m + served queries/s with p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(xy, off, pts, fare, target_s=10.0, max_n=None):
+    """CPUJoin restatement (grid 1024, avg(fare)) on the host cores."""
+    from oracle import pyoracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    g = O.Grid(xy, off, 1024, nthreads=threads)
+    n_total = len(fare)
+    probe = min(n_total, 1_000_000)
+    t = time.perf_counter()
+    O.join_exact(g, O.PointSet(pts[:2 * probe], [fare[:probe]], [rj.FLOAT], (), agg_type=rj.AVG,
+                               agg_col=0), nthreads=threads)
+    rate = probe / max(time.perf_counter() - t, 1e-6)
+    n = int(min(n_total, max(probe, rate * target_s), max_n or n_total))
+    ps = O.PointSet(pts[:2 * n], [fare[:n]], [rj.FLOAT], (), agg_type=rj.AVG, agg_col=0)
+    best = None
+    for _ in range(2):
+        t = time.perf_counter()
+        O.join_exact(g, ps, nthreads=threads)
+        dt = time.perf_counter() - t
+        best = dt if best is None else min(best, dt)
+    return {"value": n / best / 1e9, "unit": "G points/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} of the {n_total} points, exact CPUJoin join (grid 1024) with "
+                      f"avg(fare), min of 2 runs, {best:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--points", type=int, default=100_000_000)
+    ap.add_argument("--eps", type=float, default=10.0)
+    ap.add_argument("--path", type=int, default=0, help="0 auto, 1 canvas, 2 label map")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(points would shard by row range with one all-reduce of the per-polygon sums)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    n = args.points
+    cp = rj.CONFIG_POLYS["voronoi260"]
+    t0 = time.perf_counter()
+    xy, off = rj.gen_voronoi(cp["nsites"], seed=cp["seed"], spacing=cp["spacing"], jitter=cp["jitter"])
+    pts, tm, fare, pas = rj.gen_points(n, params=rj.point_params(seed=7))
+    log(f"generated {n} points, {len(off) - 1} polygons in {time.perf_counter() - t0:.1f} s")
+
+    h = rj.Handler()
+    t0 = time.perf_counter()
+    table = rj.Table(h, pts, (tm, fare, pas), (rj.UINT, rj.FLOAT, rj.UINT8), organize=True)
+    h.sync()
+    load_s = time.perf_counter() - t0
+    log(f"resident table loaded and organized in {load_s:.2f} s")
+    h.set_polygons(xy, off)
+    h.bind_table(table)
+    h.set_aggregation(rj.AVG, 1)
+    h.set_accuracy_distance(args.eps)
+    h.set_option(rj.OPT_RASTER_PATH, args.path)
+    h.set_option(rj.OPT_CACHE, 0)
+    h.set_option(rj.OPT_TIMING, 1)
+    W, H = h.canvas_size()
+
+    for _ in range(args.warmup):
+        h.execute_function(rj.RASTER)
+    h.sync()
+    ktot = {}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        h.execute_function(rj.RASTER)
+        for k, (ms, cnt) in h.kernel_times().items():
+            a = ktot.setdefault(k, [0.0, 0])
+            a[0] += ms
+            a[1] += cnt
+    h.sync()
+    dt = time.perf_counter() - t0
+    counts, sums = h.execute_function(rj.RASTER)
+    value = n * args.steps / dt / 1e9
+    path = "label" if "points_label" in ktot else "canvas"
+    log("kernels (ms total, launches):", {k: (round(v[0], 3), v[1]) for k, v in ktot.items()})
+
+    # Roofline of the point pass: per launch it reads x,y (8 B) and fare
+    # (4 B) of every point and, at most once, each 4-byte label of the map.
+    kname = "points_label" if path == "label" else "splat"
+    k_ms = ktot[kname][0] / ktot[kname][1]
+    if path == "label":
+        alg_bytes = n * 12 + W * H * 4
+    else:
+        alg_bytes = n * 12 + W * H * 20
+    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "G points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+int64",
+        "data": "synthetic",
+        "config": {"workload": "raster join eps=10 m, count()+avg(fare), 100M points, 260 Voronoi "
+                               "polygons, resident organized table",
+                   "points": n, "polygons": len(off) - 1, "eps_m": args.eps, "canvas": [W, H],
+                   "path": path, "table_load_s": round(load_s, 3),
+                   "points_in_polygons": int(counts.sum())},
+        "roofline": {"kernel": kname, "bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel_ms": round(k_ms, 4), "alg_bytes": int(alg_bytes)},
+        "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in ktot.items()},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(xy, off, pts, fare)
+    h.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

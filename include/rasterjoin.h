@@ -1,0 +1,142 @@
+/* rasterjoin.h -- C ABI of the MI355X raster-join library (librasterjoin.so).
+ *
+ * The entry points replace the reference's join API, the GLHandler singleton
+ * and its GLFunction plugins (RasterJoin/join/GLHandler.hpp:20-78), with plain
+ * pointers and sizes. Reference paths are relative to the reference root.
+ *
+ * Conventions
+ *  - Every function returns an int status: RJ_OK (0) or a negative RJ_ERR_*.
+ *    Nothing calls exit(); rj_last_error() explains the last failure.
+ *  - Output buffers are owned by the caller.
+ *  - One context per host thread; several contexts may share a device. Work is
+ *    ordered on the context's own HIP stream; entry points return after the
+ *    results they report are on the host.
+ */
+#ifndef RASTERJOIN_H
+#define RASTERJOIN_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RJ_OK 0
+#define RJ_ERR_ARG (-1)         /* bad argument */
+#define RJ_ERR_STATE (-2)       /* missing polygons / points, wrong call order */
+#define RJ_ERR_NOMEM (-3)       /* device allocation failed or memory budget exhausted */
+#define RJ_ERR_DEVICE (-4)      /* HIP runtime error */
+#define RJ_ERR_LIMIT (-5)       /* too many constraints or columns (GLFunction.cpp:250-253, :308) */
+#define RJ_ERR_UNSUPPORTED (-6) /* function not supported for this input */
+
+/* FunctionType, GLHandler.hpp:23-29 */
+enum { RJ_RASTER = 0, RJ_INDEX = 1, RJ_HYBRID = 2, RJ_ERRORBOUNDS = 3 };
+/* AttributeType, common/Common.h:20, plus a 1-byte unsigned storage type */
+enum { RJ_ATTR_LOCATION = 0, RJ_ATTR_UINT = 1, RJ_ATTR_INT = 2, RJ_ATTR_FLOAT = 3, RJ_ATTR_UINT8 = 4 };
+/* ConstraintType, common/Common.h:22 */
+enum { RJ_EQ = 0, RJ_LT = 1, RJ_LTE = 2, RJ_GT = 3, RJ_GTE = 4 };
+/* Aggregation, common/Common.h:26 */
+enum { RJ_AGG_COUNT = 0, RJ_AGG_AVG = 1, RJ_AGG_SUM = 2 };
+
+#define RJ_MAX_CONSTRAINTS 10 /* GLFunction.cpp:308 */
+#define RJ_MAX_COLUMNS 8      /* columns of one point table (a query may use at most 5) */
+
+/* 16-byte filter record, the layout of QueryConstraint (common/Common.h:34-39).
+ * attrib_id is the index of the column in the table's column list. Uint and
+ * Int columns compare as integers, Float columns as floats. */
+typedef struct rj_constraint {
+    int32_t attrib_id;
+    int32_t type;
+    union { int32_t ival; float fval; uint32_t uval; } val;
+    int32_t dummy;
+} rj_constraint;
+
+typedef struct rj_ctx rj_ctx;
+typedef struct rj_table rj_table;
+
+/* GLHandler::getInstance(gpuMemMB, inMemory), GLHandler.cpp:144-149 (the
+ * budget keeps the reference's 100 MB reserve, GLHandler.cpp:128-132). */
+int rj_create(int64_t gpu_mem_mb, int32_t inmem, rj_ctx** out);
+/* GLHandler::~GLHandler, GLHandler.cpp:136-142 */
+int rj_destroy(rj_ctx* ctx);
+const char* rj_last_error(const rj_ctx* ctx);
+
+/* Polygon collection: PolyHandler::addPolygonCollection, db/PolyHandler.cpp:49-75
+ * (CSR vertices `polys` + `pindexes`). xy: 2*off[np] floats; polygons are
+ * open rings (no repeated closing vertex) of >= 3 vertices. bbox (minx, miny,
+ * maxx, maxy) may be NULL: it is then the vertex min/max, as readPolygons
+ * computes it (PolyHandler.cpp:171-182). */
+int rj_set_polygons(rj_ctx* ctx, const float* xy, const int32_t* off, int32_t np, const double* bbox);
+
+/* Point columns: DataHandler / QueryResult::getAttribute (db/DataHandler.hpp:14-41,
+ * db/QueryResult.hpp:56-79). xy is the location column (2n floats); cols[k]
+ * has n entries of types[k] (RJ_ATTR_*). The data is copied into a context-
+ * owned device table; host buffers may be freed afterwards. */
+int rj_set_points(rj_ctx* ctx, int64_t n, const float* xy, int32_t ncols, const void* const* cols,
+                  const int32_t* types);
+
+/* Resident point tables (replacing the disk backend, db/HashGridIndex.cpp:78-137).
+ * organize = 1 orders the rows by a spatial (Morton) key so that splats and
+ * lookups stay local; join results do not depend on row order. */
+int rj_table_create(rj_ctx* ctx, int64_t n, const float* xy, int32_t ncols, const void* const* cols,
+                    const int32_t* types, int32_t organize, rj_table** out);
+int rj_table_rows(const rj_table* t, int64_t* n);
+/* Copy rows [row0, row0+n) back to the host (any pointer may be NULL). */
+int rj_table_read(const rj_table* t, int64_t row0, int64_t n, float* xy, void* const* cols);
+int rj_table_destroy(rj_table* t);
+/* Use rows [row_begin, row_end) of a table as the context's points. */
+int rj_bind_table(rj_ctx* ctx, rj_table* t, int64_t row_begin, int64_t row_end);
+
+/* DataHandler::setQueryConstraints (DataHandler.cpp:64-66); at most 10. */
+int rj_set_constraints(rj_ctx* ctx, const rj_constraint* c, int32_t n);
+/* DataHandler::setAggregation (DataHandler.cpp:68-71): type RJ_AGG_*, col = column index. */
+int rj_set_aggregation(rj_ctx* ctx, int32_t type, int32_t col);
+/* GLHandler::setAccuracyDistance (GLHandler.cpp:250-257) -> GLFunction::setMaxCellSize
+ * (GLFunction.cpp:26-30): pixel side = (eps/sqrt 2)/ground_res. ground_res is
+ * meters per coordinate unit (1 for projected meters). */
+int rj_set_accuracy(rj_ctx* ctx, double eps_m, double ground_res);
+/* GLHandler::setPolyIndexResolution (GLHandler.cpp:259-262). The grid is
+ * square; ry is accepted and ignored as in the reference (GLFunction.cpp:76). */
+int rj_set_index_res(rj_ctx* ctx, int32_t rx, int32_t ry);
+
+/* GLHandler::executeFunction (GLHandler.cpp:188-204). counts: np entries.
+ * sums (may be NULL): np doubles, the aggregate sum per polygon (avg = sum /
+ * count). legacy3np (may be NULL): the reference's int32 result layout
+ * [count | sum/100 | decimal*10] (polygon.frag:19-24, main.cpp:206-209). */
+int rj_execute(rj_ctx* ctx, int32_t fn, int64_t* counts, double* sums, int32_t* legacy3np);
+/* Exact sums of the last execute: sum = hi + lo * 2^-32 per polygon. */
+int rj_get_sums_fixed(rj_ctx* ctx, int64_t* hi, uint64_t* lo);
+/* GLHandler::getErrorBounds (GLHandler.cpp:206-212): 4*np values
+ * [fn | fp | fn1 | fp1] of the last RJ_ERRORBOUNDS execute. */
+int rj_error_bounds(rj_ctx* ctx, int64_t* out4np);
+/* GLHandler::printTimeStats (GLHandler.cpp:214-248): the tab-separated timing
+ * line (minimum over executions, ms). */
+int rj_time_stats(rj_ctx* ctx, int32_t fn, char* buf, size_t len);
+
+/* Canvas geometry of the current polygons + accuracy (RasterJoin.cpp:51-78). */
+int rj_canvas_info(rj_ctx* ctx, int32_t* W, int32_t* H);
+
+/* Tuning and path selection (no reference counterpart).
+ *   RJ_OPT_RASTER_PATH:     0 auto, 1 canvas splat + polygon scan, 2 polygon label map
+ *   RJ_OPT_TIMING:          1 records device time per kernel with HIP events
+ *   RJ_OPT_HYBRID_DILATION: border dilation radius in pixels, 0 = derived from the
+ *                           float error bound of the exact test (at least 1)
+ *   RJ_OPT_CACHE:           1 (default) keeps polygon-side structures (snapped
+ *                           polygons, label maps, grid index) across executes;
+ *                           0 rebuilds them on every execute as the reference does
+ *   RJ_OPT_ORGANIZE:        1 (default) rj_set_points orders rows spatially */
+enum { RJ_OPT_RASTER_PATH = 1, RJ_OPT_TIMING = 2, RJ_OPT_HYBRID_DILATION = 3, RJ_OPT_CACHE = 4,
+       RJ_OPT_ORGANIZE = 5 };
+int rj_set_option(rj_ctx* ctx, int32_t key, int64_t value);
+
+/* Per-kernel device time of the last execute (timing on): name i, total ms and
+ * launches. Returns the number of kernels recorded, or < 0. */
+int rj_kernel_times(rj_ctx* ctx, int32_t i, char* name, size_t name_len, double* ms, int32_t* launches);
+
+/* Synchronize the context's stream. */
+int rj_sync(rj_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

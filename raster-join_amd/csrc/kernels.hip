@@ -1,0 +1,750 @@
+// kernels.hip -- gfx950 kernels of the spatial aggregation join.
+//
+//  k_splat     point splat into the count/sum canvas (the reference's point
+//              pass, RasterJoin.cpp:91-148 + points.vert/points.frag), with
+//              wave-level run aggregation of equal pixels before the atomics.
+//  k_band      scan conversion of one band of rows of one polygon with the
+//              edge-flag rule on the fixed-point canvas grid; per mode it sums
+//              the canvas under the polygon (polygon.frag), writes the polygon
+//              label map, or accumulates the error bounds (line-bound.frag).
+//  k_touch     boundary pixels of every edge (hybrid border mask, the
+//              conservative line raster of HybridJoin::drawOutline).
+//  k_points    one pass over the points: CPUJoin grid + exact float PIP
+//              (index.glsl), label-map gather (raster), or label gather with
+//              the exact test on border pixels (hybrid.glsl).
+//
+// Compiled with -ffp-contract=off: the float PIP, the grid cell and the pixel
+// formula must round exactly like the host restatement of CPUJoin.
+#include <algorithm>
+
+#include "rj_dev.hpp"
+
+namespace rj {
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int64_t floor_div(int64_t a, int64_t b)
+{
+    int64_t q = a / b;
+    if ((a % b != 0) && (a < 0)) q--;
+    return q;
+}
+
+__device__ __forceinline__ int64_t ceil_div(int64_t a, int64_t b)
+{
+    int64_t q = a / b;
+    if ((a % b != 0) && (a > 0)) q++;
+    return q;
+}
+
+template <class T>
+__device__ __forceinline__ T cmp_op(int op, T a, T b)
+{
+    return op == RJ_EQ ? a == b : op == RJ_LT ? a < b : op == RJ_LTE ? a <= b : op == RJ_GT ? a > b : a >= b;
+}
+
+// Conjunctive filters; integer compares for Uint/Int columns (the reference's
+// float-bit equality on LTE/GTE, points.vert:99,107, is not reproduced).
+__device__ __forceinline__ bool pass_filters(const DevCols& C, const DevQuery& q, int64_t i)
+{
+    for (int k = 0; k < q.ncons; k++) {
+        const int col = q.con_col[k];
+        const int t = C.type[col];
+        const int op = q.con_op[k];
+        const uint32_t v = q.con_val[k];
+        bool ok;
+        if (t == RJ_ATTR_FLOAT) {
+            float a = static_cast<const float*>(C.col[col])[i];
+            ok = cmp_op<float>(op, a, __uint_as_float(v));
+        } else if (t == RJ_ATTR_INT) {
+            int32_t a = static_cast<const int32_t*>(C.col[col])[i];
+            ok = cmp_op<int32_t>(op, a, (int32_t)v);
+        } else {
+            uint32_t a = (t == RJ_ATTR_UINT8) ? (uint32_t) static_cast<const uint8_t*>(C.col[col])[i]
+                                              : static_cast<const uint32_t*>(C.col[col])[i];
+            ok = cmp_op<uint32_t>(op, a, v);
+        }
+        if (!ok) return false;
+    }
+    return true;
+}
+
+// Exact fixed-point aggregate value: hi = floor(v), lo = frac(v) * 2^32.
+__device__ __forceinline__ void agg_value(const DevCols& C, const DevQuery& q, int64_t i,
+                                          unsigned long long& hi, unsigned long long& lo)
+{
+    const int col = q.agg_col;
+    const int t = C.type[col];
+    lo = 0;
+    if (t == RJ_ATTR_FLOAT) {
+        float f = static_cast<const float*>(C.col[col])[i];
+        if (!(fabsf(f) < 4.611686018427387904e18f)) {
+            hi = 0;
+            return;
+        }
+        float fl = floorf(f);
+        hi = (unsigned long long)(long long)fl;
+        lo = (unsigned long long)((f - fl) * 4294967296.0f);
+    } else if (t == RJ_ATTR_INT) {
+        hi = (unsigned long long)(long long) static_cast<const int32_t*>(C.col[col])[i];
+    } else if (t == RJ_ATTR_UINT8) {
+        hi = static_cast<const uint8_t*>(C.col[col])[i];
+    } else {
+        hi = static_cast<const uint32_t*>(C.col[col])[i];
+    }
+}
+
+// The pixel formula shared by every kernel (and the host oracle).
+__device__ __forceinline__ int pixel_of(const DevCanvas& c, float x, float y)
+{
+    if (!(x >= c.minx && x < c.maxx && y >= c.miny && y < c.maxy)) return -1;
+    float fx = (x - c.minx) * c.sx;
+    float fy = (y - c.miny) * c.sy;
+    int ix = min((int)fx, c.W - 1);
+    int iy = min((int)fy, c.H - 1);
+    return iy * c.W + ix;
+}
+
+// CPUJoin getXIndex/getYIndex (GridIndexF.cpp:39-47) with the in-grid test of
+// getRegion (:163); (int) truncation done only on in-range values.
+__device__ __forceinline__ int grid_cell(const DevGrid& g, float x, float y)
+{
+    float fx = (x - g.minx) / g.width;
+    float fy = (y - g.miny) / g.height;
+    if (!(fx > -1.0f && fx < (float)g.xs && fy > -1.0f && fy < (float)g.ys)) return -1;
+    return (int)fy * g.xs + (int)fx;
+}
+
+// Crossing-number PIP, GridIndexF.cpp:383-402 (same operation order).
+__device__ __forceinline__ bool pip_f32(const float2* __restrict__ v, int n, float tx, float ty)
+{
+    bool c = false;
+    float2 vj = v[n - 1];
+    for (int i = 0; i < n; i++) {
+        float2 vi = v[i];
+        if (((vi.y > ty) != (vj.y > ty)) && (tx < (vj.x - vi.x) * (ty - vi.y) / (vj.y - vi.y) + vi.x))
+            c = !c;
+        vj = vi;
+    }
+    return c;
+}
+
+struct Runs {
+    int head;
+    bool tail;
+};
+
+// Runs of equal keys along the lanes of a wave.
+__device__ __forceinline__ Runs runs_of(int key)
+{
+    const int ln = __lane_id();
+    int prev = __shfl_up(key, 1, kWave);
+    int next = __shfl_down(key, 1, kWave);
+    bool h = (ln == 0) || prev != key;
+    bool t = (ln == kWave - 1) || next != key;
+    unsigned long long hm = __ballot(h);
+    unsigned long long upto = (ln == kWave - 1) ? ~0ull : ((2ull << ln) - 1ull);
+    Runs r;
+    r.head = kWave - 1 - __clzll(hm & upto);
+    r.tail = t;
+    return r;
+}
+
+// Inclusive sum within the run of each lane.
+template <class T>
+__device__ __forceinline__ T run_sum(T v, int head)
+{
+    const int ln = __lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        T u = __shfl_up(v, d, kWave);
+        if (ln - d >= head) v += u;
+    }
+    return v;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
+    return v;
+}
+
+// Block-wide sum of up to 4 values, then one atomic each from thread 0.
+__device__ __forceinline__ void block_add4(unsigned long long a0, unsigned long long a1,
+                                           unsigned long long a2, unsigned long long a3,
+                                           unsigned long long* d0, unsigned long long* d1,
+                                           unsigned long long* d2, unsigned long long* d3)
+{
+    __shared__ unsigned long long red[4][kBlock / kWave];
+    a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2); a3 = wave_sum(a3);
+    const int w = threadIdx.x / kWave;
+    if (__lane_id() == 0) {
+        red[0][w] = a0; red[1][w] = a1; red[2][w] = a2; red[3][w] = a3;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s[4] = {0, 0, 0, 0};
+        for (int k = 0; k < (int)(blockDim.x / kWave); k++)
+            for (int m = 0; m < 4; m++) s[m] += red[m][k];
+        if (d0 && s[0]) atomicAdd(d0, s[0]);
+        if (d1 && s[1]) atomicAdd(d1, s[1]);
+        if (d2 && s[2]) atomicAdd(d2, s[2]);
+        if (d3 && s[3]) atomicAdd(d3, s[3]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fill
+// ---------------------------------------------------------------------------
+
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// point splat into the canvas
+// ---------------------------------------------------------------------------
+
+template <bool AVG>
+__global__ __launch_bounds__(kBlock) void k_splat(const float2* __restrict__ xy, int64_t begin,
+                                                  int64_t end, int64_t per_block, DevCols C, DevQuery q,
+                                                  DevCanvas cv, uint32_t* __restrict__ cnt,
+                                                  unsigned long long* __restrict__ chi,
+                                                  unsigned long long* __restrict__ clo)
+{
+    const int64_t b0 = begin + (int64_t)blockIdx.x * per_block;
+    const int64_t b1 = min(end, b0 + per_block);
+    for (int64_t base = b0; base < b1; base += blockDim.x) {
+        const int64_t i = base + threadIdx.x;
+        int key = -1;
+        unsigned long long hi = 0, lo = 0;
+        if (i < b1) {
+            float2 p = xy[i];
+            if (pass_filters(C, q, i)) {
+                key = pixel_of(cv, p.x, p.y);
+                if (AVG && key >= 0) agg_value(C, q, i, hi, lo);
+            }
+        }
+        Runs r = runs_of(key);
+        unsigned c = run_sum<unsigned>(key >= 0 ? 1u : 0u, r.head);
+        if (AVG) {
+            hi = run_sum(hi, r.head);
+            lo = run_sum(lo, r.head);
+        }
+        if (r.tail && key >= 0) {
+            atomicAdd(&cnt[key], c);
+            if (AVG) {
+                atomicAdd(&chi[key], hi);
+                atomicAdd(&clo[key], lo);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// scan conversion of a band of one polygon
+// ---------------------------------------------------------------------------
+
+// Edge-flag toggles of the centers of cells of size cs (subunits) for rows
+// [R0, R0+NR) and columns from C0 (BW columns, WPR words per row).
+__device__ __forceinline__ void edge_toggles(int64_t ax, int64_t ay, int64_t bx, int64_t by,
+                                             int64_t cs, int R0, int NR, int C0, int BW, int WPR,
+                                             uint32_t* buf)
+{
+    if (ay == by) return;
+    if (ay > by) {
+        int64_t t = ax; ax = bx; bx = t;
+        t = ay; ay = by; by = t;
+    }
+    const int64_t h = cs / 2;
+    int64_t jlo = ceil_div(ay - h, cs), jhi = ceil_div(by - h, cs) - 1;
+    if (jlo < R0) jlo = R0;
+    if (jhi > R0 + NR - 1) jhi = R0 + NR - 1;
+    if (jlo > jhi) return;
+    const int64_t dy = by - ay, dx = bx - ax;
+    for (int64_t row = jlo; row <= jhi; row++) {
+        int64_t cy = row * cs + h;
+        int64_t N = ax * dy + dx * (cy - ay);
+        int64_t k = ceil_div(N - h * dy, cs * dy);
+        int64_t local = k - C0;
+        if (local < 0) local = 0;
+        if (local >= BW) continue;
+        atomicXor(&buf[(row - R0) * WPR + (local >> 5)], 1u << (local & 31));
+    }
+}
+
+// Pixels whose closed square meets segment ab, rows [TR0, TR0+TNR) of the
+// mask, columns [C0, C0+BW) (WPR words per row); canvas W x H.
+__device__ __forceinline__ void edge_touch(int64_t ax, int64_t ay, int64_t bx, int64_t by, int TR0,
+                                           int TNR, int C0, int BW, int WPR, int W, int H,
+                                           uint32_t* buf)
+{
+    if (ay > by) {
+        int64_t t = ax; ax = bx; bx = t;
+        t = ay; ay = by; by = t;
+    }
+    int64_t jlo = ceil_div(ay, kSub) - 1, jhi = floor_div(by, kSub);
+    if (jlo < 0) jlo = 0;
+    if (jhi > H - 1) jhi = H - 1;
+    if (jlo < TR0) jlo = TR0;
+    if (jhi > TR0 + TNR - 1) jhi = TR0 + TNR - 1;
+    for (int64_t j = jlo; j <= jhi; j++) {
+        int64_t xlo, xhi;
+        if (ay == by) {
+            xlo = min(ax, bx);
+            xhi = max(ax, bx);
+        } else {
+            int64_t ylo = max(ay, j * kSub), yhi = min(by, (j + 1) * kSub);
+            int64_t den = by - ay;
+            int64_t n0 = ax * den + (bx - ax) * (ylo - ay);
+            int64_t n1 = ax * den + (bx - ax) * (yhi - ay);
+            int64_t f0 = floor_div(n0, den), f1 = floor_div(n1, den);
+            int64_t g0 = ceil_div(n0, den), g1 = ceil_div(n1, den);
+            xlo = min(f0, f1);
+            xhi = max(g0, g1);
+        }
+        int64_t ilo = ceil_div(xlo, kSub) - 1, ihi = floor_div(xhi, kSub);
+        if (ilo < 0) ilo = 0;
+        if (ihi > W - 1) ihi = W - 1;
+        if (ilo < C0) ilo = C0;
+        if (ihi > C0 + BW - 1) ihi = C0 + BW - 1;
+        for (int64_t i = ilo; i <= ihi; i++) {
+            int64_t l = i - C0;
+            atomicOr(&buf[(j - TR0) * WPR + (l >> 5)], 1u << (l & 31));
+        }
+    }
+}
+
+__device__ __forceinline__ void prefix_xor_row(uint32_t* w, int nw)
+{
+    uint32_t carry = 0;
+    for (int k = 0; k < nw; k++) {
+        uint32_t x = w[k];
+        x ^= x << 1;
+        x ^= x << 2;
+        x ^= x << 4;
+        x ^= x << 8;
+        x ^= x << 16;
+        x ^= carry;
+        w[k] = x;
+        carry = (x >> 31) ? 0xFFFFFFFFu : 0u;
+    }
+}
+
+__device__ __forceinline__ int bit_at(const uint32_t* row, int col)
+{
+    return (row[col >> 5] >> (col & 31)) & 1;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_band(const Band* __restrict__ bands, DevPolys P, DevCanvas cv,
+                                                 const uint32_t* __restrict__ cnt,
+                                                 const unsigned long long* __restrict__ chi,
+                                                 const unsigned long long* __restrict__ clo, DevResult res,
+                                                 unsigned long long* __restrict__ bounds, uint32_t* label,
+                                                 unsigned int* conflicts, int with_sums)
+{
+    extern __shared__ uint32_t lds[];
+    const Band bd = bands[blockIdx.x];
+    const int p = bd.poly;
+    const int c0 = bd.c0, c1 = bd.c1;
+    const int r0 = bd.r0, nr = bd.nrows, wpr = bd.wpr;
+    const int bw = c1 - c0 + 1;
+    uint32_t* tog = lds;
+    uint32_t* tch = tog + nr * wpr;
+    uint32_t* sub = tch + (nr + 2) * wpr;
+    const int nw = (MODE == BAND_BOUNDS) ? (nr * wpr + (nr + 2) * wpr + 16 * nr * wpr) : nr * wpr;
+    for (int k = threadIdx.x; k < nw; k += blockDim.x) lds[k] = 0;
+    __syncthreads();
+
+    const int e0 = P.off[p], e1 = P.off[p + 1], n = e1 - e0;
+    const int64_t band_ylo = (int64_t)(r0 - 1) * kSub, band_yhi = (int64_t)(r0 + nr + 1) * kSub;
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const int i = e0 + e;
+        const int j = (e == 0) ? e1 - 1 : i - 1;
+        const int64_t ax = P.X[i], ay = P.Y[i], bx = P.X[j], by = P.Y[j];
+        if (max(ay, by) < band_ylo || min(ay, by) > band_yhi) continue;
+        edge_toggles(ax, ay, bx, by, kSub, r0, nr, c0, bw, wpr, tog);
+        if (MODE == BAND_BOUNDS) {
+            edge_toggles(ax, ay, bx, by, kSub / 4, 4 * r0, 4 * nr, 4 * c0, 4 * bw, 4 * wpr, sub);
+            edge_touch(ax, ay, bx, by, r0 - 1, nr + 2, c0, bw, wpr, cv.W, cv.H, tch);
+        }
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < nr; r += blockDim.x) prefix_xor_row(tog + r * wpr, wpr);
+    if (MODE == BAND_BOUNDS)
+        for (int r = threadIdx.x; r < 4 * nr; r += blockDim.x) prefix_xor_row(sub + r * 4 * wpr, 4 * wpr);
+    __syncthreads();
+
+    unsigned long long s_cnt = 0, s_hi = 0, s_lo = 0;
+    unsigned long long s_fn = 0, s_fp = 0, s_fn1 = 0, s_fp1 = 0;
+    const int total = nr * bw;
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
+        const int row = t / bw, col = t - row * bw;
+        const int inside = bit_at(tog + row * wpr, col);
+        const int64_t pix = (int64_t)(r0 + row) * cv.W + (c0 + col);
+        if (MODE == BAND_SUM) {
+            if (inside) {
+                s_cnt += cnt[pix];
+                if (with_sums) {
+                    s_hi += chi[pix];
+                    s_lo += clo[pix];
+                }
+            }
+        } else if (MODE == BAND_LABEL) {
+            if (inside) {
+                uint32_t old = atomicCAS(&label[pix], kLabelNone, (uint32_t)p);
+                if (old != kLabelNone) {
+                    atomicExch(&label[pix], kLabelBorder);
+                    atomicAdd(conflicts, 1u);
+                }
+            }
+        } else {
+            const uint32_t val = cnt[pix];
+            if (inside) s_cnt += val;
+            if (val != 0) {
+                int tb = 0;
+                for (int dr = 0; dr < 3 && !tb; dr++) {
+                    const uint32_t* trow = tch + (row + dr) * wpr;
+                    tb |= bit_at(trow, col);
+                    if (col > 0) tb |= bit_at(trow, col - 1);
+                    if (col + 1 < bw) tb |= bit_at(trow, col + 1);
+                }
+                if (tb) {
+                    int k16 = 0;
+                    const int bit = (4 * col) & 31, word = (4 * col) >> 5;
+                    for (int sy = 0; sy < 4; sy++)
+                        k16 += __popc((sub[(4 * row + sy) * (4 * wpr) + word] >> bit) & 0xFu);
+                    const int m = min(k16, 16 - k16);
+                    const unsigned long long add = ((unsigned long long)m * val + 15ull) / 16ull;
+                    if (inside) {
+                        s_fp += add;
+                        s_fp1 += val;
+                    } else {
+                        s_fn += add;
+                        s_fn1 += val;
+                    }
+                }
+            }
+        }
+    }
+    if (MODE == BAND_SUM) {
+        block_add4(s_cnt, s_hi, s_lo, 0, res.cnt + p, (unsigned long long*)res.hi + p,
+                   res.lo + p, nullptr);
+    } else if (MODE == BAND_BOUNDS) {
+        block_add4(s_cnt, 0, 0, 0, res.cnt + p, nullptr, nullptr, nullptr);
+        __syncthreads();
+        block_add4(s_fn, s_fp, s_fn1, s_fp1, bounds + p, bounds + P.np + p, bounds + 2 * P.np + p,
+                   bounds + 3 * P.np + p);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// border mask for the hybrid join
+// ---------------------------------------------------------------------------
+
+// One thread per polygon edge (vertex i and its predecessor prev[i]).
+__global__ void k_touch(const int32_t* __restrict__ X, const int32_t* __restrict__ Y,
+                        const int32_t* __restrict__ prev, int64_t nverts, int W, int H, int wpr,
+                        uint32_t* mask)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nverts) return;
+    int64_t j = prev[i];
+    edge_touch(X[i], Y[i], X[j], Y[j], 0, H, 0, W, wpr, W, H, mask);
+}
+
+// Inclusive pixel rectangles (i0, i1, j0, j1) set in the mask.
+__global__ void k_mark_rects(const int32_t* __restrict__ rects, int nrects, int W, int H, int wpr,
+                             uint32_t* mask)
+{
+    int r = blockIdx.x;
+    if (r >= nrects) return;
+    int i0 = max(0, rects[4 * r]), i1 = min(W - 1, rects[4 * r + 1]);
+    int j0 = max(0, rects[4 * r + 2]), j1 = min(H - 1, rects[4 * r + 3]);
+    int w = i1 - i0 + 1;
+    if (w <= 0 || j1 < j0) return;
+    for (int t = threadIdx.x; t < w * (j1 - j0 + 1); t += blockDim.x) {
+        int j = j0 + t / w, i = i0 + t % w;
+        atomicOr(&mask[(int64_t)j * wpr + (i >> 5)], 1u << (i & 31));
+    }
+}
+
+// Dilate the mask by r pixels (Chebyshev) and mark the result in the labels.
+__global__ void k_dilate_to_label(const uint32_t* __restrict__ mask, int W, int H, int wpr, int r,
+                                  uint32_t* label)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)wpr * H) return;
+    const int row = (int)(t / wpr), w = (int)(t % wpr);
+    uint32_t m = 0, ml = 0, mr = 0;
+    for (int dr = -r; dr <= r; dr++) {
+        int rr = row + dr;
+        if (rr < 0 || rr >= H) continue;
+        const uint32_t* mrow = mask + (int64_t)rr * wpr;
+        m |= mrow[w];
+        if (w > 0) ml |= mrow[w - 1];
+        if (w + 1 < wpr) mr |= mrow[w + 1];
+    }
+    uint32_t d = m;
+    for (int s = 1; s <= r; s++) {
+        d |= (m << s) | (ml >> (32 - s));
+        d |= (m >> s) | (mr << (32 - s));
+    }
+    if (!d) return;
+    uint32_t* lrow = label + (int64_t)row * W;
+    for (int b = 0; b < 32; b++) {
+        int col = w * 32 + b;
+        if (col >= W) break;
+        if ((d >> b) & 1u) lrow[col] = kLabelBorder;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// point pass: index join, label-map raster join, hybrid join
+// ---------------------------------------------------------------------------
+
+struct Acc {
+    unsigned long long* cnt;  // LDS (lds_np entries) or null
+    unsigned long long* hi;
+    unsigned long long* lo;
+    DevResult g;
+    int lds_np;
+};
+
+__device__ __forceinline__ void acc_add(const Acc& A, int key, unsigned long long c, unsigned long long hi,
+                                        unsigned long long lo, bool avg)
+{
+    if (key < A.lds_np) {
+        atomicAdd(&A.cnt[key], c);
+        if (avg) {
+            atomicAdd(&A.hi[key], hi);
+            atomicAdd(&A.lo[key], lo);
+        }
+    } else {
+        atomicAdd(&A.g.cnt[key], c);
+        if (avg) {
+            atomicAdd((unsigned long long*)&A.g.hi[key], hi);
+            atomicAdd(&A.g.lo[key], lo);
+        }
+    }
+}
+
+// CPUJoin getRegion over the cell's candidates. Returns the first polygon that
+// contains the point; further containing polygons are added directly.
+template <bool AVG>
+__device__ __forceinline__ int exact_first(const DevGrid& g, const DevPolys& P, const Acc& A, float x,
+                                           float y, unsigned long long hi, unsigned long long lo)
+{
+    const int cell = grid_cell(g, x, y);
+    if (cell < 0) return -1;
+    int first = -1;
+    const int k1 = g.cell_off[cell + 1];
+    for (int k = g.cell_off[cell]; k < k1; k++) {
+        const int p = g.cand[k];
+        const int v0 = P.off[p];
+        if (pip_f32(reinterpret_cast<const float2*>(P.xy) + v0, P.off[p + 1] - v0, x, y)) {
+            if (first < 0) first = p;
+            else acc_add(A, p, 1ull, hi, lo, AVG);
+        }
+    }
+    return first;
+}
+
+template <int MODE, bool AVG>
+__global__ __launch_bounds__(kBlock) void k_points(const float2* __restrict__ xy, int64_t begin,
+                                                   int64_t end, int64_t per_block, DevCols C, DevQuery q,
+                                                   DevCanvas cv, DevGrid g, DevPolys P,
+                                                   const uint32_t* __restrict__ label, DevResult res,
+                                                   int lds_np)
+{
+    extern __shared__ unsigned long long sacc[];
+    Acc A;
+    A.lds_np = lds_np;
+    A.cnt = sacc;
+    A.hi = sacc + lds_np;
+    A.lo = sacc + 2 * lds_np;
+    A.g = res;
+    const int nacc = AVG ? 3 * lds_np : lds_np;
+    for (int k = threadIdx.x; k < nacc; k += blockDim.x) sacc[k] = 0;
+    __syncthreads();
+
+    const int64_t b0 = begin + (int64_t)blockIdx.x * per_block;
+    const int64_t b1 = min(end, b0 + per_block);
+    for (int64_t base = b0; base < b1; base += blockDim.x) {
+        const int64_t i = base + threadIdx.x;
+        int key = -1;
+        unsigned long long hi = 0, lo = 0;
+        if (i < b1 && pass_filters(C, q, i)) {
+            const float2 pt = xy[i];
+            if (AVG) agg_value(C, q, i, hi, lo);
+            if (MODE == PT_INDEX) {
+                key = exact_first<AVG>(g, P, A, pt.x, pt.y, hi, lo);
+            } else {
+                const int pix = pixel_of(cv, pt.x, pt.y);
+                if (pix >= 0) {
+                    const uint32_t lab = label[pix];
+                    if (lab < kLabelMaxId) key = (int)lab;
+                    else if (MODE == PT_LABEL_HYBRID && lab == kLabelBorder)
+                        key = exact_first<AVG>(g, P, A, pt.x, pt.y, hi, lo);
+                }
+            }
+        }
+        Runs r = runs_of(key);
+        unsigned long long c = run_sum<unsigned long long>(key >= 0 ? 1ull : 0ull, r.head);
+        if (AVG) {
+            hi = run_sum(key >= 0 ? hi : 0ull, r.head);
+            lo = run_sum(key >= 0 ? lo : 0ull, r.head);
+        }
+        if (r.tail && key >= 0) acc_add(A, key, c, hi, lo, AVG);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < lds_np; k += blockDim.x) {
+        unsigned long long c = sacc[k];
+        if (c) {
+            atomicAdd(&res.cnt[k], c);
+            if (AVG) {
+                atomicAdd((unsigned long long*)&res.hi[k], sacc[lds_np + k]);
+                atomicAdd(&res.lo[k], sacc[2 * lds_np + k]);
+            }
+        }
+    }
+}
+
+inline int blocks_for(int64_t n, int64_t per, int cap)
+{
+    int64_t b = (n + per - 1) / per;
+    if (b < 1) b = 1;
+    return (int)(b > cap ? cap : b);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v)
+{
+    if (n == 0) return;
+    int blocks = (int)std::min<size_t>((n + kBlock - 1) / kBlock, 8192);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(kBlock), 0, s, p, n, v);
+}
+
+// Contiguous row chunks per block (a multiple of the block size) so that a
+// wave reads consecutive rows and spatially ordered tables stay local.
+static void chunking(int64_t n, int cap, int* blocks, int64_t* per_block)
+{
+    int b = blocks_for(n, kBlock, cap);
+    int64_t per = (n + b - 1) / b;
+    per = (per + kBlock - 1) / kBlock * kBlock;
+    if (per < kBlock) per = kBlock;
+    *blocks = (int)std::max<int64_t>(1, (n + per - 1) / per);
+    *per_block = per;
+}
+
+void launch_splat(hipStream_t s, const float* xy, int64_t begin, int64_t end, const DevCols& cols,
+                  const DevQuery& q, const DevCanvas& c, uint32_t* cnt, long long* chi,
+                  unsigned long long* clo)
+{
+    int64_t n = end - begin;
+    if (n <= 0) return;
+    int blocks;
+    int64_t per;
+    chunking(n, 16384, &blocks, &per);
+    if (q.agg_type != RJ_AGG_COUNT)
+        hipLaunchKernelGGL(k_splat<true>, dim3(blocks), dim3(kBlock), 0, s, (const float2*)xy, begin,
+                           end, per, cols, q, c, cnt, (unsigned long long*)chi, clo);
+    else
+        hipLaunchKernelGGL(k_splat<false>, dim3(blocks), dim3(kBlock), 0, s, (const float2*)xy, begin,
+                           end, per, cols, q, c, cnt, (unsigned long long*)chi, clo);
+}
+
+void launch_band(hipStream_t s, int mode, const Band* bands, int nbands, const DevPolys& P,
+                 const DevCanvas& c, const uint32_t* cnt, const long long* chi,
+                 const unsigned long long* clo, DevResult res, long long* bounds, uint32_t* label,
+                 unsigned int* conflicts, size_t lds_bytes, int with_sums)
+{
+    if (nbands <= 0) return;
+    const unsigned long long* hi = (const unsigned long long*)chi;
+    unsigned long long* bd = (unsigned long long*)bounds;
+    if (mode == BAND_SUM)
+        hipLaunchKernelGGL(k_band<BAND_SUM>, dim3(nbands), dim3(kBlock), lds_bytes, s, bands, P, c, cnt,
+                           hi, clo, res, bd, label, conflicts, with_sums);
+    else if (mode == BAND_LABEL)
+        hipLaunchKernelGGL(k_band<BAND_LABEL>, dim3(nbands), dim3(kBlock), lds_bytes, s, bands, P, c,
+                           cnt, hi, clo, res, bd, label, conflicts, with_sums);
+    else
+        hipLaunchKernelGGL(k_band<BAND_BOUNDS>, dim3(nbands), dim3(kBlock), lds_bytes, s, bands, P, c,
+                           cnt, hi, clo, res, bd, label, conflicts, with_sums);
+}
+
+void launch_touch(hipStream_t s, const int32_t* X, const int32_t* Y, const int32_t* prev,
+                  int64_t nverts, const DevCanvas& c, uint32_t* mask_bits)
+{
+    if (nverts <= 0) return;
+    int wpr = (c.W + 31) / 32;
+    int blocks = (int)((nverts + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_touch, dim3(blocks), dim3(kBlock), 0, s, X, Y, prev, nverts, c.W, c.H, wpr,
+                       mask_bits);
+}
+
+void launch_mark_rects(hipStream_t s, const int32_t* rects, int nrects, const DevCanvas& c,
+                       uint32_t* mask_bits)
+{
+    if (nrects <= 0) return;
+    int wpr = (c.W + 31) / 32;
+    hipLaunchKernelGGL(k_mark_rects, dim3(nrects), dim3(kBlock), 0, s, rects, nrects, c.W, c.H, wpr,
+                       mask_bits);
+}
+
+void launch_dilate_to_label(hipStream_t s, const uint32_t* mask_bits, const DevCanvas& c, int r,
+                            uint32_t* label)
+{
+    int wpr = (c.W + 31) / 32;
+    int64_t n = (int64_t)wpr * c.H;
+    int blocks = (int)((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_dilate_to_label, dim3(blocks), dim3(kBlock), 0, s, mask_bits, c.W, c.H, wpr, r,
+                       label);
+}
+
+void launch_points(hipStream_t s, int mode, const float* xy, int64_t begin, int64_t end,
+                   const DevCols& cols, const DevQuery& q, const DevCanvas& c, const DevGrid& g,
+                   const DevPolys& P, const uint32_t* label, DevResult res, int lds_np)
+{
+    int64_t n = end - begin;
+    if (n <= 0) return;
+    int blocks;
+    int64_t per;
+    chunking(n, 4096, &blocks, &per);
+    const bool avg = q.agg_type != RJ_AGG_COUNT;
+    size_t lds = (size_t)(avg ? 3 : 1) * lds_np * sizeof(unsigned long long);
+    const float2* p = (const float2*)xy;
+#define RJ_LAUNCH_PTS(M, A)                                                                             \
+    hipLaunchKernelGGL((k_points<M, A>), dim3(blocks), dim3(kBlock), lds, s, p, begin, end, per, cols, q, \
+                       c, g, P, label, res, lds_np)
+    if (mode == PT_INDEX) {
+        if (avg) RJ_LAUNCH_PTS(PT_INDEX, true);
+        else RJ_LAUNCH_PTS(PT_INDEX, false);
+    } else if (mode == PT_LABEL) {
+        if (avg) RJ_LAUNCH_PTS(PT_LABEL, true);
+        else RJ_LAUNCH_PTS(PT_LABEL, false);
+    } else {
+        if (avg) RJ_LAUNCH_PTS(PT_LABEL_HYBRID, true);
+        else RJ_LAUNCH_PTS(PT_LABEL_HYBRID, false);
+    }
+#undef RJ_LAUNCH_PTS
+}
+
+}  // namespace rj

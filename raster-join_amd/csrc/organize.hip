@@ -1,0 +1,100 @@
+// organize.hip -- spatial organization of a resident point table: rows are
+// reordered by the Morton key of their location (16 bits per axis over the
+// table's bbox), so the rows one workgroup reads are close on the canvas and
+// the splat / label gather stay in cache. Join results do not depend on the
+// row order.
+#include <hipcub/hipcub.hpp>
+
+#include "rj_dev.hpp"
+
+namespace rj {
+namespace {
+
+__device__ __forceinline__ uint32_t spread16(uint32_t v)
+{
+    v &= 0xFFFFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+__global__ void k_morton(const float2* __restrict__ xy, int64_t n, float minx, float miny, float sx,
+                         float sy, uint32_t* keys, uint32_t* idx)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float2 p = xy[i];
+    float fx = (p.x - minx) * sx, fy = (p.y - miny) * sy;
+    fx = fminf(fmaxf(fx, 0.0f), 65535.0f);
+    fy = fminf(fmaxf(fy, 0.0f), 65535.0f);
+    if (!(fx == fx)) fx = 0.0f;
+    if (!(fy == fy)) fy = 0.0f;
+    keys[i] = spread16((uint32_t)fx) | (spread16((uint32_t)fy) << 1);
+    idx[i] = (uint32_t)i;
+}
+
+template <class T>
+__global__ void k_gather(const T* __restrict__ src, const uint32_t* __restrict__ perm, int64_t n,
+                         T* __restrict__ dst)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[perm[i]];
+}
+
+}  // namespace
+
+// Sorts the rows of xy (+ columns) by Morton key in place. scratch must hold
+// at least organize_scratch_bytes(n) bytes. Returns a hipError_t.
+size_t organize_scratch_bytes(int64_t n)
+{
+    size_t temp = 0;
+    hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                       (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 32);
+    size_t kv = (size_t)n * 4 * 4;       // keys in/out + values in/out
+    size_t col = (size_t)n * 8;          // the widest column (xy)
+    return kv + col + temp + 1024;
+}
+
+int organize_rows(hipStream_t s, int64_t n, float* xy, void** cols, const int32_t* types, int ncols,
+                  const float bbox[4], void* scratch)
+{
+    if (n <= 1) return 0;
+    char* base = (char*)scratch;
+    uint32_t* k_in = (uint32_t*)base;
+    uint32_t* k_out = k_in + n;
+    uint32_t* v_in = k_out + n;
+    uint32_t* v_out = v_in + n;
+    char* colbuf = (char*)(v_out + n);
+    char* temp = colbuf + (size_t)n * 8;
+    size_t temp_bytes = 0;
+    hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, k_in, k_out, v_in, v_out, (int)n, 0, 32, s);
+    const float sx = 65535.0f / fmaxf(bbox[2] - bbox[0], 1e-30f);
+    const float sy = 65535.0f / fmaxf(bbox[3] - bbox[1], 1e-30f);
+    const int blocks = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(k_morton, dim3(blocks), dim3(256), 0, s, (const float2*)xy, n, bbox[0], bbox[1], sx,
+                       sy, k_in, v_in);
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, k_in, k_out, v_in, v_out, (int)n, 0,
+                                                      32, s);
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(k_gather<float2>, dim3(blocks), dim3(256), 0, s, (const float2*)xy, v_out, n,
+                       (float2*)colbuf);
+    e = hipMemcpyAsync(xy, colbuf, (size_t)n * 8, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return (int)e;
+    for (int c = 0; c < ncols; c++) {
+        if (types[c] == RJ_ATTR_UINT8) {
+            hipLaunchKernelGGL(k_gather<uint8_t>, dim3(blocks), dim3(256), 0, s, (const uint8_t*)cols[c],
+                               v_out, n, (uint8_t*)colbuf);
+            e = hipMemcpyAsync(cols[c], colbuf, (size_t)n, hipMemcpyDeviceToDevice, s);
+        } else {
+            hipLaunchKernelGGL(k_gather<uint32_t>, dim3(blocks), dim3(256), 0, s, (const uint32_t*)cols[c],
+                               v_out, n, (uint32_t*)colbuf);
+            e = hipMemcpyAsync(cols[c], colbuf, (size_t)n * 4, hipMemcpyDeviceToDevice, s);
+        }
+        if (e != hipSuccess) return (int)e;
+    }
+    return (int)hipGetLastError();
+}
+
+}  // namespace rj

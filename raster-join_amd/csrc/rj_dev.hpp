@@ -1,0 +1,109 @@
+// rj_dev.hpp -- plain argument structs shared by the host engine and the HIP
+// kernels, and the kernel launch entry points (defined in kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "rasterjoin.h"
+
+namespace rj {
+
+constexpr int kSub = 256;          // subpixel units per pixel of the fixed-point canvas grid
+constexpr int kSubShift = 8;
+constexpr uint32_t kLabelNone = 0xFFFFFFFFu;    // pixel center in no polygon
+constexpr uint32_t kLabelBorder = 0xFFFFFFFEu;  // pixel handled by the exact test
+constexpr uint32_t kLabelMaxId = 0xFFFFFFF0u;
+
+// Columns of a point table, as seen by a kernel.
+struct DevCols {
+    const void* col[RJ_MAX_COLUMNS];
+    int32_t type[RJ_MAX_COLUMNS];
+};
+
+// Filters and aggregate of one query.
+struct DevQuery {
+    int32_t ncons;
+    int32_t agg_type;  // RJ_AGG_*
+    int32_t agg_col;
+    int32_t pad;
+    int32_t con_col[RJ_MAX_CONSTRAINTS];
+    int32_t con_op[RJ_MAX_CONSTRAINTS];
+    uint32_t con_val[RJ_MAX_CONSTRAINTS];
+};
+
+// Canvas geometry (same numbers as the host plan).
+struct DevCanvas {
+    int32_t W, H;
+    float minx, miny, maxx, maxy;
+    float sx, sy;
+};
+
+// CPUJoin grid index on the device (cell = y*xs + x).
+struct DevGrid {
+    int32_t xs, ys;
+    float minx, miny, width, height;
+    const int32_t* cell_off;
+    const int32_t* cand;
+};
+
+// Polygon collection on the device.
+struct DevPolys {
+    int32_t np;
+    const float* xy;       // float vertices (CSR), 2 per vertex
+    const int32_t* off;    // np+1
+    const int32_t* X;      // snapped canvas coordinates (subpixel units)
+    const int32_t* Y;
+    const int32_t* region; // 4 per polygon: c0, c1, r0, r1 (inclusive pixel bbox)
+};
+
+// Per-polygon accumulators.
+struct DevResult {
+    unsigned long long* cnt;
+    long long* hi;
+    unsigned long long* lo;
+};
+
+// One band of rows of one polygon for the scan-conversion kernels.
+struct Band {
+    int32_t poly;
+    int32_t r0;      // first row
+    int32_t nrows;
+    int32_t wpr;     // 32-bit words per row of the band
+    int32_t c0, c1;  // first and last column (inclusive)
+};
+
+enum BandMode { BAND_SUM = 0, BAND_LABEL = 1, BAND_BOUNDS = 2 };
+enum PointMode { PT_INDEX = 0, PT_LABEL = 1, PT_LABEL_HYBRID = 2 };
+
+// LDS bytes a band of `nrows` x `wpr` words needs in each mode.
+inline size_t band_lds_bytes(int mode, int nrows, int wpr)
+{
+    size_t w = (size_t)nrows * wpr;
+    if (mode == BAND_BOUNDS) {
+        size_t wpr4 = (size_t)4 * wpr;
+        return 4 * (w + (size_t)(nrows + 2) * wpr + (size_t)4 * nrows * wpr4);
+    }
+    return 4 * w;
+}
+
+// ---- launchers (kernels.hip) ----
+void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);
+void launch_splat(hipStream_t s, const float* xy, int64_t begin, int64_t end, const DevCols& cols,
+                  const DevQuery& q, const DevCanvas& c, uint32_t* cnt, long long* chi,
+                  unsigned long long* clo);
+void launch_band(hipStream_t s, int mode, const Band* bands, int nbands, const DevPolys& P,
+                 const DevCanvas& c, const uint32_t* cnt, const long long* chi,
+                 const unsigned long long* clo, DevResult res, long long* bounds, uint32_t* label,
+                 unsigned int* conflicts, size_t lds_bytes, int with_sums);
+void launch_touch(hipStream_t s, const int32_t* X, const int32_t* Y, const int32_t* prev,
+                  int64_t nverts, const DevCanvas& c, uint32_t* mask_bits);
+void launch_mark_rects(hipStream_t s, const int32_t* rects, int nrects, const DevCanvas& c,
+                       uint32_t* mask_bits);
+void launch_dilate_to_label(hipStream_t s, const uint32_t* mask_bits, const DevCanvas& c, int r,
+                            uint32_t* label);
+void launch_points(hipStream_t s, int mode, const float* xy, int64_t begin, int64_t end,
+                   const DevCols& cols, const DevQuery& q, const DevCanvas& c, const DevGrid& g,
+                   const DevPolys& P, const uint32_t* label, DevResult res, int lds_np);
+
+}  // namespace rj

@@ -1,0 +1,70 @@
+// rj_impl.hpp -- host-side internals: polygon store, canvas plan, polygon
+// scan-conversion bands and the CPUJoin grid index.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "rasterjoin.h"
+#include "rj_dev.hpp"
+
+namespace rj {
+
+// A polygon collection in CSR form (PolyHandler::polys / pindexes,
+// db/PolyHandler.cpp:49-75) with its bbox.
+struct PolySet {
+    int32_t np = 0;
+    std::vector<float> xy;
+    std::vector<int32_t> off;
+    double bbox[4] = {0, 0, 0, 0};  // minx, miny, maxx, maxy (double of the float bounds)
+    float bboxf[4] = {0, 0, 0, 0};
+};
+
+// Canvas geometry for a bbox and accuracy (GLFunction.cpp:26-30,
+// RasterJoin.cpp:51-78 with a maximum tile side of 8192).
+struct CanvasPlan {
+    int32_t W = 0, H = 0, splitx = 0, splity = 0;
+    double minx = 0, miny = 0, maxx = 0, maxy = 0;
+    float minx_f = 0, miny_f = 0, maxx_f = 0, maxy_f = 0;
+    float sx_f = 0, sy_f = 0;
+    double fxs = 0, fys = 0;  // fixed-point scale: W*256/(maxx-minx)
+    double cell = 0;
+};
+int make_plan(const double bbox[4], double eps_m, double ground_res, CanvasPlan* p);
+DevCanvas dev_canvas(const CanvasPlan& p);
+
+// Polygons on one canvas: snapped vertices, per-polygon pixel regions and the
+// bands the scan-conversion kernels run on.
+struct PolyRaster {
+    std::vector<int32_t> X, Y;        // snapped canvas coordinates (1/256 pixel)
+    std::vector<int32_t> prev;        // predecessor vertex of every vertex (edge list)
+    std::vector<int32_t> region;      // c0, c1, r0, r1 per polygon (inclusive)
+    std::vector<Band> bands_sum;      // BAND_SUM / BAND_LABEL
+    std::vector<Band> bands_bounds;   // BAND_BOUNDS (full region width per band)
+    size_t lds_sum = 0, lds_bounds = 0;
+    bool bounds_ok = true;            // every region fits a BAND_BOUNDS band
+};
+void rasterize_polys(const PolySet& ps, const CanvasPlan& plan, PolyRaster* out);
+
+// CPUJoin grid index (CPUJoin/GridIndexF.cpp:60-155): cell = y*xs + x.
+// `deficient` lists cells whose candidate list misses a polygon that meets
+// the cell (possible through the reference's cell test); the hybrid join
+// treats their pixels as border pixels so its result stays CPUJoin's.
+struct GridIndex {
+    int32_t xs = 0, ys = 0;
+    float minx = 0, miny = 0, width = 0, height = 0;
+    std::vector<int32_t> cell_off;
+    std::vector<int32_t> cand;
+    std::vector<int32_t> deficient;
+};
+void build_grid(const PolySet& ps, int res, GridIndex* g);
+// Pixel rectangles (i0, i1, j0, j1) covering the deficient cells, one pixel margin.
+void deficient_rects(const GridIndex& g, const CanvasPlan& plan, std::vector<int32_t>* rects);
+
+// Fixed-point aggregate value decoding: sum = hi + lo * 2^-32.
+inline double fixed_to_double(int64_t hi, uint64_t lo)
+{
+    return (double)hi + (double)lo * (1.0 / 4294967296.0);
+}
+
+}  // namespace rj

@@ -1,0 +1,311 @@
+"""Python binding of librasterjoin.so (the C ABI in include/rasterjoin.h).
+
+`Handler` mirrors the reference's join API, the GLHandler singleton
+(RasterJoin/join/GLHandler.hpp:20-78): set the data, the accuracy and the
+polygon index resolution, then `execute_function(fn)` for one of the four
+FunctionType values, `get_error_bounds(fn)` and `print_time_stats(fn)`.
+All joins run on the GPU through the library; there is no CPU fallback, and
+loading fails loudly if the library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "lib", "librasterjoin.so")
+
+# FunctionType (GLHandler.hpp:23-29)
+RASTER, INDEX, HYBRID, ERRORBOUNDS = 0, 1, 2, 3
+FUNCTIONS = {"raster": RASTER, "index": INDEX, "hybrid": HYBRID, "errorbounds": ERRORBOUNDS}
+# AttributeType (Common.h:20) + u8 storage
+UINT, INT, FLOAT, UINT8 = 1, 2, 3, 4
+# ConstraintType (Common.h:22)
+EQ, LT, LTE, GT, GTE = 0, 1, 2, 3, 4
+# Aggregation (Common.h:26)
+COUNT, AVG, SUM = 0, 1, 2
+# options
+OPT_RASTER_PATH, OPT_TIMING, OPT_HYBRID_DILATION, OPT_CACHE, OPT_ORGANIZE = 1, 2, 3, 4, 5
+
+ERRORS = {0: "ok", -1: "bad argument", -2: "bad state", -3: "out of memory",
+          -4: "device error", -5: "limit exceeded", -6: "unsupported"}
+
+_lib = None
+
+
+class RJError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)} ({code}): {msg}")
+        self.code = code
+
+
+class Constraint(C.Structure):
+    """QueryConstraint (Common.h:34-39): 16 bytes."""
+    _fields_ = [("attrib_id", C.c_int32), ("type", C.c_int32), ("val", C.c_uint32),
+                ("dummy", C.c_int32)]
+
+
+class PointParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("box", C.c_double), ("ncenters", C.c_int32),
+                ("frac_cluster", C.c_double), ("sigma_lo", C.c_double), ("sigma_hi", C.c_double),
+                ("t0", C.c_uint32), ("tspan", C.c_uint32), ("fare_mu", C.c_double),
+                ("fare_sigma", C.c_double)]
+
+
+class VoronoiParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("nsites", C.c_int32), ("box", C.c_double),
+                ("spacing_lo", C.c_double), ("spacing_hi", C.c_double), ("jitter", C.c_double)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C raster-join_amd`")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64, f64 = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+        sig = {
+            "rj_create": [i64, i32, vp], "rj_destroy": [vp], "rj_set_polygons": [vp, vp, vp, i32, vp],
+            "rj_set_points": [vp, i64, vp, i32, vp, vp], "rj_set_constraints": [vp, vp, i32],
+            "rj_set_aggregation": [vp, i32, i32], "rj_set_accuracy": [vp, f64, f64],
+            "rj_set_index_res": [vp, i32, i32], "rj_execute": [vp, i32, vp, vp, vp],
+            "rj_get_sums_fixed": [vp, vp, vp], "rj_error_bounds": [vp, vp],
+            "rj_time_stats": [vp, i32, vp, C.c_size_t], "rj_canvas_info": [vp, vp, vp],
+            "rj_set_option": [vp, i32, i64], "rj_kernel_times": [vp, i32, vp, C.c_size_t, vp, vp],
+            "rj_sync": [vp], "rj_table_create": [vp, i64, vp, i32, vp, vp, i32, vp],
+            "rj_table_rows": [vp, vp], "rj_table_read": [vp, i64, i64, vp, vp],
+            "rj_table_destroy": [vp], "rj_bind_table": [vp, vp, i64, i64],
+            "rj_gen_points": [vp, i64, i64, vp, vp, vp, vp], "rj_gen_voronoi": [vp, vp],
+            "rj_polys_size": [vp, vp, vp], "rj_polys_copy": [vp, vp, vp],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        L.rj_last_error.argtypes = [vp]
+        L.rj_last_error.restype = C.c_char_p
+        L.rj_polys_free.argtypes = [vp]
+        L.rj_polys_free.restype = None
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+# ---------------------------------------------------------------------------
+# synthetic inputs (rasterjoin_datagen.h)
+# ---------------------------------------------------------------------------
+
+def point_params(seed=7, box=50000.0, ncenters=40, frac_cluster=0.7, sigma=(200.0, 2000.0),
+                 t0=1230768000, tspan=31536000, fare=(2.5, 0.6)):
+    return PointParams(seed, box, ncenters, frac_cluster, sigma[0], sigma[1], t0, tspan, fare[0],
+                       fare[1])
+
+
+def gen_points(n, first=0, params=None, columns=True):
+    """Seeded point table rows [first, first+n): xy (2n f32), time u32, fare f32,
+    passengers u8."""
+    params = params or point_params()
+    xy = np.empty(2 * n, np.float32)
+    if columns:
+        t = np.empty(n, np.uint32)
+        f = np.empty(n, np.float32)
+        ps = np.empty(n, np.uint8)
+    else:
+        t = f = ps = None
+    rc = lib().rj_gen_points(C.byref(params), first, n, _p(xy), _p(t), _p(f), _p(ps))
+    if rc:
+        raise RJError(rc, "rj_gen_points")
+    return xy, t, f, ps
+
+
+def gen_voronoi(nsites, seed=1, spacing=(60.0, 60.0), jitter=30.0, box=50000.0):
+    """Voronoi polygon set with shared jittered edges: (xy f32 CSR, offsets i32)."""
+    p = VoronoiParams(seed, nsites, box, spacing[0], spacing[1], jitter)
+    h = C.c_void_p()
+    rc = lib().rj_gen_voronoi(C.byref(p), C.byref(h))
+    if rc:
+        raise RJError(rc, "rj_gen_voronoi")
+    np_, nv = C.c_int32(), C.c_int64()
+    lib().rj_polys_size(h, C.byref(np_), C.byref(nv))
+    xy = np.empty(2 * nv.value, np.float32)
+    off = np.empty(np_.value + 1, np.int32)
+    lib().rj_polys_copy(h, _p(xy), _p(off))
+    lib().rj_polys_free(h)
+    return xy, off
+
+
+# The benchmark configurations (point counts, polygon sets, accuracy).
+CONFIG_POLYS = {
+    "voronoi260": dict(nsites=260, seed=1, spacing=(60.0, 60.0), jitter=30.0),
+    "tracts2200": dict(nsites=2200, seed=3, spacing=(2.2, 140.0), jitter=30.0),
+}
+
+
+def make_constraints(cons):
+    """cons: iterable of (column, ConstraintType, value); float values are
+    stored as float bits, ints as int/uint."""
+    cons = list(cons)
+    arr = (Constraint * max(1, len(cons)))()
+    for i, (col, typ, val) in enumerate(cons):
+        arr[i].attrib_id = col
+        arr[i].type = typ
+        if isinstance(val, (float, np.floating)):
+            arr[i].val = int(np.array([val], np.float32).view(np.uint32)[0])
+        else:
+            arr[i].val = int(val) & 0xFFFFFFFF
+    return arr, len(cons)
+
+
+# ---------------------------------------------------------------------------
+# join API
+# ---------------------------------------------------------------------------
+
+class Table:
+    """A resident device point table (rj_table)."""
+
+    def __init__(self, handler, xy, cols=(), types=(), organize=True):
+        self.h = C.c_void_p()
+        xy = np.ascontiguousarray(xy, np.float32)
+        cols = [np.ascontiguousarray(c) for c in cols]
+        colptr = (C.c_void_p * max(1, len(cols)))(*[_p(c) for c in cols])
+        typ = np.array(list(types) or [0], np.int32)
+        handler._check(lib().rj_table_create(handler.ctx, len(xy) // 2, _p(xy), len(cols),
+                                             C.cast(colptr, C.c_void_p), _p(typ), int(organize),
+                                             C.byref(self.h)))
+        self.n = len(xy) // 2
+        self.types = list(types)
+
+    def read(self, row0=0, n=None):
+        n = self.n - row0 if n is None else n
+        xy = np.empty(2 * n, np.float32)
+        cols = []
+        for t in self.types:
+            cols.append(np.empty(n, np.uint8 if t == UINT8 else
+                                 (np.float32 if t == FLOAT else
+                                  (np.int32 if t == INT else np.uint32))))
+        colptr = (C.c_void_p * max(1, len(cols)))(*[_p(c) for c in cols])
+        rc = lib().rj_table_read(self.h, row0, n, _p(xy), C.cast(colptr, C.c_void_p))
+        if rc:
+            raise RJError(rc, "rj_table_read")
+        return xy, cols
+
+    def close(self):
+        if self.h:
+            lib().rj_table_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Handler:
+    """GLHandler-shaped front end of one rj_ctx."""
+
+    def __init__(self, gpu_mem_mb=0, inmem=True):
+        self.ctx = C.c_void_p()
+        rc = lib().rj_create(int(gpu_mem_mb), int(bool(inmem)), C.byref(self.ctx))
+        if rc:
+            raise RJError(rc, "rj_create (is a GPU visible?)")
+        self.np = 0
+        self._keep = []
+
+    def _check(self, rc):
+        if rc:
+            raise RJError(rc, lib().rj_last_error(self.ctx).decode())
+        return rc
+
+    def close(self):
+        if self.ctx:
+            lib().rj_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        self.close()
+
+    # --- setters (DataHandler / PolyHandler / GLHandler) ---
+    def set_polygons(self, xy, off, bbox=None):
+        xy = np.ascontiguousarray(xy, np.float32)
+        off = np.ascontiguousarray(off, np.int32)
+        b = None if bbox is None else np.ascontiguousarray(bbox, np.float64)
+        self._check(lib().rj_set_polygons(self.ctx, _p(xy), _p(off), len(off) - 1, _p(b)))
+        self.np = len(off) - 1
+
+    def set_points(self, xy, cols=(), types=()):
+        xy = np.ascontiguousarray(xy, np.float32)
+        cols = [np.ascontiguousarray(c) for c in cols]
+        colptr = (C.c_void_p * max(1, len(cols)))(*[_p(c) for c in cols])
+        typ = np.array(list(types) or [0], np.int32)
+        self._check(lib().rj_set_points(self.ctx, len(xy) // 2, _p(xy), len(cols),
+                                        C.cast(colptr, C.c_void_p), _p(typ)))
+
+    def bind_table(self, table, row_begin=0, row_end=None):
+        row_end = table.n if row_end is None else row_end
+        self._check(lib().rj_bind_table(self.ctx, table.h, row_begin, row_end))
+        self._keep.append(table)
+
+    def set_query_constraints(self, cons):
+        arr, n = make_constraints(cons)
+        self._check(lib().rj_set_constraints(self.ctx, C.cast(arr, C.c_void_p), n))
+
+    def set_aggregation(self, agg=COUNT, col=-1):
+        self._check(lib().rj_set_aggregation(self.ctx, agg, col))
+
+    def set_accuracy_distance(self, meters, ground_res=1.0):
+        self._check(lib().rj_set_accuracy(self.ctx, float(meters), float(ground_res)))
+
+    def set_poly_index_resolution(self, rx, ry=None):
+        self._check(lib().rj_set_index_res(self.ctx, rx, rx if ry is None else ry))
+
+    def set_option(self, key, value):
+        self._check(lib().rj_set_option(self.ctx, key, int(value)))
+
+    # --- execution ---
+    def execute_function(self, fn, legacy=False):
+        """Returns (counts int64[np], sums float64[np]) or with legacy=True the
+        reference's int32[3*np] result vector as well."""
+        counts = np.zeros(self.np, np.int64)
+        sums = np.zeros(self.np, np.float64)
+        leg = np.zeros(3 * self.np, np.int32) if legacy else None
+        self._check(lib().rj_execute(self.ctx, fn, _p(counts), _p(sums), _p(leg)))
+        return (counts, sums, leg) if legacy else (counts, sums)
+
+    def sums_fixed(self):
+        hi = np.zeros(self.np, np.int64)
+        lo = np.zeros(self.np, np.uint64)
+        self._check(lib().rj_get_sums_fixed(self.ctx, _p(hi), _p(lo)))
+        return hi, lo
+
+    def get_error_bounds(self, fn=ERRORBOUNDS):
+        if fn != ERRORBOUNDS:
+            return np.zeros(0, np.int64)
+        out = np.zeros(4 * self.np, np.int64)
+        self._check(lib().rj_error_bounds(self.ctx, _p(out)))
+        return out
+
+    def print_time_stats(self, fn):
+        buf = C.create_string_buffer(1024)
+        self._check(lib().rj_time_stats(self.ctx, fn, buf, 1024))
+        return buf.value.decode()
+
+    def canvas_size(self):
+        W, H = C.c_int32(), C.c_int32()
+        self._check(lib().rj_canvas_info(self.ctx, C.byref(W), C.byref(H)))
+        return W.value, H.value
+
+    def kernel_times(self):
+        out = {}
+        name = C.create_string_buffer(64)
+        ms = C.c_double()
+        n = C.c_int32()
+        k = lib().rj_kernel_times(self.ctx, -1, None, 0, None, None)
+        for i in range(max(0, k)):
+            lib().rj_kernel_times(self.ctx, i, name, 64, C.byref(ms), C.byref(n))
+            out[name.value.decode()] = (ms.value, n.value)
+        return out
+
+    def sync(self):
+        self._check(lib().rj_sync(self.ctx))

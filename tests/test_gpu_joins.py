@@ -1,0 +1,265 @@
+"""Parity of the four HIP joins with the oracle: bit-exact counts and
+fixed-point sums (index and hybrid against the CPUJoin restatement and the
+reference's own GridIndexF fixtures; raster and errorbounds against the CPU
+emulation of the same pixel-center canvas rule)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+import rasterjoin as rj
+from common import COLS_TYPES, PASS_LE2, WINDOW_30, points, polys
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def oracle_exact(xy, off, pts, cols=(), cons=(), agg=rj.COUNT, agg_col=0, res=256):
+    g = O.Grid(xy, off, res)
+    types = COLS_TYPES[:len(cols)]
+    return O.join_exact(g, O.PointSet(pts, cols, types, cons, agg_type=agg, agg_col=agg_col))
+
+
+def run(h, fn, xy, off, pts, cols=(), cons=(), agg=rj.COUNT, agg_col=0, eps=None, res=None):
+    h.set_polygons(xy, off)
+    h.set_points(pts, cols, COLS_TYPES[:len(cols)])
+    h.set_query_constraints(cons)
+    h.set_aggregation(agg, agg_col if agg != rj.COUNT else -1)
+    if eps is not None:
+        h.set_accuracy_distance(eps)
+    if res is not None:
+        h.set_poly_index_resolution(res)
+    counts, sums = h.execute_function(fn)
+    hi, lo = h.sums_fixed()
+    return counts, sums, hi, lo
+
+
+# ---------------------------------------------------------------------------
+# index join (K4 grid + K5 PIP), CPUJoin semantics
+# ---------------------------------------------------------------------------
+
+def test_index_toy(handler):
+    d = np.load(os.path.join(GOLD, "toy_squares.npz"))
+    c, _, _, _ = run(handler, rj.INDEX, d["xy"], d["off"], d["pts"], res=4)
+    assert c.tolist() == [2, 3]
+
+
+@pytest.mark.parametrize("name", ["squares_edges", "voronoi20"])
+def test_index_golden_reference_fixture(handler, name):
+    d = np.load(os.path.join(GOLD, name + ".npz"))
+    c, _, _, _ = run(handler, rj.INDEX, d["xy"], d["off"], d["pts"], res=int(d["res"]))
+    assert np.array_equal(c, d["counts"])
+
+
+def test_index_config1_matches_reference_counts(handler):
+    """1M points, 260 polygons, grid 256 (the CPUJoin-comparable workload)."""
+    d = np.load(os.path.join(GOLD, "config1.npz"))
+    xy, off = polys("voronoi260")
+    pts = points(1_000_000, 7)[0]
+    c, _, _, _ = run(handler, rj.INDEX, xy, off, pts, res=256)
+    assert np.array_equal(c, d["counts"])
+
+
+def test_index_filters_and_avg_exact(handler):
+    xy, off = polys("voronoi260")
+    pts, t, fare, pas = points(600_000, 8)
+    cons = WINDOW_30 + PASS_LE2
+    c, s, hi, lo = run(handler, rj.INDEX, xy, off, pts, (t, fare, pas), cons, rj.AVG, 1, res=256)
+    oc, ohi, olo, of64 = oracle_exact(xy, off, pts, (t, fare, pas), cons, rj.AVG, 1, res=256)
+    assert np.array_equal(c, oc)
+    assert np.array_equal(hi, ohi) and np.array_equal(lo, olo)
+    assert np.allclose(s, of64, rtol=1e-9, atol=0)
+
+
+def test_index_overlapping_polygons(handler):
+    xy, off = polys("stars")
+    pts = points(400_000, 9)[0]
+    c, _, _, _ = run(handler, rj.INDEX, xy, off, pts, res=64)
+    oc, _, _, _ = oracle_exact(xy, off, pts, res=64)
+    assert np.array_equal(c, oc)
+    assert c.sum() > 400_000 // 4
+
+
+def test_index_edge_cases(handler):
+    xy, off = polys("voronoi20")
+    # empty input
+    c, _, _, _ = run(handler, rj.INDEX, xy, off, np.zeros(0, np.float32), res=32)
+    assert c.sum() == 0
+    # outside the bbox, NaN, infinities, max edges
+    pts = np.array([-1, 5, 5, -1, 50000, 10, 10, 50000, np.nan, 3, 3, np.nan, np.inf, 4,
+                    -np.inf, 4, 0, 0, 49999.996, 49999.996], np.float32)
+    c, _, _, _ = run(handler, rj.INDEX, xy, off, pts, res=32)
+    oc, _, _, _ = oracle_exact(xy, off, pts, res=32)
+    assert np.array_equal(c, oc)
+
+
+# ---------------------------------------------------------------------------
+# raster join (K1 splat + K2 scan, or label map + gather)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("eps", [10.0, 50.0])
+def test_raster_paths_match_emulator(handler, eps):
+    xy, off = polys("voronoi260")
+    pts, t, fare, pas = points(2_000_000, 10)
+    cons = [(2, rj.LTE, 4)]
+    p = O.plan(O.poly_bbox(xy).astype(np.float64), eps)
+    oc, ohi, olo = O.raster_join(p, xy, off, O.PointSet(pts, (t, fare, pas), COLS_TYPES, cons,
+                                                          agg_type=rj.AVG, agg_col=1))
+    for path in (1, 2, 0):
+        handler.set_option(rj.OPT_RASTER_PATH, path)
+        c, s, hi, lo = run(handler, rj.RASTER, xy, off, pts, (t, fare, pas), cons, rj.AVG, 1, eps=eps)
+        assert np.array_equal(c, oc), path
+        assert np.array_equal(hi, ohi) and np.array_equal(lo, olo), path
+    handler.set_option(rj.OPT_RASTER_PATH, 0)
+
+
+def test_raster_tiled_canvas_eps5(handler):
+    xy, off = polys("voronoi260")
+    pts = points(1_000_000, 11)[0]
+    p = O.plan(O.poly_bbox(xy).astype(np.float64), 5.0)
+    assert p.splitx == 2
+    oc, _, _ = O.raster_join(p, xy, off, O.PointSet(pts))
+    c, _, _, _ = run(handler, rj.RASTER, xy, off, pts, eps=5.0)
+    assert handler.canvas_size() == (p.W, p.H)
+    assert np.array_equal(c, oc)
+
+
+def test_raster_overlapping_polygons_use_the_canvas(handler):
+    xy, off = polys("stars")
+    pts = points(500_000, 12)[0]
+    p = O.plan(O.poly_bbox(xy).astype(np.float64), 40.0)
+    oc, _, _ = O.raster_join(p, xy, off, O.PointSet(pts))
+    c, _, _, _ = run(handler, rj.RASTER, xy, off, pts, eps=40.0)
+    assert np.array_equal(c, oc)
+    handler.set_option(rj.OPT_RASTER_PATH, 2)
+    with pytest.raises(rj.RJError):
+        handler.execute_function(rj.RASTER)
+    handler.set_option(rj.OPT_RASTER_PATH, 0)
+
+
+def test_raster_pixel_aligned_squares_are_exact(handler):
+    bbox = np.array([0.0, 0.0, 1000.0, 1000.0])
+    squares = [np.array([i0, j0, i1, j0, i1, j1, i0, j1], np.float32) * 10
+               for (i0, j0, i1, j1) in [(0, 0, 10, 10), (10, 0, 35, 10), (5, 20, 45, 70), (50, 50, 100, 100)]]
+    xy = np.concatenate(squares)
+    off = np.arange(0, 17, 4, dtype=np.int32)
+    rng = np.random.default_rng(1)
+    pts = rng.uniform(0, 1000, (300_000, 2)).astype(np.float32)
+    pts = pts[~np.any(np.abs(pts / 10 - np.round(pts / 10)) < 1e-3, axis=1)].ravel()
+    handler.set_polygons(xy, off, bbox)
+    handler.set_points(pts)
+    handler.set_query_constraints([])
+    handler.set_aggregation(rj.COUNT)
+    handler.set_accuracy_distance(10.0 * np.sqrt(2))
+    c, _ = handler.execute_function(rj.RASTER)
+    exact, _, _, _ = O.join_exact(O.Grid(xy, off, 16, bbox=[0, 0, 1000, 1000]), O.PointSet(pts))
+    assert np.array_equal(c, exact)
+
+
+# ---------------------------------------------------------------------------
+# hybrid join (K6 mask + K7 point pass with exact border test)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name,eps,res", [("voronoi260", 20.0, 256), ("voronoi260", 5.0, 1024),
+                                          ("tracts400", 20.0, 256), ("stars", 100.0, 64)])
+def test_hybrid_equals_cpujoin(handler, name, eps, res):
+    xy, off = polys(name)
+    pts, t, fare, pas = points(1_000_000, 14)
+    cons = WINDOW_30 + PASS_LE2
+    c, s, hi, lo = run(handler, rj.HYBRID, xy, off, pts, (t, fare, pas), cons, rj.AVG, 1, eps=eps, res=res)
+    oc, ohi, olo, of64 = oracle_exact(xy, off, pts, (t, fare, pas), cons, rj.AVG, 1, res=res)
+    assert np.array_equal(c, oc)
+    assert np.array_equal(hi, ohi) and np.array_equal(lo, olo)
+    assert np.allclose(s, of64, rtol=1e-9, atol=0)
+
+
+def test_hybrid_count_config1_reference(handler):
+    d = np.load(os.path.join(GOLD, "config1.npz"))
+    xy, off = polys("voronoi260")
+    pts = points(1_000_000, 7)[0]
+    c, _, _, _ = run(handler, rj.HYBRID, xy, off, pts, eps=20.0, res=256)
+    assert np.array_equal(c, d["counts"])
+
+
+# ---------------------------------------------------------------------------
+# errorbounds (K1 count splat + K3 boundary bands)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("eps", [20.0, 100.0])
+def test_errorbounds_match_emulator_and_contain_exact(handler, eps):
+    xy, off = polys("voronoi260")
+    pts, t, fare, pas = points(1_000_000, 15)
+    cons = PASS_LE2
+    ps = O.PointSet(pts, (t, fare, pas), COLS_TYPES, cons)
+    p = O.plan(O.poly_bbox(xy).astype(np.float64), eps)
+    oc, _, _, canvas = O.raster_join(p, xy, off, ps, want_canvas=True)
+    ob = O.errorbounds(p, xy, off, canvas)
+    c, _, _, _ = run(handler, rj.ERRORBOUNDS, xy, off, pts, (t, fare, pas), cons, eps=eps)
+    b = handler.get_error_bounds(rj.ERRORBOUNDS)
+    assert np.array_equal(c, oc)
+    assert np.array_equal(b, ob)
+    np_ = len(off) - 1
+    exact, _, _, _ = oracle_exact(xy, off, pts, (t, fare, pas), cons)
+    assert np.all(c - b[3 * np_:] <= exact) and np.all(exact <= c + b[2 * np_:3 * np_])
+
+
+# ---------------------------------------------------------------------------
+# API behaviour
+# ---------------------------------------------------------------------------
+
+def test_legacy_result_layout_and_time_stats(handler):
+    xy, off = polys("voronoi20")
+    pts, t, fare, pas = points(100_000, 16)
+    run(handler, rj.RASTER, xy, off, pts, (t, fare, pas), (), rj.AVG, 1, eps=100.0)
+    counts, sums, leg = handler.execute_function(rj.RASTER, legacy=True)
+    np_ = len(off) - 1
+    assert np.array_equal(leg[:np_], counts)
+    approx = leg[np_:2 * np_] * 100.0 + leg[2 * np_:] / 10.0   # main.cpp:206-209 decode
+    assert np.all(np.abs(approx - sums) <= 0.1 + 1e-9 * np.abs(sums))
+    line = handler.print_time_stats(rj.RASTER).split("\t")
+    assert line[0] == "100000" and line[2] == str(np_) and line[3] == "0"
+    assert len(line) == 15
+
+
+def test_errors_are_status_codes(handler):
+    with pytest.raises(rj.RJError) as e:
+        handler.execute_function(rj.RASTER)
+    assert e.value.code == -2
+    xy, off = polys("voronoi20")
+    handler.set_polygons(xy, off)
+    pts, t, fare, pas = points(1000, 1)
+    handler.set_points(pts, (t, fare, pas), COLS_TYPES)
+    with pytest.raises(rj.RJError) as e:
+        handler.set_query_constraints([(0, rj.GT, 1)] * 11)
+    assert e.value.code == -5
+    handler.set_query_constraints([(7, rj.GT, 1)])
+    with pytest.raises(rj.RJError) as e:
+        handler.execute_function(rj.INDEX)
+    assert e.value.code == -1
+    with pytest.raises(rj.RJError) as e:
+        handler.execute_function(9)
+    assert e.value.code == -6
+
+
+def test_memory_budget_rejects_oversized_inputs():
+    h = rj.Handler(gpu_mem_mb=110)   # 10 MB after the reserve
+    xy, off = polys("voronoi20")
+    h.set_polygons(xy, off)
+    pts = points(2_000_000, 2)[0]
+    with pytest.raises(rj.RJError) as e:
+        h.set_points(pts)
+    assert e.value.code == -3
+    h.close()
+
+
+def test_table_order_does_not_change_results(handler):
+    xy, off = polys("voronoi260")
+    pts, t, fare, pas = points(500_000, 18)
+    res = []
+    for org in (0, 1):
+        handler.set_option(rj.OPT_ORGANIZE, org)
+        res.append(run(handler, rj.HYBRID, xy, off, pts, (t, fare, pas), (), rj.AVG, 1, eps=20.0, res=256))
+    handler.set_option(rj.OPT_ORGANIZE, 1)
+    for a, b in zip(res[0], res[1]):
+        assert np.array_equal(a, b)
