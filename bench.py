@@ -68,12 +68,23 @@ def main():
     ap.add_argument("--eps", type=float, default=10.0)
     ap.add_argument("--path", type=int, default=0, help="0 auto, 1 canvas, 2 label map")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="first run a plain 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(points would shard by row range with one all-reduce of the per-polygon sums)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.calib:
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 1 GiB read + 1 GiB written")
+        del src, dst
 
     n = args.points
     cp = rj.CONFIG_POLYS["voronoi260"]

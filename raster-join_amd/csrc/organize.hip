@@ -50,7 +50,7 @@ __global__ void k_gather(const T* __restrict__ src, const uint32_t* __restrict__
 size_t organize_scratch_bytes(int64_t n)
 {
     size_t temp = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (uint32_t*)nullptr, (uint32_t*)nullptr,
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (uint32_t*)nullptr, (uint32_t*)nullptr,
                                        (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 32);
     size_t kv = (size_t)n * 4 * 4;       // keys in/out + values in/out
     size_t col = (size_t)n * 8;          // the widest column (xy)
@@ -69,7 +69,7 @@ int organize_rows(hipStream_t s, int64_t n, float* xy, void** cols, const int32_
     char* colbuf = (char*)(v_out + n);
     char* temp = colbuf + (size_t)n * 8;
     size_t temp_bytes = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, k_in, k_out, v_in, v_out, (int)n, 0, 32, s);
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, k_in, k_out, v_in, v_out, (int)n, 0, 32, s);
     const float sx = 65535.0f / fmaxf(bbox[2] - bbox[0], 1e-30f);
     const float sy = 65535.0f / fmaxf(bbox[3] - bbox[1], 1e-30f);
     const int blocks = (int)((n + 255) / 256);

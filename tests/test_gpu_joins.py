@@ -217,9 +217,10 @@ def test_legacy_result_layout_and_time_stats(handler):
     assert np.array_equal(leg[:np_], counts)
     approx = leg[np_:2 * np_] * 100.0 + leg[2 * np_:] / 10.0   # main.cpp:206-209 decode
     assert np.all(np.abs(approx - sums) <= 0.1 + 1e-9 * np.abs(sums))
-    line = handler.print_time_stats(rj.RASTER).split("\t")
-    assert line[0] == "100000" and line[2] == str(np_) and line[3] == "0"
-    assert len(line) == 15
+    # GLHandler.cpp:228-240: N, np, fn, passes, nConstraints, 9 times, accuracy
+    line = handler.print_time_stats(rj.RASTER).split()
+    assert line[0] == "100000" and line[1] == str(np_) and line[2] == "0"
+    assert len(line) == 15 and float(line[-1]) == 100.0
 
 
 def test_errors_are_status_codes(handler):
