@@ -306,8 +306,15 @@ struct rj_ctx {
     int build_query(DevQuery* q)
     {
         std::memset(q, 0, sizeof(*q));
+        q->agg_slot = -1;
         if ((int)cons.size() > RJ_MAX_CONSTRAINTS) return fail(RJ_ERR_LIMIT, "at most 10 constraints");
-        std::vector<int> used_cols;
+        auto slot_of = [&](int col) {
+            for (int s = 0; s < q->nslots; s++)
+                if (q->slot_col[s] == col) return s;
+            if (q->nslots == kMaxSlots) return -1;
+            q->slot_col[q->nslots] = col;
+            return q->nslots++;
+        };
         for (size_t k = 0; k < cons.size(); k++) {
             const rj_constraint& c = cons[k];
             if (c.attrib_id < 0 || c.attrib_id >= table->ncols)
@@ -316,7 +323,8 @@ struct rj_ctx {
             q->con_col[k] = c.attrib_id;
             q->con_op[k] = c.type;
             q->con_val[k] = c.val.uval;
-            used_cols.push_back(c.attrib_id);
+            q->con_slot[k] = slot_of(c.attrib_id);
+            if (q->con_slot[k] < 0) return fail(RJ_ERR_LIMIT, "a query may use at most 5 attributes");
         }
         q->ncons = (int)cons.size();
         q->agg_type = agg_type;
@@ -325,11 +333,9 @@ struct rj_ctx {
             if (agg_col < 0 || agg_col >= table->ncols)
                 return fail(RJ_ERR_ARG, "aggregation column out of range");
             q->agg_col = agg_col;
-            used_cols.push_back(agg_col);
+            q->agg_slot = slot_of(agg_col);
+            if (q->agg_slot < 0) return fail(RJ_ERR_LIMIT, "a query may use at most 5 attributes");
         }
-        std::sort(used_cols.begin(), used_cols.end());
-        used_cols.erase(std::unique(used_cols.begin(), used_cols.end()), used_cols.end());
-        if (used_cols.size() > 5) return fail(RJ_ERR_LIMIT, "a query may use at most 5 attributes");
         return RJ_OK;
     }
 
@@ -372,34 +378,53 @@ struct rj_ctx {
         return RJ_OK;
     }
 
-    unsigned int* conflicts_ptr() const
+    // d_res: [count | sum hi | sum lo] (3 np) [error bounds] (4 np) [aux] (4):
+    // aux[0] = pixels covered summed over polygons, aux[1] = labeled pixels.
+    unsigned long long* aux_ptr() const { return d_res.as<unsigned long long>() + 7 * (size_t)polys.np; }
+
+    // Label map of the polygons on the canvas with plain stores. Returns
+    // without waiting; the overlap test (aux[0] == aux[1]) is read later.
+    int build_labels(uint32_t* lab)
     {
-        return reinterpret_cast<unsigned int*>(d_res.as<unsigned long long>() + 7 * (size_t)polys.np);
+        const size_t npx = (size_t)plan.W * plan.H;
+        DevCanvas cv = dev_canvas(plan);
+        DevPolys P = dev_polys();
+        unsigned long long* aux = aux_ptr();
+        int rc;
+        if ((rc = launch("fill_label", [&] {
+                 launch_fill_u32(stream, lab, npx, kLabelNone);
+                 launch_fill_u32(stream, (uint32_t*)aux, 8, 0);
+             })))
+            return rc;
+        if ((rc = launch("band_label", [&] {
+                 launch_band(stream, BAND_LABEL, d_bands_sum.as<Band>(), (int)pr.bands_sum.size(), P, cv,
+                             nullptr, nullptr, nullptr, DevResult{}, nullptr, lab, aux, pr.lds_sum, 0);
+             })))
+            return rc;
+        return launch("count_labeled", [&] { launch_count_labeled(stream, lab, npx, aux + 1); });
     }
 
-    int ensure_label_raster()
+    int read_overlap(bool* overlap)
     {
+        unsigned long long a[2] = {0, 0};
+        hipError_t e = hipMemcpyAsync(a, aux_ptr(), 16, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return hip_fail(e, "overlap test");
+        *overlap = a[0] != a[1];
+        return RJ_OK;
+    }
+
+    int ensure_label_raster(bool* fresh)
+    {
+        *fresh = false;
         if (label_raster_valid && cache) return RJ_OK;
         const size_t npx = (size_t)plan.W * plan.H;
         int rc = alloc(d_label_raster, npx * 4, "label map");
         if (rc) return rc;
-        DevCanvas cv = dev_canvas(plan);
-        DevPolys P = dev_polys();
-        uint32_t* lab = d_label_raster.as<uint32_t>();
-        unsigned int* conf = conflicts_ptr();
-        if ((rc = launch("fill_label", [&] { launch_fill_u32(stream, lab, npx, kLabelNone); }))) return rc;
-        if ((rc = launch("fill_conflicts", [&] { launch_fill_u32(stream, conf, 1, 0); }))) return rc;
-        if ((rc = launch("band_label", [&] {
-                 launch_band(stream, BAND_LABEL, d_bands_sum.as<Band>(), (int)pr.bands_sum.size(), P, cv,
-                             nullptr, nullptr, nullptr, DevResult{}, nullptr, lab, conf, pr.lds_sum, 0);
-             })))
-            return rc;
-        unsigned int nconf = 0;
-        hipError_t e = hipMemcpyAsync(&nconf, conf, 4, hipMemcpyDeviceToHost, stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        if (e != hipSuccess) return hip_fail(e, "label conflicts");
-        label_raster_conflict = nconf != 0;
+        if ((rc = build_labels(d_label_raster.as<uint32_t>()))) return rc;
         label_raster_valid = true;
+        label_raster_conflict = false;  // confirmed after the point pass
+        *fresh = true;
         return RJ_OK;
     }
 
@@ -429,14 +454,19 @@ struct rj_ctx {
         DevPolys P = dev_polys();
         uint32_t* lab = d_label_hybrid.as<uint32_t>();
         uint32_t* mask = d_mask.as<uint32_t>();
-        unsigned int* conf = conflicts_ptr();
-        if ((rc = launch("fill_label", [&] { launch_fill_u32(stream, lab, npx, kLabelNone); }))) return rc;
+        if ((rc = build_labels(lab))) return rc;
+        bool overlap = false;
+        if ((rc = read_overlap(&overlap))) return rc;
+        if (overlap) {
+            // pixels in two or more polygons go to the exact test
+            if ((rc = launch("fill_label", [&] { launch_fill_u32(stream, lab, npx, kLabelNone); }))) return rc;
+            if ((rc = launch("band_label_atomic", [&] {
+                     launch_band(stream, BAND_LABEL_ATOMIC, d_bands_sum.as<Band>(), (int)pr.bands_sum.size(), P,
+                                 cv, nullptr, nullptr, nullptr, DevResult{}, nullptr, lab, nullptr, pr.lds_sum, 0);
+                 })))
+                return rc;
+        }
         if ((rc = launch("fill_mask", [&] { launch_fill_u32(stream, mask, wpr * plan.H, 0); }))) return rc;
-        if ((rc = launch("band_label", [&] {
-                 launch_band(stream, BAND_LABEL, d_bands_sum.as<Band>(), (int)pr.bands_sum.size(), P, cv,
-                             nullptr, nullptr, nullptr, DevResult{}, nullptr, lab, conf, pr.lds_sum, 0);
-             })))
-            return rc;
         if ((rc = launch("touch", [&] {
                  launch_touch(stream, d_X.as<int32_t>(), d_Y.as<int32_t>(), d_prev.as<int32_t>(),
                               (int64_t)pr.X.size(), cv, mask);
@@ -489,67 +519,88 @@ int rj_ctx::execute(int fn, int64_t* out_counts, double* out_sums, int32_t* lega
     DevQuery q;
     int rc = build_query(&q);
     if (rc) return rc;
-    if (fn == RJ_ERRORBOUNDS) q.agg_type = RJ_AGG_COUNT;  // counts only (RasterJoinBounds.cpp:124)
+    if (fn == RJ_ERRORBOUNDS) {  // counts only (RasterJoinBounds.cpp:124)
+        q.agg_type = RJ_AGG_COUNT;
+        q.agg_slot = -1;
+    }
     const bool avg = q.agg_type != RJ_AGG_COUNT;
     const int np = polys.np;
-    if ((rc = alloc(d_res, (size_t)np * 7 * 8 + 8, "results"))) return rc;
+    if ((rc = alloc(d_res, (size_t)np * 7 * 8 + 32, "results"))) return rc;
     unsigned long long* res_base = d_res.as<unsigned long long>();
     DevResult res{res_base, (long long*)(res_base + np), res_base + 2 * (size_t)np};
     long long* bnd = (long long*)(res_base + 3 * (size_t)np);
-    if ((rc = launch("fill_results", [&] { launch_fill_u32(stream, (uint32_t*)res_base, (size_t)np * 14, 0); })))
-        return rc;
+    auto clear_results = [&] {
+        return launch("fill_results", [&] { launch_fill_u32(stream, (uint32_t*)res_base, (size_t)np * 14, 0); });
+    };
+    if ((rc = clear_results())) return rc;
 
-    const DevCols cols = dev_cols();
-    const float* xy = static_cast<const float*>(table->xy);
-    int path_used = 0;
+    PointArgs pa;
+    std::memset(&pa, 0, sizeof(pa));
+    pa.xy = static_cast<const float*>(table->xy);
+    pa.begin = rb;
+    pa.end = re;
+    pa.cols = dev_cols();
+    pa.q = q;
+    pa.res = res;
+    pa.lds_np = (np <= 2048) ? np : 0;
+    auto canvas_pass = [&](bool sums) -> int {
+        int r2 = ensure_canvas(sums);
+        if (r2) return r2;
+        PointArgs sp = pa;
+        sp.cv = dev_canvas(plan);
+        sp.cnt = d_cnt.as<uint32_t>();
+        sp.chi = sums ? d_chi.as<unsigned long long>() : nullptr;
+        sp.clo = sums ? d_clo.as<unsigned long long>() : nullptr;
+        return launch("splat", [&] { launch_points(stream, PT_SPLAT, sp); });
+    };
+
     if (fn == RJ_INDEX) {
         if ((rc = ensure_grid())) return rc;
-        DevCanvas cv{};
-        const int lds_np = (np <= 2048) ? np : 0;
-        if ((rc = launch("points_index", [&] {
-                 launch_points(stream, PT_INDEX, xy, rb, re, cols, q, cv, dev_grid(), dev_polys(), nullptr, res,
-                               lds_np);
-             })))
-            return rc;
+        pa.g = dev_grid();
+        pa.P = dev_polys();
+        if ((rc = launch("points_index", [&] { launch_points(stream, PT_INDEX, pa); }))) return rc;
     } else if (fn == RJ_HYBRID) {
         if ((rc = ensure_raster())) return rc;
         if ((rc = ensure_grid())) return rc;
         if ((rc = ensure_label_hybrid())) return rc;
-        const int lds_np = (np <= 2048) ? np : 0;
-        DevCanvas cv = dev_canvas(plan);
-        if ((rc = launch("points_hybrid", [&] {
-                 launch_points(stream, PT_LABEL_HYBRID, xy, rb, re, cols, q, cv, dev_grid(), dev_polys(),
-                               d_label_hybrid.as<uint32_t>(), res, lds_np);
-             })))
-            return rc;
+        pa.cv = dev_canvas(plan);
+        pa.g = dev_grid();
+        pa.P = dev_polys();
+        pa.label = d_label_hybrid.as<uint32_t>();
+        if ((rc = launch("points_hybrid", [&] { launch_points(stream, PT_LABEL_HYBRID, pa); }))) return rc;
     } else if (fn == RJ_RASTER) {
         if ((rc = ensure_raster())) return rc;
-        int path = raster_path;
-        if (path != 1) {
-            if ((rc = ensure_label_raster())) return rc;
+        bool use_label = raster_path != 1;
+        bool fresh = false;
+        if (use_label) {
+            if ((rc = ensure_label_raster(&fresh))) return rc;
             if (label_raster_conflict) {
-                if (path == 2) return fail(RJ_ERR_UNSUPPORTED, "polygons overlap: the label-map path needs disjoint polygons");
-                path = 1;
-            } else {
-                path = 2;
+                if (raster_path == 2)
+                    return fail(RJ_ERR_UNSUPPORTED, "polygons overlap: the label-map path needs disjoint polygons");
+                use_label = false;
             }
         }
-        DevCanvas cv = dev_canvas(plan);
-        if (path == 2) {
-            const int lds_np = (np <= 2048) ? np : 0;
-            DevGrid g{};
-            if ((rc = launch("points_label", [&] {
-                     launch_points(stream, PT_LABEL, xy, rb, re, cols, q, cv, g, dev_polys(),
-                                   d_label_raster.as<uint32_t>(), res, lds_np);
-                 })))
-                return rc;
-        } else {
-            if ((rc = ensure_canvas(avg))) return rc;
-            if ((rc = launch("splat", [&] {
-                     launch_splat(stream, xy, rb, re, cols, q, cv, d_cnt.as<uint32_t>(), d_chi.as<long long>(),
-                                  d_clo.as<unsigned long long>());
-                 })))
-                return rc;
+        if (use_label) {
+            pa.cv = dev_canvas(plan);
+            pa.P = dev_polys();
+            pa.label = d_label_raster.as<uint32_t>();
+            if ((rc = launch("points_label", [&] { launch_points(stream, PT_LABEL, pa); }))) return rc;
+            if (fresh) {
+                // the overlap test of the freshly built map, read with the results
+                bool overlap = false;
+                if ((rc = read_overlap(&overlap))) return rc;
+                if (overlap) {
+                    label_raster_conflict = true;
+                    if (raster_path == 2)
+                        return fail(RJ_ERR_UNSUPPORTED, "polygons overlap: the label-map path needs disjoint polygons");
+                    if ((rc = clear_results())) return rc;
+                    use_label = false;
+                }
+            }
+        }
+        if (!use_label) {
+            if ((rc = canvas_pass(avg))) return rc;
+            DevCanvas cv = dev_canvas(plan);
             if ((rc = launch("band_sum", [&] {
                      launch_band(stream, BAND_SUM, d_bands_sum.as<Band>(), (int)pr.bands_sum.size(), dev_polys(),
                                  cv, d_cnt.as<uint32_t>(), d_chi.as<long long>(), d_clo.as<unsigned long long>(),
@@ -557,16 +608,11 @@ int rj_ctx::execute(int fn, int64_t* out_counts, double* out_sums, int32_t* lega
                  })))
                 return rc;
         }
-        path_used = path;
     } else {
         if ((rc = ensure_raster())) return rc;
         if (!pr.bounds_ok) return fail(RJ_ERR_UNSUPPORTED, "a polygon is too wide for the error-bounds kernel");
         DevCanvas cv = dev_canvas(plan);
-        if ((rc = ensure_canvas(false))) return rc;
-        if ((rc = launch("splat", [&] {
-                 launch_splat(stream, xy, rb, re, cols, q, cv, d_cnt.as<uint32_t>(), nullptr, nullptr);
-             })))
-            return rc;
+        if ((rc = canvas_pass(false))) return rc;
         if ((rc = launch("band_bounds", [&] {
                  launch_band(stream, BAND_BOUNDS, d_bands_bounds.as<Band>(), (int)pr.bands_bounds.size(),
                              dev_polys(), cv, d_cnt.as<uint32_t>(), nullptr, nullptr, res, bnd, nullptr, nullptr,
@@ -574,7 +620,6 @@ int rj_ctx::execute(int fn, int64_t* out_counts, double* out_sums, int32_t* lega
              })))
             return rc;
     }
-    (void)path_used;
 
     counts.assign(np, 0);
     hi.assign(np, 0);

@@ -1,8 +1,5 @@
 // kernels.hip -- gfx950 kernels of the spatial aggregation join.
 //
-//  k_splat     point splat into the count/sum canvas (the reference's point
-//              pass, RasterJoin.cpp:91-148 + points.vert/points.frag), with
-//              wave-level run aggregation of equal pixels before the atomics.
 //  k_band      scan conversion of one band of rows of one polygon with the
 //              edge-flag rule on the fixed-point canvas grid; per mode it sums
 //              the canvas under the polygon (polygon.frag), writes the polygon
@@ -10,8 +7,9 @@
 //  k_touch     boundary pixels of every edge (hybrid border mask, the
 //              conservative line raster of HybridJoin::drawOutline).
 //  k_points    one pass over the points: CPUJoin grid + exact float PIP
-//              (index.glsl), label-map gather (raster), or label gather with
-//              the exact test on border pixels (hybrid.glsl).
+//              (index.glsl), label-map gather (raster), label gather with the
+//              exact test on border pixels (hybrid.glsl), or the splat into
+//              the count/sum canvas (RasterJoin.cpp:91-148, points.vert/.frag).
 //
 // Compiled with -ffp-contract=off: the float PIP, the grid cell and the pixel
 // formula must round exactly like the host restatement of CPUJoin.
@@ -49,54 +47,69 @@ __device__ __forceinline__ T cmp_op(int op, T a, T b)
     return op == RJ_EQ ? a == b : op == RJ_LT ? a < b : op == RJ_LTE ? a <= b : op == RJ_GT ? a > b : a >= b;
 }
 
-// Conjunctive filters; integer compares for Uint/Int columns (the reference's
-// float-bit equality on LTE/GTE, points.vert:99,107, is not reproduced).
-__device__ __forceinline__ bool pass_filters(const DevCols& C, const DevQuery& q, int64_t i)
+constexpr int kRows = 8;  // consecutive rows per thread and iteration of a point pass
+
+// Typed comparison of 32-bit column patterns: Float columns compare as
+// floats, Int as signed and Uint (and u8) as unsigned integers; the
+// reference's float-bit equality on LTE/GTE (points.vert:99,107) is not kept.
+__device__ __forceinline__ bool cmp_bits(int type, int op, uint32_t a, uint32_t v)
 {
-    for (int k = 0; k < q.ncons; k++) {
-        const int col = q.con_col[k];
-        const int t = C.type[col];
-        const int op = q.con_op[k];
-        const uint32_t v = q.con_val[k];
-        bool ok;
-        if (t == RJ_ATTR_FLOAT) {
-            float a = static_cast<const float*>(C.col[col])[i];
-            ok = cmp_op<float>(op, a, __uint_as_float(v));
-        } else if (t == RJ_ATTR_INT) {
-            int32_t a = static_cast<const int32_t*>(C.col[col])[i];
-            ok = cmp_op<int32_t>(op, a, (int32_t)v);
-        } else {
-            uint32_t a = (t == RJ_ATTR_UINT8) ? (uint32_t) static_cast<const uint8_t*>(C.col[col])[i]
-                                              : static_cast<const uint32_t*>(C.col[col])[i];
-            ok = cmp_op<uint32_t>(op, a, v);
-        }
-        if (!ok) return false;
-    }
-    return true;
+    if (type == RJ_ATTR_FLOAT) return cmp_op<float>(op, __uint_as_float(a), __uint_as_float(v));
+    if (type == RJ_ATTR_INT) return cmp_op<int32_t>(op, (int32_t)a, (int32_t)v);
+    return cmp_op<uint32_t>(op, a, v);
 }
 
-// Exact fixed-point aggregate value: hi = floor(v), lo = frac(v) * 2^32.
-__device__ __forceinline__ void agg_value(const DevCols& C, const DevQuery& q, int64_t i,
-                                          unsigned long long& hi, unsigned long long& lo)
+// Exact fixed-point aggregate value of a 32-bit pattern: hi = floor(v),
+// lo = frac(v) * 2^32 (integer columns: hi = value, lo = 0).
+__device__ __forceinline__ void agg_bits(int type, uint32_t a, unsigned long long& hi, unsigned long long& lo)
 {
-    const int col = q.agg_col;
-    const int t = C.type[col];
     lo = 0;
-    if (t == RJ_ATTR_FLOAT) {
-        float f = static_cast<const float*>(C.col[col])[i];
+    if (type == RJ_ATTR_FLOAT) {
+        const float f = __uint_as_float(a);
         if (!(fabsf(f) < 4.611686018427387904e18f)) {
             hi = 0;
             return;
         }
-        float fl = floorf(f);
+        const float fl = floorf(f);
         hi = (unsigned long long)(long long)fl;
         lo = (unsigned long long)((f - fl) * 4294967296.0f);
-    } else if (t == RJ_ATTR_INT) {
-        hi = (unsigned long long)(long long) static_cast<const int32_t*>(C.col[col])[i];
-    } else if (t == RJ_ATTR_UINT8) {
-        hi = static_cast<const uint8_t*>(C.col[col])[i];
+    } else if (type == RJ_ATTR_INT) {
+        hi = (unsigned long long)(long long)(int32_t)a;
     } else {
-        hi = static_cast<const uint32_t*>(C.col[col])[i];
+        hi = a;
+    }
+}
+
+// Rows [i0, i0+kRows) of one column as 32-bit patterns (u8 widened). Vector
+// loads when the whole group is inside the table, guarded loads otherwise.
+__device__ __forceinline__ void load_rows(const DevCols& C, int col, int64_t i0, bool full, int64_t end,
+                                          uint32_t (&v)[kRows])
+{
+    const int t = C.type[col];
+    if (t == RJ_ATTR_UINT8) {
+        const uint8_t* p = static_cast<const uint8_t*>(C.col[col]);
+        if (full) {
+            const uint2 w = *reinterpret_cast<const uint2*>(p + i0);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                v[r] = (w.x >> (8 * r)) & 0xFFu;
+                v[r + 4] = (w.y >> (8 * r)) & 0xFFu;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < kRows; r++) v[r] = (i0 + r < end) ? (uint32_t)p[i0 + r] : 0u;
+        }
+    } else {
+        const uint32_t* p = static_cast<const uint32_t*>(C.col[col]);
+        if (full) {
+            const uint4 a = *reinterpret_cast<const uint4*>(p + i0);
+            const uint4 b = *reinterpret_cast<const uint4*>(p + i0 + 4);
+            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+            v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+        } else {
+#pragma unroll
+            for (int r = 0; r < kRows; r++) v[r] = (i0 + r < end) ? p[i0 + r] : 0u;
+        }
     }
 }
 
@@ -133,40 +146,6 @@ __device__ __forceinline__ bool pip_f32(const float2* __restrict__ v, int n, flo
         vj = vi;
     }
     return c;
-}
-
-struct Runs {
-    int head;
-    bool tail;
-};
-
-// Runs of equal keys along the lanes of a wave.
-__device__ __forceinline__ Runs runs_of(int key)
-{
-    const int ln = __lane_id();
-    int prev = __shfl_up(key, 1, kWave);
-    int next = __shfl_down(key, 1, kWave);
-    bool h = (ln == 0) || prev != key;
-    bool t = (ln == kWave - 1) || next != key;
-    unsigned long long hm = __ballot(h);
-    unsigned long long upto = (ln == kWave - 1) ? ~0ull : ((2ull << ln) - 1ull);
-    Runs r;
-    r.head = kWave - 1 - __clzll(hm & upto);
-    r.tail = t;
-    return r;
-}
-
-// Inclusive sum within the run of each lane.
-template <class T>
-__device__ __forceinline__ T run_sum(T v, int head)
-{
-    const int ln = __lane_id();
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        T u = __shfl_up(v, d, kWave);
-        if (ln - d >= head) v += u;
-    }
-    return v;
 }
 
 template <class T>
@@ -210,46 +189,6 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v)
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = v;
-}
-
-// ---------------------------------------------------------------------------
-// point splat into the canvas
-// ---------------------------------------------------------------------------
-
-template <bool AVG>
-__global__ __launch_bounds__(kBlock) void k_splat(const float2* __restrict__ xy, int64_t begin,
-                                                  int64_t end, int64_t per_block, DevCols C, DevQuery q,
-                                                  DevCanvas cv, uint32_t* __restrict__ cnt,
-                                                  unsigned long long* __restrict__ chi,
-                                                  unsigned long long* __restrict__ clo)
-{
-    const int64_t b0 = begin + (int64_t)blockIdx.x * per_block;
-    const int64_t b1 = min(end, b0 + per_block);
-    for (int64_t base = b0; base < b1; base += blockDim.x) {
-        const int64_t i = base + threadIdx.x;
-        int key = -1;
-        unsigned long long hi = 0, lo = 0;
-        if (i < b1) {
-            float2 p = xy[i];
-            if (pass_filters(C, q, i)) {
-                key = pixel_of(cv, p.x, p.y);
-                if (AVG && key >= 0) agg_value(C, q, i, hi, lo);
-            }
-        }
-        Runs r = runs_of(key);
-        unsigned c = run_sum<unsigned>(key >= 0 ? 1u : 0u, r.head);
-        if (AVG) {
-            hi = run_sum(hi, r.head);
-            lo = run_sum(lo, r.head);
-        }
-        if (r.tail && key >= 0) {
-            atomicAdd(&cnt[key], c);
-            if (AVG) {
-                atomicAdd(&chi[key], hi);
-                atomicAdd(&clo[key], lo);
-            }
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -353,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void k_band(const Band* __restrict__ bands,
                                                  const unsigned long long* __restrict__ chi,
                                                  const unsigned long long* __restrict__ clo, DevResult res,
                                                  unsigned long long* __restrict__ bounds, uint32_t* label,
-                                                 unsigned int* conflicts, int with_sums)
+                                                 unsigned long long* aux, int with_sums)
 {
     extern __shared__ uint32_t lds[];
     const Band bd = bands[blockIdx.x];
@@ -404,11 +343,13 @@ __global__ __launch_bounds__(kBlock) void k_band(const Band* __restrict__ bands,
             }
         } else if (MODE == BAND_LABEL) {
             if (inside) {
-                uint32_t old = atomicCAS(&label[pix], kLabelNone, (uint32_t)p);
-                if (old != kLabelNone) {
-                    atomicExch(&label[pix], kLabelBorder);
-                    atomicAdd(conflicts, 1u);
-                }
+                label[pix] = (uint32_t)p;
+                s_cnt++;
+            }
+        } else if (MODE == BAND_LABEL_ATOMIC) {
+            if (inside) {
+                const uint32_t old = atomicCAS(&label[pix], kLabelNone, (uint32_t)p);
+                if (old != kLabelNone) atomicExch(&label[pix], kLabelBorder);
             }
         } else {
             const uint32_t val = cnt[pix];
@@ -442,6 +383,9 @@ __global__ __launch_bounds__(kBlock) void k_band(const Band* __restrict__ bands,
     if (MODE == BAND_SUM) {
         block_add4(s_cnt, s_hi, s_lo, 0, res.cnt + p, (unsigned long long*)res.hi + p,
                    res.lo + p, nullptr);
+    } else if (MODE == BAND_LABEL) {
+        // total coverage; equals the labeled pixel count iff no pixel is in two polygons
+        block_add4(s_cnt, 0, 0, 0, aux, nullptr, nullptr, nullptr);
     } else if (MODE == BAND_BOUNDS) {
         block_add4(s_cnt, 0, 0, 0, res.cnt + p, nullptr, nullptr, nullptr);
         __syncthreads();
@@ -512,11 +456,21 @@ __global__ void k_dilate_to_label(const uint32_t* __restrict__ mask, int W, int 
 }
 
 // ---------------------------------------------------------------------------
-// point pass: index join, label-map raster join, hybrid join
+// point pass: index join, label-map raster join, hybrid join, canvas splat
 // ---------------------------------------------------------------------------
 
+// Count of labeled pixels (the other half of the overlap test).
+__global__ void k_count_labeled(const uint32_t* __restrict__ label, size_t n, unsigned long long* out)
+{
+    unsigned long long c = 0;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) c += (label[i] != kLabelNone);
+    block_add4(c, 0, 0, 0, out, nullptr, nullptr, nullptr);
+}
+
 struct Acc {
-    unsigned long long* cnt;  // LDS (lds_np entries) or null
+    unsigned long long* cnt;  // LDS (lds_np entries)
     unsigned long long* hi;
     unsigned long long* lo;
     DevResult g;
@@ -562,61 +516,147 @@ __device__ __forceinline__ int exact_first(const DevGrid& g, const DevPolys& P, 
     return first;
 }
 
+// One pass over rows [begin, end). Every thread takes kRows consecutive rows
+// per iteration (vector loads of x,y and of the query's columns), computes one
+// key per row (polygon id, or pixel for PT_SPLAT) and keeps a running
+// (key, count, sum) that it flushes only when the key changes: rows of a
+// spatially ordered table mostly repeat the key, so flushes are rare.
 template <int MODE, bool AVG>
-__global__ __launch_bounds__(kBlock) void k_points(const float2* __restrict__ xy, int64_t begin,
-                                                   int64_t end, int64_t per_block, DevCols C, DevQuery q,
-                                                   DevCanvas cv, DevGrid g, DevPolys P,
-                                                   const uint32_t* __restrict__ label, DevResult res,
-                                                   int lds_np)
+__global__ __launch_bounds__(kBlock) void k_points(PointArgs a)
 {
     extern __shared__ unsigned long long sacc[];
+    const DevQuery& q = a.q;
     Acc A;
-    A.lds_np = lds_np;
+    A.lds_np = (MODE == PT_SPLAT) ? 0 : a.lds_np;
     A.cnt = sacc;
-    A.hi = sacc + lds_np;
-    A.lo = sacc + 2 * lds_np;
-    A.g = res;
-    const int nacc = AVG ? 3 * lds_np : lds_np;
+    A.hi = sacc + A.lds_np;
+    A.lo = sacc + 2 * A.lds_np;
+    A.g = a.res;
+    const int nacc = AVG ? 3 * A.lds_np : A.lds_np;
     for (int k = threadIdx.x; k < nacc; k += blockDim.x) sacc[k] = 0;
     __syncthreads();
 
-    const int64_t b0 = begin + (int64_t)blockIdx.x * per_block;
-    const int64_t b1 = min(end, b0 + per_block);
-    for (int64_t base = b0; base < b1; base += blockDim.x) {
-        const int64_t i = base + threadIdx.x;
-        int key = -1;
-        unsigned long long hi = 0, lo = 0;
-        if (i < b1 && pass_filters(C, q, i)) {
-            const float2 pt = xy[i];
-            if (AVG) agg_value(C, q, i, hi, lo);
-            if (MODE == PT_INDEX) {
-                key = exact_first<AVG>(g, P, A, pt.x, pt.y, hi, lo);
+    const int64_t b0 = a.start + (int64_t)blockIdx.x * a.per_block;
+    const int64_t b1 = min(a.end, b0 + a.per_block);
+    const int64_t lo_row = max(a.begin, b0);
+    int ck = -1;
+    unsigned long long cc = 0, ch = 0, cl = 0;
+    for (int64_t base = b0; base < b1; base += (int64_t)blockDim.x * kRows) {
+        const int64_t i0 = base + (int64_t)threadIdx.x * kRows;
+        if (i0 >= b1) continue;
+        const bool full = (i0 + kRows <= a.end);
+        float px[kRows], py[kRows];
+        if (full) {
+            const float4* p4 = reinterpret_cast<const float4*>(a.xy + 2 * i0);
+#pragma unroll
+            for (int k = 0; k < kRows / 2; k++) {
+                const float4 f = p4[k];
+                px[2 * k] = f.x; py[2 * k] = f.y;
+                px[2 * k + 1] = f.z; py[2 * k + 1] = f.w;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < kRows; r++) {
+                const bool in = i0 + r < a.end;
+                px[r] = in ? a.xy[2 * (i0 + r)] : 0.0f;
+                py[r] = in ? a.xy[2 * (i0 + r) + 1] : 0.0f;
+            }
+        }
+        bool ok[kRows];
+#pragma unroll
+        for (int r = 0; r < kRows; r++) ok[r] = (i0 + r >= lo_row) && (i0 + r < b1);
+        unsigned long long vh[kRows], vl[kRows];
+#pragma unroll
+        for (int r = 0; r < kRows; r++) vh[r] = vl[r] = 0;
+#pragma unroll
+        for (int s = 0; s < kMaxSlots; s++) {
+            if (s >= q.nslots) break;
+            uint32_t v[kRows];
+            const int col = q.slot_col[s];
+            load_rows(a.cols, col, i0, full, a.end, v);
+            const int t = a.cols.type[col];
+            for (int k = 0; k < q.ncons; k++) {
+                if (q.con_slot[k] != s) continue;
+                const int op = q.con_op[k];
+                const uint32_t cv = q.con_val[k];
+#pragma unroll
+                for (int r = 0; r < kRows; r++) ok[r] = ok[r] && cmp_bits(t, op, v[r], cv);
+            }
+            if (AVG && s == q.agg_slot) {
+#pragma unroll
+                for (int r = 0; r < kRows; r++) agg_bits(t, v[r], vh[r], vl[r]);
+            }
+        }
+        int key[kRows];
+        if (MODE == PT_INDEX) {
+#pragma unroll
+            for (int r = 0; r < kRows; r++)
+                key[r] = ok[r] ? exact_first<AVG>(a.g, a.P, A, px[r], py[r], vh[r], vl[r]) : -1;
+        } else {
+            int pix[kRows];
+#pragma unroll
+            for (int r = 0; r < kRows; r++) pix[r] = ok[r] ? pixel_of(a.cv, px[r], py[r]) : -1;
+            if (MODE == PT_SPLAT) {
+#pragma unroll
+                for (int r = 0; r < kRows; r++) key[r] = pix[r];
             } else {
-                const int pix = pixel_of(cv, pt.x, pt.y);
-                if (pix >= 0) {
-                    const uint32_t lab = label[pix];
-                    if (lab < kLabelMaxId) key = (int)lab;
-                    else if (MODE == PT_LABEL_HYBRID && lab == kLabelBorder)
-                        key = exact_first<AVG>(g, P, A, pt.x, pt.y, hi, lo);
+                uint32_t lab[kRows];
+#pragma unroll
+                for (int r = 0; r < kRows; r++) lab[r] = pix[r] >= 0 ? a.label[pix[r]] : kLabelNone;
+#pragma unroll
+                for (int r = 0; r < kRows; r++) {
+                    key[r] = lab[r] < kLabelMaxId ? (int)lab[r] : -1;
+                    if (MODE == PT_LABEL_HYBRID && lab[r] == kLabelBorder)
+                        key[r] = exact_first<AVG>(a.g, a.P, A, px[r], py[r], vh[r], vl[r]);
                 }
             }
         }
-        Runs r = runs_of(key);
-        unsigned long long c = run_sum<unsigned long long>(key >= 0 ? 1ull : 0ull, r.head);
-        if (AVG) {
-            hi = run_sum(key >= 0 ? hi : 0ull, r.head);
-            lo = run_sum(key >= 0 ? lo : 0ull, r.head);
-        }
-        if (r.tail && key >= 0) acc_add(A, key, c, hi, lo, AVG);
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < lds_np; k += blockDim.x) {
-        unsigned long long c = sacc[k];
-        if (c) {
-            atomicAdd(&res.cnt[k], c);
+#pragma unroll
+        for (int r = 0; r < kRows; r++) {
+            const int k = key[r];
+            if (k < 0) continue;
+            if (k != ck) {
+                if (ck >= 0) {
+                    if (MODE == PT_SPLAT) {
+                        atomicAdd(&a.cnt[ck], (uint32_t)cc);
+                        if (AVG) {
+                            atomicAdd(&a.chi[ck], ch);
+                            atomicAdd(&a.clo[ck], cl);
+                        }
+                    } else {
+                        acc_add(A, ck, cc, ch, cl, AVG);
+                    }
+                }
+                ck = k;
+                cc = ch = cl = 0;
+            }
+            cc++;
             if (AVG) {
-                atomicAdd((unsigned long long*)&res.hi[k], sacc[lds_np + k]);
-                atomicAdd(&res.lo[k], sacc[2 * lds_np + k]);
+                ch += vh[r];
+                cl += vl[r];
+            }
+        }
+    }
+    if (ck >= 0) {
+        if (MODE == PT_SPLAT) {
+            atomicAdd(&a.cnt[ck], (uint32_t)cc);
+            if (AVG) {
+                atomicAdd(&a.chi[ck], ch);
+                atomicAdd(&a.clo[ck], cl);
+            }
+        } else {
+            acc_add(A, ck, cc, ch, cl, AVG);
+        }
+    }
+    if (MODE == PT_SPLAT) return;
+    __syncthreads();
+    for (int k = threadIdx.x; k < A.lds_np; k += blockDim.x) {
+        const unsigned long long c = sacc[k];
+        if (c) {
+            atomicAdd(&a.res.cnt[k], c);
+            if (AVG) {
+                atomicAdd((unsigned long long*)&a.res.hi[k], sacc[A.lds_np + k]);
+                atomicAdd(&a.res.lo[k], sacc[2 * A.lds_np + k]);
             }
         }
     }
@@ -654,40 +694,26 @@ static void chunking(int64_t n, int cap, int* blocks, int64_t* per_block)
     *per_block = per;
 }
 
-void launch_splat(hipStream_t s, const float* xy, int64_t begin, int64_t end, const DevCols& cols,
-                  const DevQuery& q, const DevCanvas& c, uint32_t* cnt, long long* chi,
-                  unsigned long long* clo)
-{
-    int64_t n = end - begin;
-    if (n <= 0) return;
-    int blocks;
-    int64_t per;
-    chunking(n, 16384, &blocks, &per);
-    if (q.agg_type != RJ_AGG_COUNT)
-        hipLaunchKernelGGL(k_splat<true>, dim3(blocks), dim3(kBlock), 0, s, (const float2*)xy, begin,
-                           end, per, cols, q, c, cnt, (unsigned long long*)chi, clo);
-    else
-        hipLaunchKernelGGL(k_splat<false>, dim3(blocks), dim3(kBlock), 0, s, (const float2*)xy, begin,
-                           end, per, cols, q, c, cnt, (unsigned long long*)chi, clo);
-}
-
 void launch_band(hipStream_t s, int mode, const Band* bands, int nbands, const DevPolys& P,
                  const DevCanvas& c, const uint32_t* cnt, const long long* chi,
                  const unsigned long long* clo, DevResult res, long long* bounds, uint32_t* label,
-                 unsigned int* conflicts, size_t lds_bytes, int with_sums)
+                 unsigned long long* aux, size_t lds_bytes, int with_sums)
 {
     if (nbands <= 0) return;
     const unsigned long long* hi = (const unsigned long long*)chi;
     unsigned long long* bd = (unsigned long long*)bounds;
     if (mode == BAND_SUM)
         hipLaunchKernelGGL(k_band<BAND_SUM>, dim3(nbands), dim3(kBlock), lds_bytes, s, bands, P, c, cnt,
-                           hi, clo, res, bd, label, conflicts, with_sums);
+                           hi, clo, res, bd, label, aux, with_sums);
     else if (mode == BAND_LABEL)
         hipLaunchKernelGGL(k_band<BAND_LABEL>, dim3(nbands), dim3(kBlock), lds_bytes, s, bands, P, c,
-                           cnt, hi, clo, res, bd, label, conflicts, with_sums);
+                           cnt, hi, clo, res, bd, label, aux, with_sums);
+    else if (mode == BAND_LABEL_ATOMIC)
+        hipLaunchKernelGGL(k_band<BAND_LABEL_ATOMIC>, dim3(nbands), dim3(kBlock), lds_bytes, s, bands, P, c,
+                           cnt, hi, clo, res, bd, label, aux, with_sums);
     else
         hipLaunchKernelGGL(k_band<BAND_BOUNDS>, dim3(nbands), dim3(kBlock), lds_bytes, s, bands, P, c,
-                           cnt, hi, clo, res, bd, label, conflicts, with_sums);
+                           cnt, hi, clo, res, bd, label, aux, with_sums);
 }
 
 void launch_touch(hipStream_t s, const int32_t* X, const int32_t* Y, const int32_t* prev,
@@ -719,30 +745,35 @@ void launch_dilate_to_label(hipStream_t s, const uint32_t* mask_bits, const DevC
                        label);
 }
 
-void launch_points(hipStream_t s, int mode, const float* xy, int64_t begin, int64_t end,
-                   const DevCols& cols, const DevQuery& q, const DevCanvas& c, const DevGrid& g,
-                   const DevPolys& P, const uint32_t* label, DevResult res, int lds_np)
+void launch_count_labeled(hipStream_t s, const uint32_t* label, size_t n, unsigned long long* out)
 {
-    int64_t n = end - begin;
-    if (n <= 0) return;
-    int blocks;
-    int64_t per;
-    chunking(n, 4096, &blocks, &per);
-    const bool avg = q.agg_type != RJ_AGG_COUNT;
-    size_t lds = (size_t)(avg ? 3 : 1) * lds_np * sizeof(unsigned long long);
-    const float2* p = (const float2*)xy;
-#define RJ_LAUNCH_PTS(M, A)                                                                             \
-    hipLaunchKernelGGL((k_points<M, A>), dim3(blocks), dim3(kBlock), lds, s, p, begin, end, per, cols, q, \
-                       c, g, P, label, res, lds_np)
-    if (mode == PT_INDEX) {
-        if (avg) RJ_LAUNCH_PTS(PT_INDEX, true);
-        else RJ_LAUNCH_PTS(PT_INDEX, false);
-    } else if (mode == PT_LABEL) {
-        if (avg) RJ_LAUNCH_PTS(PT_LABEL, true);
-        else RJ_LAUNCH_PTS(PT_LABEL, false);
-    } else {
-        if (avg) RJ_LAUNCH_PTS(PT_LABEL_HYBRID, true);
-        else RJ_LAUNCH_PTS(PT_LABEL_HYBRID, false);
+    int blocks = (int)std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
+    hipLaunchKernelGGL(k_count_labeled, dim3(blocks), dim3(kBlock), 0, s, label, n, out);
+}
+
+void launch_points(hipStream_t s, int mode, PointArgs a)
+{
+    if (a.end <= a.begin) return;
+    // Row groups start at a multiple of kRows so that vector loads are aligned.
+    a.start = a.begin - a.begin % kRows;
+    const int64_t n = a.end - a.start;
+    const int64_t iter_rows = (int64_t)kBlock * kRows;
+    const int64_t cap = 4096;
+    int64_t per = (n + cap - 1) / cap;
+    per = (per + iter_rows - 1) / iter_rows * iter_rows;
+    a.per_block = per;
+    const int blocks = (int)((n + per - 1) / per);
+    const bool avg = a.q.agg_type != RJ_AGG_COUNT;
+    const size_t lds = (mode == PT_SPLAT) ? 0 : (size_t)(avg ? 3 : 1) * a.lds_np * sizeof(unsigned long long);
+#define RJ_LAUNCH_PTS(M, A) \
+    hipLaunchKernelGGL((k_points<M, A>), dim3(blocks), dim3(kBlock), lds, s, a)
+    switch (mode) {
+    case PT_INDEX: if (avg) RJ_LAUNCH_PTS(PT_INDEX, true); else RJ_LAUNCH_PTS(PT_INDEX, false); break;
+    case PT_LABEL: if (avg) RJ_LAUNCH_PTS(PT_LABEL, true); else RJ_LAUNCH_PTS(PT_LABEL, false); break;
+    case PT_LABEL_HYBRID:
+        if (avg) RJ_LAUNCH_PTS(PT_LABEL_HYBRID, true); else RJ_LAUNCH_PTS(PT_LABEL_HYBRID, false);
+        break;
+    default: if (avg) RJ_LAUNCH_PTS(PT_SPLAT, true); else RJ_LAUNCH_PTS(PT_SPLAT, false); break;
     }
 #undef RJ_LAUNCH_PTS
 }

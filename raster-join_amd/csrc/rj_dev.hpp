@@ -21,15 +21,21 @@ struct DevCols {
     int32_t type[RJ_MAX_COLUMNS];
 };
 
-// Filters and aggregate of one query.
+constexpr int kMaxSlots = 5;  // distinct columns one query may read (GLFunction.cpp:250-253)
+
+// Filters and aggregate of one query. The columns it reads are numbered as
+// slots; every constraint and the aggregate name their slot.
 struct DevQuery {
     int32_t ncons;
     int32_t agg_type;  // RJ_AGG_*
     int32_t agg_col;
-    int32_t pad;
+    int32_t nslots;
     int32_t con_col[RJ_MAX_CONSTRAINTS];
     int32_t con_op[RJ_MAX_CONSTRAINTS];
     uint32_t con_val[RJ_MAX_CONSTRAINTS];
+    int32_t con_slot[RJ_MAX_CONSTRAINTS];
+    int32_t slot_col[kMaxSlots];
+    int32_t agg_slot;  // -1 for count
 };
 
 // Canvas geometry (same numbers as the host plan).
@@ -73,8 +79,27 @@ struct Band {
     int32_t c0, c1;  // first and last column (inclusive)
 };
 
-enum BandMode { BAND_SUM = 0, BAND_LABEL = 1, BAND_BOUNDS = 2 };
-enum PointMode { PT_INDEX = 0, PT_LABEL = 1, PT_LABEL_HYBRID = 2 };
+enum BandMode { BAND_SUM = 0, BAND_LABEL = 1, BAND_BOUNDS = 2, BAND_LABEL_ATOMIC = 3 };
+enum PointMode { PT_INDEX = 0, PT_LABEL = 1, PT_LABEL_HYBRID = 2, PT_SPLAT = 3 };
+
+// Arguments of one point pass.
+struct PointArgs {
+    const float* xy;
+    int64_t begin, end;   // rows [begin, end)
+    int64_t start;        // begin rounded down to the row group size
+    int64_t per_block;    // rows per workgroup (multiple of the rows per iteration)
+    DevCols cols;
+    DevQuery q;
+    DevCanvas cv;
+    DevGrid g;
+    DevPolys P;
+    const uint32_t* label;
+    DevResult res;
+    int lds_np;           // polygons accumulated in LDS (0: global atomics)
+    uint32_t* cnt;        // canvas (PT_SPLAT)
+    unsigned long long* chi;
+    unsigned long long* clo;
+};
 
 // LDS bytes a band of `nrows` x `wpr` words needs in each mode.
 inline size_t band_lds_bytes(int mode, int nrows, int wpr)
@@ -89,21 +114,18 @@ inline size_t band_lds_bytes(int mode, int nrows, int wpr)
 
 // ---- launchers (kernels.hip) ----
 void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);
-void launch_splat(hipStream_t s, const float* xy, int64_t begin, int64_t end, const DevCols& cols,
-                  const DevQuery& q, const DevCanvas& c, uint32_t* cnt, long long* chi,
-                  unsigned long long* clo);
 void launch_band(hipStream_t s, int mode, const Band* bands, int nbands, const DevPolys& P,
                  const DevCanvas& c, const uint32_t* cnt, const long long* chi,
                  const unsigned long long* clo, DevResult res, long long* bounds, uint32_t* label,
-                 unsigned int* conflicts, size_t lds_bytes, int with_sums);
+                 unsigned long long* aux, size_t lds_bytes, int with_sums);
+void launch_count_labeled(hipStream_t s, const uint32_t* label, size_t n, unsigned long long* out);
 void launch_touch(hipStream_t s, const int32_t* X, const int32_t* Y, const int32_t* prev,
                   int64_t nverts, const DevCanvas& c, uint32_t* mask_bits);
 void launch_mark_rects(hipStream_t s, const int32_t* rects, int nrects, const DevCanvas& c,
                        uint32_t* mask_bits);
 void launch_dilate_to_label(hipStream_t s, const uint32_t* mask_bits, const DevCanvas& c, int r,
                             uint32_t* label);
-void launch_points(hipStream_t s, int mode, const float* xy, int64_t begin, int64_t end,
-                   const DevCols& cols, const DevQuery& q, const DevCanvas& c, const DevGrid& g,
-                   const DevPolys& P, const uint32_t* label, DevResult res, int lds_np);
+// mode PT_*; a.start and a.per_block are filled in by the launcher.
+void launch_points(hipStream_t s, int mode, PointArgs a);
 
 }  // namespace rj
