@@ -125,6 +125,8 @@ def main():
     value = n * args.steps / dt / 1e9
     path = "label" if "points_label" in ktot else "canvas"
     log("kernels (ms total, launches):", {k: (round(v[0], 3), v[1]) for k, v in ktot.items()})
+    log("timing line (GLHandler::printTimeStats layout, min over executes, ms):",
+        h.print_time_stats(rj.RASTER))
 
     # Roofline of the point pass: per launch it reads x,y (8 B) and fare
     # (4 B) of every point and, at most once, each 4-byte label of the map.

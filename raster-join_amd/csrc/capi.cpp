@@ -92,7 +92,7 @@ struct rj_ctx {
     // polygons
     PolySet polys;
     bool has_polys = false;
-    DevBuf d_pxy, d_poff;
+    DevBuf d_pxy, d_poff, d_prev;
 
     // points
     rj_table* table = nullptr;
@@ -116,7 +116,12 @@ struct rj_ctx {
     bool raster_valid = false;
     CanvasPlan plan;
     PolyRaster pr;
-    DevBuf d_X, d_Y, d_prev, d_region, d_bands_sum, d_bands_bounds;
+    DevBuf d_pc;                      // regions + bands (one allocation)
+    DevBuf d_XY;                      // snapped vertices: X then Y
+    void* h_stage = nullptr;          // pinned staging buffer for its upload
+    size_t h_stage_bytes = 0;
+    int32_t *p_X = nullptr, *p_Y = nullptr, *p_region = nullptr;
+    Band *p_bands_sum = nullptr, *p_bands_bounds = nullptr;
 
     // CPUJoin grid
     bool grid_valid = false;
@@ -126,15 +131,18 @@ struct rj_ctx {
     // label maps
     bool label_raster_valid = false;
     bool label_raster_conflict = false;
-    DevBuf d_label_raster;
+    DevBuf d_label_raster, d_tiles_raster;
     bool label_hybrid_valid = false;
-    DevBuf d_label_hybrid, d_mask, d_rects;
+    int label_hybrid16 = 0;
+    DevBuf d_label_hybrid, d_tiles_hybrid, d_mask, d_rects;
 
     // canvas
     DevBuf d_cnt, d_chi, d_clo;
 
     // results
     DevBuf d_res;
+    unsigned long long* h_res = nullptr;  // pinned copy of d_res
+    size_t h_res_bytes = 0;
     std::vector<int64_t> counts, hi, bounds;
     std::vector<uint64_t> lo;
     int last_fn = -1;
@@ -165,6 +173,8 @@ struct rj_ctx {
     ~rj_ctx()
     {
         if (own_table) delete table;
+        if (h_stage) (void)hipHostFree(h_stage);
+        if (h_res) (void)hipHostFree(h_res);
         for (auto e : pool) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -287,9 +297,9 @@ struct rj_ctx {
         P.np = polys.np;
         P.xy = d_pxy.as<float>();
         P.off = d_poff.as<int32_t>();
-        P.X = d_X.as<int32_t>();
-        P.Y = d_Y.as<int32_t>();
-        P.region = d_region.as<int32_t>();
+        P.X = p_X;
+        P.Y = p_Y;
+        P.region = p_region;
         return P;
     }
 
@@ -349,14 +359,44 @@ struct rj_ctx {
         rasterize_polys(polys, plan, &pr);
         cur_setup += now_ms() - t0;
         t0 = now_ms();
-        if ((rc = upload(d_X, pr.X.data(), pr.X.size() * 4, "snapped x"))) return rc;
-        if ((rc = upload(d_Y, pr.Y.data(), pr.Y.size() * 4, "snapped y"))) return rc;
-        if ((rc = upload(d_prev, pr.prev.data(), pr.prev.size() * 4, "edge list"))) return rc;
-        if ((rc = upload(d_region, pr.region.data(), pr.region.size() * 4, "regions"))) return rc;
-        if ((rc = upload(d_bands_sum, pr.bands_sum.data(), pr.bands_sum.size() * sizeof(Band), "bands")))
-            return rc;
-        if ((rc = upload(d_bands_bounds, pr.bands_bounds.data(), pr.bands_bounds.size() * sizeof(Band),
-                         "bands")))
+        // regions and bands: one pinned staging buffer, one copy
+        auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+        const size_t parts[3] = {pr.region.size() * 4, pr.bands_sum.size() * sizeof(Band),
+                                 pr.bands_bounds.size() * sizeof(Band)};
+        const void* srcs[3] = {pr.region.data(), pr.bands_sum.data(), pr.bands_bounds.data()};
+        size_t offs[3], total = 0;
+        for (int k = 0; k < 3; k++) {
+            offs[k] = total;
+            total += al(parts[k]);
+        }
+        if (total > h_stage_bytes) {
+            if (h_stage) (void)hipHostFree(h_stage);
+            h_stage = nullptr;
+            hipError_t e = hipHostMalloc(&h_stage, total);
+            if (e != hipSuccess) {
+                h_stage = nullptr;
+                h_stage_bytes = 0;
+                return hip_fail(e, "pinned staging");
+            }
+            h_stage_bytes = total;
+        }
+        for (int k = 0; k < 3; k++)
+            if (parts[k]) std::memcpy((char*)h_stage + offs[k], srcs[k], parts[k]);
+        if ((rc = alloc(d_pc, total, "polygon regions"))) return rc;
+        hipError_t e = hipMemcpyAsync(d_pc.p, h_stage, total, hipMemcpyHostToDevice, stream);
+        if (e != hipSuccess) return hip_fail(e, "polygon regions");
+        char* base = d_pc.as<char>();
+        p_region = (int32_t*)(base + offs[0]);
+        p_bands_sum = (Band*)(base + offs[1]);
+        p_bands_bounds = (Band*)(base + offs[2]);
+        // snapped vertices on the device
+        const int64_t nv = polys.off[polys.np];
+        if ((rc = alloc(d_XY, (size_t)nv * 8, "snapped vertices"))) return rc;
+        p_X = d_XY.as<int32_t>();
+        p_Y = p_X + nv;
+        if ((rc = launch("snap", [&] {
+                 launch_snap(stream, d_pxy.as<float>(), nv, plan.minx, plan.miny, plan.fxs, plan.fys, p_X, p_Y);
+             })))
             return rc;
         cur_polymem += now_ms() - t0;
         raster_valid = true;
@@ -384,24 +424,28 @@ struct rj_ctx {
 
     // Label map of the polygons on the canvas with plain stores. Returns
     // without waiting; the overlap test (aux[0] == aux[1]) is read later.
-    int build_labels(uint32_t* lab)
+    // Label map (plain stores) and its tile map. Returns without waiting; the
+    // overlap test aux[0] == aux[1] (coverage summed over polygons == labeled
+    // pixels) is read later.
+    int build_labels(void* lab, int label16, DevBuf& tiles)
     {
-        const size_t npx = (size_t)plan.W * plan.H;
+        const size_t npx = label_pixels();
+        const size_t words = label16 ? npx / 2 : npx;
         DevCanvas cv = dev_canvas(plan);
         DevPolys P = dev_polys();
         unsigned long long* aux = aux_ptr();
         int rc;
         if ((rc = launch("fill_label", [&] {
-                 launch_fill_u32(stream, lab, npx, kLabelNone);
+                 launch_fill_u32(stream, static_cast<uint32_t*>(lab), words, 0xFFFFFFFFu);
                  launch_fill_u32(stream, (uint32_t*)aux, 8, 0);
              })))
             return rc;
         if ((rc = launch("band_label", [&] {
-                 launch_band(stream, BAND_LABEL, d_bands_sum.as<Band>(), (int)pr.bands_sum.size(), P, cv,
-                             nullptr, nullptr, nullptr, DevResult{}, nullptr, lab, aux, pr.lds_sum, 0);
+                 launch_band(stream, BAND_LABEL, p_bands_sum, (int)pr.bands_sum.size(), P, cv, nullptr, nullptr,
+                             nullptr, DevResult{}, nullptr, lab, label16, aux, pr.lds_sum, 0);
              })))
             return rc;
-        return launch("count_labeled", [&] { launch_count_labeled(stream, lab, npx, aux + 1); });
+        return build_tiles(lab, label16, tiles, aux + 1);
     }
 
     int read_overlap(bool* overlap)
@@ -414,14 +458,33 @@ struct rj_ctx {
         return RJ_OK;
     }
 
+    int label16_ok() const { return polys.np < (int)0xFFF0 ? 1 : 0; }
+    // label maps are stored with rows of LW = W rounded up to 16 pixels
+    size_t label_pixels() const { return (size_t)dev_canvas(plan).LW * plan.H; }
+    int tiles_w() const { return (plan.W + (1 << kTileShift) - 1) >> kTileShift; }
+    size_t ntiles() const
+    {
+        return (size_t)tiles_w() * (size_t)((plan.H + (1 << kTileShift) - 1) >> kTileShift);
+    }
+
+    int build_tiles(const void* lab, int label16, DevBuf& tiles, unsigned long long* labeled)
+    {
+        int rc = alloc(tiles, ntiles() * 4, "label tiles");
+        if (rc) return rc;
+        return launch("tile_labels", [&] {
+            launch_tile_labels(stream, lab, label16, dev_canvas(plan), tiles.as<uint32_t>(), labeled);
+        });
+    }
+
     int ensure_label_raster(bool* fresh)
     {
         *fresh = false;
         if (label_raster_valid && cache) return RJ_OK;
-        const size_t npx = (size_t)plan.W * plan.H;
-        int rc = alloc(d_label_raster, npx * 4, "label map");
+        const size_t npx = label_pixels();
+        const int l16 = label16_ok();
+        int rc = alloc(d_label_raster, npx * (l16 ? 2 : 4) + 16, "label map");
         if (rc) return rc;
-        if ((rc = build_labels(d_label_raster.as<uint32_t>()))) return rc;
+        if ((rc = build_labels(d_label_raster.p, l16, d_tiles_raster))) return rc;
         label_raster_valid = true;
         label_raster_conflict = false;  // confirmed after the point pass
         *fresh = true;
@@ -441,35 +504,38 @@ struct rj_ctx {
     int ensure_label_hybrid()
     {
         if (label_hybrid_valid && cache) return RJ_OK;
-        const size_t npx = (size_t)plan.W * plan.H;
+        const size_t npx = label_pixels();
         const size_t wpr = ((size_t)plan.W + 31) / 32;
-        int rc = alloc(d_label_hybrid, npx * 4, "hybrid label map");
+        int rc = alloc(d_mask, wpr * plan.H * 4, "border mask");
         if (rc) return rc;
-        if ((rc = alloc(d_mask, wpr * plan.H * 4, "border mask"))) return rc;
         std::vector<int32_t> rects;
         deficient_rects(grid, plan, &rects);
         if (!rects.empty() && (rc = upload(d_rects, rects.data(), rects.size() * 4, "deficient cells")))
             return rc;
         DevCanvas cv = dev_canvas(plan);
         DevPolys P = dev_polys();
-        uint32_t* lab = d_label_hybrid.as<uint32_t>();
         uint32_t* mask = d_mask.as<uint32_t>();
-        if ((rc = build_labels(lab))) return rc;
+        label_hybrid16 = label16_ok();
+        if ((rc = alloc(d_label_hybrid, npx * (label_hybrid16 ? 2 : 4) + 16, "hybrid label map"))) return rc;
+        if ((rc = build_labels(d_label_hybrid.p, label_hybrid16, d_tiles_hybrid))) return rc;
         bool overlap = false;
         if ((rc = read_overlap(&overlap))) return rc;
         if (overlap) {
-            // pixels in two or more polygons go to the exact test
-            if ((rc = launch("fill_label", [&] { launch_fill_u32(stream, lab, npx, kLabelNone); }))) return rc;
+            // pixels in two or more polygons go to the exact test (32-bit map, CAS)
+            label_hybrid16 = 0;
+            if ((rc = alloc(d_label_hybrid, npx * 4 + 16, "hybrid label map"))) return rc;
+            uint32_t* lab32 = d_label_hybrid.as<uint32_t>();
+            if ((rc = launch("fill_label", [&] { launch_fill_u32(stream, lab32, npx, kLabelNone); }))) return rc;
             if ((rc = launch("band_label_atomic", [&] {
-                     launch_band(stream, BAND_LABEL_ATOMIC, d_bands_sum.as<Band>(), (int)pr.bands_sum.size(), P,
-                                 cv, nullptr, nullptr, nullptr, DevResult{}, nullptr, lab, nullptr, pr.lds_sum, 0);
+                     launch_band(stream, BAND_LABEL_ATOMIC, p_bands_sum, (int)pr.bands_sum.size(), P, cv, nullptr,
+                                 nullptr, nullptr, DevResult{}, nullptr, lab32, 0, nullptr, pr.lds_sum, 0);
                  })))
                 return rc;
         }
+        void* lab = d_label_hybrid.p;
         if ((rc = launch("fill_mask", [&] { launch_fill_u32(stream, mask, wpr * plan.H, 0); }))) return rc;
         if ((rc = launch("touch", [&] {
-                 launch_touch(stream, d_X.as<int32_t>(), d_Y.as<int32_t>(), d_prev.as<int32_t>(),
-                              (int64_t)pr.X.size(), cv, mask);
+                 launch_touch(stream, p_X, p_Y, d_prev.as<int32_t>(), (int64_t)polys.off[polys.np], cv, mask);
              })))
             return rc;
         if (!rects.empty())
@@ -479,7 +545,9 @@ struct rj_ctx {
                 return rc;
         const int r = hybrid_radius();
         if (r > 31) return fail(RJ_ERR_UNSUPPORTED, "accuracy too fine for the float margin of the exact test");
-        if ((rc = launch("dilate", [&] { launch_dilate_to_label(stream, mask, cv, r, lab); }))) return rc;
+        if ((rc = launch("dilate", [&] { launch_dilate_to_label(stream, mask, cv, r, lab, label_hybrid16); })))
+            return rc;
+        if ((rc = build_tiles(lab, label_hybrid16, d_tiles_hybrid, nullptr))) return rc;
         label_hybrid_valid = true;
         return RJ_OK;
     }
@@ -534,6 +602,7 @@ int rj_ctx::execute(int fn, int64_t* out_counts, double* out_sums, int32_t* lega
     };
     if ((rc = clear_results())) return rc;
 
+    bool check_overlap = false;
     PointArgs pa;
     std::memset(&pa, 0, sizeof(pa));
     pa.xy = static_cast<const float*>(table->xy);
@@ -566,11 +635,17 @@ int rj_ctx::execute(int fn, int64_t* out_counts, double* out_sums, int32_t* lega
         pa.cv = dev_canvas(plan);
         pa.g = dev_grid();
         pa.P = dev_polys();
-        pa.label = d_label_hybrid.as<uint32_t>();
+        pa.label = d_label_hybrid.p;
+        pa.label16 = label_hybrid16;
+        pa.tiles = d_tiles_hybrid.as<uint32_t>();
+        pa.tiles_w = tiles_w();
         if ((rc = launch("points_hybrid", [&] { launch_points(stream, PT_LABEL_HYBRID, pa); }))) return rc;
     } else if (fn == RJ_RASTER) {
         if ((rc = ensure_raster())) return rc;
-        bool use_label = raster_path != 1;
+        const bool known_overlap = label_raster_valid && cache && label_raster_conflict;
+        if (raster_path == 2 && known_overlap)
+            return fail(RJ_ERR_UNSUPPORTED, "polygons overlap: the label-map path needs disjoint polygons");
+        bool use_label = raster_path != 1 && !known_overlap;
         bool fresh = false;
         if (use_label) {
             if ((rc = ensure_label_raster(&fresh))) return rc;
@@ -583,28 +658,20 @@ int rj_ctx::execute(int fn, int64_t* out_counts, double* out_sums, int32_t* lega
         if (use_label) {
             pa.cv = dev_canvas(plan);
             pa.P = dev_polys();
-            pa.label = d_label_raster.as<uint32_t>();
+            pa.label = d_label_raster.p;
+            pa.label16 = label16_ok();
+            pa.tiles = d_tiles_raster.as<uint32_t>();
+            pa.tiles_w = tiles_w();
             if ((rc = launch("points_label", [&] { launch_points(stream, PT_LABEL, pa); }))) return rc;
-            if (fresh) {
-                // the overlap test of the freshly built map, read with the results
-                bool overlap = false;
-                if ((rc = read_overlap(&overlap))) return rc;
-                if (overlap) {
-                    label_raster_conflict = true;
-                    if (raster_path == 2)
-                        return fail(RJ_ERR_UNSUPPORTED, "polygons overlap: the label-map path needs disjoint polygons");
-                    if ((rc = clear_results())) return rc;
-                    use_label = false;
-                }
-            }
+            check_overlap = fresh;
         }
         if (!use_label) {
             if ((rc = canvas_pass(avg))) return rc;
             DevCanvas cv = dev_canvas(plan);
             if ((rc = launch("band_sum", [&] {
-                     launch_band(stream, BAND_SUM, d_bands_sum.as<Band>(), (int)pr.bands_sum.size(), dev_polys(),
+                     launch_band(stream, BAND_SUM, p_bands_sum, (int)pr.bands_sum.size(), dev_polys(),
                                  cv, d_cnt.as<uint32_t>(), d_chi.as<long long>(), d_clo.as<unsigned long long>(),
-                                 res, nullptr, nullptr, nullptr, pr.lds_sum, avg ? 1 : 0);
+                                 res, nullptr, nullptr, 0, nullptr, pr.lds_sum, avg ? 1 : 0);
                  })))
                 return rc;
         }
@@ -614,24 +681,51 @@ int rj_ctx::execute(int fn, int64_t* out_counts, double* out_sums, int32_t* lega
         DevCanvas cv = dev_canvas(plan);
         if ((rc = canvas_pass(false))) return rc;
         if ((rc = launch("band_bounds", [&] {
-                 launch_band(stream, BAND_BOUNDS, d_bands_bounds.as<Band>(), (int)pr.bands_bounds.size(),
-                             dev_polys(), cv, d_cnt.as<uint32_t>(), nullptr, nullptr, res, bnd, nullptr, nullptr,
-                             pr.lds_bounds, 0);
+                 launch_band(stream, BAND_BOUNDS, p_bands_bounds, (int)pr.bands_bounds.size(),
+                             dev_polys(), cv, d_cnt.as<uint32_t>(), nullptr, nullptr, res, bnd, nullptr, 0,
+                             nullptr, pr.lds_bounds, 0);
              })))
             return rc;
     }
 
-    counts.assign(np, 0);
-    hi.assign(np, 0);
-    lo.assign(np, 0);
-    bounds.assign(4 * (size_t)np, 0);
-    hipError_t e = hipMemcpyAsync(counts.data(), res_base, (size_t)np * 8, hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(hi.data(), res_base + np, (size_t)np * 8, hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(lo.data(), res_base + 2 * (size_t)np, (size_t)np * 8, hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess && fn == RJ_ERRORBOUNDS)
-        e = hipMemcpyAsync(bounds.data(), bnd, (size_t)np * 32, hipMemcpyDeviceToHost, stream);
+    // one pinned copy of [results | bounds | aux]
+    const size_t res_bytes = ((size_t)np * 7 + 4) * 8;
+    if (res_bytes > h_res_bytes) {
+        if (h_res) (void)hipHostFree(h_res);
+        h_res = nullptr;
+        h_res_bytes = 0;
+        hipError_t e = hipHostMalloc((void**)&h_res, res_bytes);
+        if (e != hipSuccess) {
+            h_res = nullptr;
+            return hip_fail(e, "pinned results");
+        }
+        h_res_bytes = res_bytes;
+    }
+    hipError_t e = hipMemcpyAsync(h_res, res_base, res_bytes, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "execute");
+    if (check_overlap && h_res[7 * (size_t)np] != h_res[7 * (size_t)np + 1]) {
+        // the fresh label map has pixels in two polygons: redo on the canvas
+        label_raster_conflict = true;
+        if (raster_path == 2)
+            return fail(RJ_ERR_UNSUPPORTED, "polygons overlap: the label-map path needs disjoint polygons");
+        if ((rc = clear_results())) return rc;
+        if ((rc = canvas_pass(avg))) return rc;
+        DevCanvas cv = dev_canvas(plan);
+        if ((rc = launch("band_sum", [&] {
+                 launch_band(stream, BAND_SUM, p_bands_sum, (int)pr.bands_sum.size(), dev_polys(), cv,
+                             d_cnt.as<uint32_t>(), d_chi.as<long long>(), d_clo.as<unsigned long long>(), res,
+                             nullptr, nullptr, 0, nullptr, pr.lds_sum, avg ? 1 : 0);
+             })))
+            return rc;
+        e = hipMemcpyAsync(h_res, res_base, res_bytes, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return hip_fail(e, "execute");
+    }
+    counts.assign((const int64_t*)h_res, (const int64_t*)h_res + np);
+    hi.assign((const int64_t*)h_res + np, (const int64_t*)h_res + 2 * np);
+    lo.assign(h_res + 2 * np, h_res + 3 * np);
+    bounds.assign((const int64_t*)h_res + 3 * np, (const int64_t*)h_res + 7 * np);
     last_fn = fn;
 
     for (int p = 0; p < np; p++) {
@@ -738,9 +832,15 @@ int rj_set_polygons(rj_ctx* ctx, const float* xy, const int32_t* off, int32_t np
     for (int d = 0; d < 4; d++) ps.bboxf[d] = (float)ps.bbox[d];
     if (!(ps.bbox[2] > ps.bbox[0]) || !(ps.bbox[3] > ps.bbox[1])) return ctx->fail(RJ_ERR_ARG, "degenerate bbox");
     double t0 = now_ms();
+    ps.derive();
     ctx->polys = std::move(ps);
     int rc = ctx->upload(ctx->d_pxy, ctx->polys.xy.data(), ctx->polys.xy.size() * 4, "polygons");
     if (!rc) rc = ctx->upload(ctx->d_poff, ctx->polys.off.data(), ctx->polys.off.size() * 4, "polygon offsets");
+    if (!rc) rc = ctx->upload(ctx->d_prev, ctx->polys.prev.data(), ctx->polys.prev.size() * 4, "edge list");
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);  // the host vectors may move later
+        if (e != hipSuccess) rc = ctx->hip_fail(e, "polygon upload");
+    }
     if (rc) return rc;
     ctx->cur_polymem = now_ms() - t0;
     ctx->has_polys = true;

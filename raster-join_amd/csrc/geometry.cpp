@@ -56,6 +56,7 @@ DevCanvas dev_canvas(const CanvasPlan& p)
 {
     DevCanvas c;
     c.W = p.W; c.H = p.H;
+    c.LW = (p.W + 15) & ~15;
     c.minx = p.minx_f; c.miny = p.miny_f; c.maxx = p.maxx_f; c.maxy = p.maxy_f;
     c.sx = p.sx_f; c.sy = p.sy_f;
     return c;
@@ -72,33 +73,39 @@ static int64_t floor_div(int64_t a, int64_t b)
     return q;
 }
 
+void PolySet::derive()
+{
+    pbox.assign(4 * (size_t)np, 0.0f);
+    prev.resize(off.empty() ? 0 : off[np]);
+    for (int p = 0; p < np; p++) {
+        const int v0 = off[p], v1 = off[p + 1];
+        float mnx = xy[2 * v0], mxx = mnx, mny = xy[2 * v0 + 1], mxy = mny;
+        for (int k = v0; k < v1; k++) {
+            prev[k] = (k == v0) ? v1 - 1 : k - 1;
+            mnx = std::min(mnx, xy[2 * k]);
+            mxx = std::max(mxx, xy[2 * k]);
+            mny = std::min(mny, xy[2 * k + 1]);
+            mxy = std::max(mxy, xy[2 * k + 1]);
+        }
+        pbox[4 * p] = mnx; pbox[4 * p + 1] = mny; pbox[4 * p + 2] = mxx; pbox[4 * p + 3] = mxy;
+    }
+}
+
+int64_t snap_x(const CanvasPlan& plan, float x) { return std::llrint(((double)x - plan.minx) * plan.fxs); }
+int64_t snap_y(const CanvasPlan& plan, float y) { return std::llrint(((double)y - plan.miny) * plan.fys); }
+
 void rasterize_polys(const PolySet& ps, const CanvasPlan& plan, PolyRaster* out)
 {
-    const int64_t nv = ps.off.empty() ? 0 : ps.off[ps.np];
-    out->X.resize(nv);
-    out->Y.resize(nv);
-    out->prev.resize(nv);
     out->region.assign(4 * (size_t)ps.np, 0);
     out->bands_sum.clear();
     out->bands_bounds.clear();
     out->lds_sum = out->lds_bounds = 0;
     out->bounds_ok = true;
-    for (int64_t k = 0; k < nv; k++) {
-        out->X[k] = (int32_t)std::llrint(((double)ps.xy[2 * k] - plan.minx) * plan.fxs);
-        out->Y[k] = (int32_t)std::llrint(((double)ps.xy[2 * k + 1] - plan.miny) * plan.fys);
-    }
     const int sum_words = 12288;    // LDS words per band, sum / label kernels
     const int bound_words = 16384;  // LDS words per band, error-bounds kernel
     for (int p = 0; p < ps.np; p++) {
-        const int v0 = ps.off[p], v1 = ps.off[p + 1];
-        int64_t mnx = out->X[v0], mxx = mnx, mny = out->Y[v0], mxy = mny;
-        for (int k = v0; k < v1; k++) {
-            out->prev[k] = (k == v0) ? v1 - 1 : k - 1;
-            mnx = std::min<int64_t>(mnx, out->X[k]);
-            mxx = std::max<int64_t>(mxx, out->X[k]);
-            mny = std::min<int64_t>(mny, out->Y[k]);
-            mxy = std::max<int64_t>(mxy, out->Y[k]);
-        }
+        const int64_t mnx = snap_x(plan, ps.pbox[4 * p]), mny = snap_y(plan, ps.pbox[4 * p + 1]);
+        const int64_t mxx = snap_x(plan, ps.pbox[4 * p + 2]), mxy = snap_y(plan, ps.pbox[4 * p + 3]);
         // Cells whose center can be inside, plus one ring for the dilation of
         // the error bounds.
         int64_t c0 = std::max<int64_t>(0, floor_div(mnx, kSub) - 2);

@@ -60,19 +60,23 @@ __device__ __forceinline__ bool cmp_bits(int type, int op, uint32_t a, uint32_t 
 }
 
 // Exact fixed-point aggregate value of a 32-bit pattern: hi = floor(v),
-// lo = frac(v) * 2^32 (integer columns: hi = value, lo = 0).
+// lo = frac(v) * 2^32 (integer columns: hi = value, lo = 0). The fraction
+// times 2^32 is below 2^32 - 255, so lo always fits 32 bits.
 __device__ __forceinline__ void agg_bits(int type, uint32_t a, unsigned long long& hi, unsigned long long& lo)
 {
     lo = 0;
     if (type == RJ_ATTR_FLOAT) {
         const float f = __uint_as_float(a);
-        if (!(fabsf(f) < 4.611686018427387904e18f)) {
-            hi = 0;
+        const float fl = floorf(f);
+        if (fabsf(fl) < 2147483648.0f) {
+            hi = (unsigned long long)(long long)(int32_t)fl;  // 32-bit conversion suffices
+        } else if (fabsf(f) < 4.611686018427387904e18f) {
+            hi = (unsigned long long)(long long)fl;
+        } else {
+            hi = 0;  // non-finite or |v| >= 2^62 contributes nothing
             return;
         }
-        const float fl = floorf(f);
-        hi = (unsigned long long)(long long)fl;
-        lo = (unsigned long long)((f - fl) * 4294967296.0f);
+        lo = (uint32_t)((f - fl) * 4294967296.0f);
     } else if (type == RJ_ATTR_INT) {
         hi = (unsigned long long)(long long)(int32_t)a;
     } else {
@@ -113,14 +117,19 @@ __device__ __forceinline__ void load_rows(const DevCols& C, int col, int64_t i0,
     }
 }
 
-// The pixel formula shared by every kernel (and the host oracle).
-__device__ __forceinline__ int pixel_of(const DevCanvas& c, float x, float y)
+// The pixel formula shared by every kernel (and the host oracle): canvas
+// membership by float comparison with the bbox ([min, max) per axis), then
+// int((v - min) * s) clamped to the last pixel. Returns -1 outside.
+__device__ __forceinline__ int pixel_xy(const DevCanvas& c, float x, float y, int& ix, int& iy)
 {
-    if (!(x >= c.minx && x < c.maxx && y >= c.miny && y < c.maxy)) return -1;
-    float fx = (x - c.minx) * c.sx;
-    float fy = (y - c.miny) * c.sy;
-    int ix = min((int)fx, c.W - 1);
-    int iy = min((int)fy, c.H - 1);
+    if (!(x >= c.minx && x < c.maxx && y >= c.miny && y < c.maxy)) {
+        ix = iy = 0;
+        return -1;
+    }
+    const float fx = (x - c.minx) * c.sx;
+    const float fy = (y - c.miny) * c.sy;
+    ix = min((int)fx, c.W - 1);
+    iy = min((int)fy, c.H - 1);
     return iy * c.W + ix;
 }
 
@@ -181,8 +190,19 @@ __device__ __forceinline__ void block_add4(unsigned long long a0, unsigned long 
 }
 
 // ---------------------------------------------------------------------------
-// fill
+// fill, snap
 // ---------------------------------------------------------------------------
+
+// Vertices to the fixed-point canvas grid: rint(((double)v - min) * scale),
+// round half to even, the same IEEE double operations as the host.
+__global__ void k_snap(const float* __restrict__ xy, int64_t nv, double minx, double miny, double fxs,
+                       double fys, int32_t* __restrict__ X, int32_t* __restrict__ Y)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nv) return;
+    X[k] = (int32_t)rint(((double)xy[2 * k] - minx) * fxs);
+    Y[k] = (int32_t)rint(((double)xy[2 * k + 1] - miny) * fys);
+}
 
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v)
 {
@@ -291,8 +311,8 @@ __global__ __launch_bounds__(kBlock) void k_band(const Band* __restrict__ bands,
                                                  const uint32_t* __restrict__ cnt,
                                                  const unsigned long long* __restrict__ chi,
                                                  const unsigned long long* __restrict__ clo, DevResult res,
-                                                 unsigned long long* __restrict__ bounds, uint32_t* label,
-                                                 unsigned long long* aux, int with_sums)
+                                                 unsigned long long* __restrict__ bounds, void* label,
+                                                 int label16, unsigned long long* aux, int with_sums)
 {
     extern __shared__ uint32_t lds[];
     const Band bd = bands[blockIdx.x];
@@ -328,53 +348,61 @@ __global__ __launch_bounds__(kBlock) void k_band(const Band* __restrict__ bands,
 
     unsigned long long s_cnt = 0, s_hi = 0, s_lo = 0;
     unsigned long long s_fn = 0, s_fp = 0, s_fn1 = 0, s_fp1 = 0;
-    const int total = nr * bw;
-    for (int t = threadIdx.x; t < total; t += blockDim.x) {
-        const int row = t / bw, col = t - row * bw;
-        const int inside = bit_at(tog + row * wpr, col);
-        const int64_t pix = (int64_t)(r0 + row) * cv.W + (c0 + col);
-        if (MODE == BAND_SUM) {
-            if (inside) {
-                s_cnt += cnt[pix];
-                if (with_sums) {
-                    s_hi += chi[pix];
-                    s_lo += clo[pix];
+    // Threads cover 2^lg columns of (256 >> lg) rows at a time: no division.
+    const int lg = bw >= kBlock ? 8 : (32 - __clz(bw - 1));
+    const int colstep = 1 << lg;
+    for (int row = threadIdx.x >> lg; row < nr; row += (kBlock >> lg)) {
+        const uint32_t* trow = tog + row * wpr;
+        const int64_t prow = (int64_t)(r0 + row) * cv.W + c0;
+        const int64_t lrow = (int64_t)(r0 + row) * cv.LW + c0;
+        for (int col = threadIdx.x & (colstep - 1); col < bw; col += colstep) {
+            const int inside = bit_at(trow, col);
+            const int64_t pix = prow + col;
+            if (MODE == BAND_SUM) {
+                if (inside) {
+                    s_cnt += cnt[pix];
+                    if (with_sums) {
+                        s_hi += chi[pix];
+                        s_lo += clo[pix];
+                    }
                 }
-            }
-        } else if (MODE == BAND_LABEL) {
-            if (inside) {
-                label[pix] = (uint32_t)p;
-                s_cnt++;
-            }
-        } else if (MODE == BAND_LABEL_ATOMIC) {
-            if (inside) {
-                const uint32_t old = atomicCAS(&label[pix], kLabelNone, (uint32_t)p);
-                if (old != kLabelNone) atomicExch(&label[pix], kLabelBorder);
-            }
-        } else {
-            const uint32_t val = cnt[pix];
-            if (inside) s_cnt += val;
-            if (val != 0) {
-                int tb = 0;
-                for (int dr = 0; dr < 3 && !tb; dr++) {
-                    const uint32_t* trow = tch + (row + dr) * wpr;
-                    tb |= bit_at(trow, col);
-                    if (col > 0) tb |= bit_at(trow, col - 1);
-                    if (col + 1 < bw) tb |= bit_at(trow, col + 1);
+            } else if (MODE == BAND_LABEL) {
+                if (inside) {
+                    if (label16) static_cast<uint16_t*>(label)[lrow + col] = (uint16_t)p;
+                    else static_cast<uint32_t*>(label)[lrow + col] = (uint32_t)p;
+                    s_cnt++;
                 }
-                if (tb) {
-                    int k16 = 0;
-                    const int bit = (4 * col) & 31, word = (4 * col) >> 5;
-                    for (int sy = 0; sy < 4; sy++)
-                        k16 += __popc((sub[(4 * row + sy) * (4 * wpr) + word] >> bit) & 0xFu);
-                    const int m = min(k16, 16 - k16);
-                    const unsigned long long add = ((unsigned long long)m * val + 15ull) / 16ull;
-                    if (inside) {
-                        s_fp += add;
-                        s_fp1 += val;
-                    } else {
-                        s_fn += add;
-                        s_fn1 += val;
+            } else if (MODE == BAND_LABEL_ATOMIC) {
+                if (inside) {
+                    uint32_t* l32 = static_cast<uint32_t*>(label);
+                    const uint32_t old = atomicCAS(&l32[lrow + col], kLabelNone, (uint32_t)p);
+                    if (old != kLabelNone) atomicExch(&l32[lrow + col], kLabelBorder);
+                }
+            } else {
+                const uint32_t val = cnt[pix];
+                if (inside) s_cnt += val;
+                if (val != 0) {
+                    int tb = 0;
+                    for (int dr = 0; dr < 3 && !tb; dr++) {
+                        const uint32_t* tr = tch + (row + dr) * wpr;
+                        tb |= bit_at(tr, col);
+                        if (col > 0) tb |= bit_at(tr, col - 1);
+                        if (col + 1 < bw) tb |= bit_at(tr, col + 1);
+                    }
+                    if (tb) {
+                        int k16 = 0;
+                        const int bit = (4 * col) & 31, word = (4 * col) >> 5;
+                        for (int sy = 0; sy < 4; sy++)
+                            k16 += __popc((sub[(4 * row + sy) * (4 * wpr) + word] >> bit) & 0xFu);
+                        const int m = min(k16, 16 - k16);
+                        const unsigned long long add = ((unsigned long long)m * val + 15ull) / 16ull;
+                        if (inside) {
+                            s_fp += add;
+                            s_fp1 += val;
+                        } else {
+                            s_fn += add;
+                            s_fn1 += val;
+                        }
                     }
                 }
             }
@@ -426,8 +454,8 @@ __global__ void k_mark_rects(const int32_t* __restrict__ rects, int nrects, int 
 }
 
 // Dilate the mask by r pixels (Chebyshev) and mark the result in the labels.
-__global__ void k_dilate_to_label(const uint32_t* __restrict__ mask, int W, int H, int wpr, int r,
-                                  uint32_t* label)
+__global__ void k_dilate_to_label(const uint32_t* __restrict__ mask, int W, int H, int LW, int wpr, int r,
+                                  void* label, int label16)
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)wpr * H) return;
@@ -447,11 +475,14 @@ __global__ void k_dilate_to_label(const uint32_t* __restrict__ mask, int W, int 
         d |= (m >> s) | (mr << (32 - s));
     }
     if (!d) return;
-    uint32_t* lrow = label + (int64_t)row * W;
+    const int64_t rowpix = (int64_t)row * LW;
     for (int b = 0; b < 32; b++) {
-        int col = w * 32 + b;
+        const int col = w * 32 + b;
         if (col >= W) break;
-        if ((d >> b) & 1u) lrow[col] = kLabelBorder;
+        if ((d >> b) & 1u) {
+            if (label16) static_cast<uint16_t*>(label)[rowpix + col] = (uint16_t)kLabelBorder;
+            else static_cast<uint32_t*>(label)[rowpix + col] = kLabelBorder;
+        }
     }
 }
 
@@ -459,14 +490,71 @@ __global__ void k_dilate_to_label(const uint32_t* __restrict__ mask, int W, int 
 // point pass: index join, label-map raster join, hybrid join, canvas splat
 // ---------------------------------------------------------------------------
 
-// Count of labeled pixels (the other half of the overlap test).
-__global__ void k_count_labeled(const uint32_t* __restrict__ label, size_t n, unsigned long long* out)
+__device__ __forceinline__ uint32_t label_at(const void* label, int label16, int64_t pix)
 {
-    unsigned long long c = 0;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) c += (label[i] != kLabelNone);
-    block_add4(c, 0, 0, 0, out, nullptr, nullptr, nullptr);
+    if (label16) {
+        const uint32_t v16 = static_cast<const uint16_t*>(label)[pix];
+        return v16 >= 0xFFF0u ? (0xFFFF0000u | v16) : v16;  // top codes map onto none/border
+    }
+    return static_cast<const uint32_t*>(label)[pix];
+}
+
+// Tile map of the label map: per 16x16 tile its single label, or kLabelMixed,
+// plus the count of labeled pixels (the overlap test). Threads read 16-byte
+// vectors (8 or 4 labels) down a 16-row strip and the 2 or 4 threads of one
+// tile combine with shuffles. The map is small enough to stay in L2 during the
+// point pass.
+__global__ void k_tile_labels(const void* __restrict__ label, int label16, int LW, int H, int TW,
+                              uint32_t* __restrict__ tiles, unsigned long long* __restrict__ labeled)
+{
+    const int per = label16 ? 8 : 4;  // labels per vector
+    const int tpt = 16 / per;         // threads per tile
+    const int nvec = LW / per;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ty = blockIdx.y;
+    const bool active = v < nvec;
+    uint32_t first = 0;
+    int uni = 1;
+    unsigned long long nlab = 0;
+    if (active) {
+        const int y0 = ty << kTileShift;
+        const size_t row_bytes = (size_t)LW * (label16 ? 2 : 4);
+        uint4 w[1 << kTileShift];
+#pragma unroll
+        for (int k = 0; k < (1 << kTileShift); k++)
+            if (y0 + k < H)
+                w[k] = reinterpret_cast<const uint4*>(static_cast<const char*>(label) + (size_t)(y0 + k) * row_bytes)[v];
+#pragma unroll
+        for (int k = 0; k < (1 << kTileShift); k++) {
+            if (y0 + k >= H) break;
+            const uint32_t x[4] = {w[k].x, w[k].y, w[k].z, w[k].w};
+            if (label16) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const uint32_t a = x[j] & 0xFFFFu, b = x[j] >> 16;
+                    const uint32_t va = a >= 0xFFF0u ? (0xFFFF0000u | a) : a;
+                    const uint32_t vb = b >= 0xFFF0u ? (0xFFFF0000u | b) : b;
+                    if (k == 0 && j == 0) first = va;
+                    uni &= (va == first) & (vb == first);
+                    nlab += (va != kLabelNone) + (vb != kLabelNone);
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    if (k == 0 && j == 0) first = x[j];
+                    uni &= (x[j] == first);
+                    nlab += (x[j] != kLabelNone);
+                }
+            }
+        }
+    }
+    for (int d = 1; d < tpt; d <<= 1) {
+        const uint32_t of = __shfl_xor(first, d, 64);
+        const int ou = __shfl_xor(uni, d, 64);
+        uni = uni && ou && (of == first);
+    }
+    if (active && (v % tpt) == 0) tiles[(size_t)ty * TW + v / tpt] = uni ? first : kLabelMixed;
+    if (labeled) block_add4(nlab, 0, 0, 0, labeled, nullptr, nullptr, nullptr);
 }
 
 struct Acc {
@@ -499,7 +587,7 @@ __device__ __forceinline__ void acc_add(const Acc& A, int key, unsigned long lon
 // contains the point; further containing polygons are added directly.
 template <bool AVG>
 __device__ __forceinline__ int exact_first(const DevGrid& g, const DevPolys& P, const Acc& A, float x,
-                                           float y, unsigned long long hi, unsigned long long lo)
+                                           float y, int agg_t, uint32_t vraw)
 {
     const int cell = grid_cell(g, x, y);
     if (cell < 0) return -1;
@@ -509,8 +597,13 @@ __device__ __forceinline__ int exact_first(const DevGrid& g, const DevPolys& P, 
         const int p = g.cand[k];
         const int v0 = P.off[p];
         if (pip_f32(reinterpret_cast<const float2*>(P.xy) + v0, P.off[p + 1] - v0, x, y)) {
-            if (first < 0) first = p;
-            else acc_add(A, p, 1ull, hi, lo, AVG);
+            if (first < 0) {
+                first = p;
+            } else {
+                unsigned long long h = 0, l = 0;
+                if (AVG) agg_bits(agg_t, vraw, h, l);
+                acc_add(A, p, 1ull, h, l, AVG);
+            }
         }
     }
     return first;
@@ -540,7 +633,9 @@ __global__ __launch_bounds__(kBlock) void k_points(PointArgs a)
     const int64_t b1 = min(a.end, b0 + a.per_block);
     const int64_t lo_row = max(a.begin, b0);
     int ck = -1;
-    unsigned long long cc = 0, ch = 0, cl = 0;
+    uint32_t cc = 0;
+    unsigned long long ch = 0, cl = 0;
+    const int agg_t = AVG ? a.cols.type[q.slot_col[q.agg_slot < 0 ? 0 : q.agg_slot]] : RJ_ATTR_UINT;
     for (int64_t base = b0; base < b1; base += (int64_t)blockDim.x * kRows) {
         const int64_t i0 = base + (int64_t)threadIdx.x * kRows;
         if (i0 >= b1) continue;
@@ -565,9 +660,10 @@ __global__ __launch_bounds__(kBlock) void k_points(PointArgs a)
         bool ok[kRows];
 #pragma unroll
         for (int r = 0; r < kRows; r++) ok[r] = (i0 + r >= lo_row) && (i0 + r < b1);
-        unsigned long long vh[kRows], vl[kRows];
+        // raw 32-bit aggregate values; decoded to fixed point only when added
+        uint32_t vraw[kRows];
 #pragma unroll
-        for (int r = 0; r < kRows; r++) vh[r] = vl[r] = 0;
+        for (int r = 0; r < kRows; r++) vraw[r] = 0;
 #pragma unroll
         for (int s = 0; s < kMaxSlots; s++) {
             if (s >= q.nslots) break;
@@ -584,30 +680,43 @@ __global__ __launch_bounds__(kBlock) void k_points(PointArgs a)
             }
             if (AVG && s == q.agg_slot) {
 #pragma unroll
-                for (int r = 0; r < kRows; r++) agg_bits(t, v[r], vh[r], vl[r]);
+                for (int r = 0; r < kRows; r++) vraw[r] = v[r];
             }
         }
         int key[kRows];
         if (MODE == PT_INDEX) {
 #pragma unroll
             for (int r = 0; r < kRows; r++)
-                key[r] = ok[r] ? exact_first<AVG>(a.g, a.P, A, px[r], py[r], vh[r], vl[r]) : -1;
+                key[r] = ok[r] ? exact_first<AVG>(a.g, a.P, A, px[r], py[r], agg_t, vraw[r]) : -1;
         } else {
-            int pix[kRows];
+            int pix[kRows], ixs[kRows], iys[kRows];
 #pragma unroll
-            for (int r = 0; r < kRows; r++) pix[r] = ok[r] ? pixel_of(a.cv, px[r], py[r]) : -1;
+            for (int r = 0; r < kRows; r++) {
+                pix[r] = pixel_xy(a.cv, px[r], py[r], ixs[r], iys[r]);
+                if (!ok[r]) pix[r] = -1;
+            }
             if (MODE == PT_SPLAT) {
 #pragma unroll
                 for (int r = 0; r < kRows; r++) key[r] = pix[r];
             } else {
                 uint32_t lab[kRows];
+                // the tile map answers for whole tiles; mixed tiles read the pixel
 #pragma unroll
-                for (int r = 0; r < kRows; r++) lab[r] = pix[r] >= 0 ? a.label[pix[r]] : kLabelNone;
+                for (int r = 0; r < kRows; r++) {
+                    if (pix[r] < 0) {
+                        lab[r] = kLabelNone;
+                    } else {
+                        lab[r] = a.tiles[(iys[r] >> kTileShift) * a.tiles_w + (ixs[r] >> kTileShift)];
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < kRows; r++)
+                    if (lab[r] == kLabelMixed) lab[r] = label_at(a.label, a.label16, (int64_t)iys[r] * a.cv.LW + ixs[r]);
 #pragma unroll
                 for (int r = 0; r < kRows; r++) {
                     key[r] = lab[r] < kLabelMaxId ? (int)lab[r] : -1;
                     if (MODE == PT_LABEL_HYBRID && lab[r] == kLabelBorder)
-                        key[r] = exact_first<AVG>(a.g, a.P, A, px[r], py[r], vh[r], vl[r]);
+                        key[r] = exact_first<AVG>(a.g, a.P, A, px[r], py[r], agg_t, vraw[r]);
                 }
             }
         }
@@ -632,8 +741,10 @@ __global__ __launch_bounds__(kBlock) void k_points(PointArgs a)
             }
             cc++;
             if (AVG) {
-                ch += vh[r];
-                cl += vl[r];
+                unsigned long long h, l;
+                agg_bits(agg_t, vraw[r], h, l);
+                ch += h;
+                cl += l;
             }
         }
     }
@@ -675,6 +786,14 @@ inline int blocks_for(int64_t n, int64_t per, int cap)
 // launchers
 // ---------------------------------------------------------------------------
 
+void launch_snap(hipStream_t s, const float* xy, int64_t nv, double minx, double miny, double fxs, double fys,
+                 int32_t* X, int32_t* Y)
+{
+    if (nv <= 0) return;
+    hipLaunchKernelGGL(k_snap, dim3((unsigned)((nv + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, xy, nv, minx,
+                       miny, fxs, fys, X, Y);
+}
+
 void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v)
 {
     if (n == 0) return;
@@ -696,24 +815,24 @@ static void chunking(int64_t n, int cap, int* blocks, int64_t* per_block)
 
 void launch_band(hipStream_t s, int mode, const Band* bands, int nbands, const DevPolys& P,
                  const DevCanvas& c, const uint32_t* cnt, const long long* chi,
-                 const unsigned long long* clo, DevResult res, long long* bounds, uint32_t* label,
-                 unsigned long long* aux, size_t lds_bytes, int with_sums)
+                 const unsigned long long* clo, DevResult res, long long* bounds, void* label,
+                 int label16, unsigned long long* aux, size_t lds_bytes, int with_sums)
 {
     if (nbands <= 0) return;
     const unsigned long long* hi = (const unsigned long long*)chi;
     unsigned long long* bd = (unsigned long long*)bounds;
     if (mode == BAND_SUM)
         hipLaunchKernelGGL(k_band<BAND_SUM>, dim3(nbands), dim3(kBlock), lds_bytes, s, bands, P, c, cnt,
-                           hi, clo, res, bd, label, aux, with_sums);
+                           hi, clo, res, bd, label, label16, aux, with_sums);
     else if (mode == BAND_LABEL)
         hipLaunchKernelGGL(k_band<BAND_LABEL>, dim3(nbands), dim3(kBlock), lds_bytes, s, bands, P, c,
-                           cnt, hi, clo, res, bd, label, aux, with_sums);
+                           cnt, hi, clo, res, bd, label, label16, aux, with_sums);
     else if (mode == BAND_LABEL_ATOMIC)
         hipLaunchKernelGGL(k_band<BAND_LABEL_ATOMIC>, dim3(nbands), dim3(kBlock), lds_bytes, s, bands, P, c,
-                           cnt, hi, clo, res, bd, label, aux, with_sums);
+                           cnt, hi, clo, res, bd, label, label16, aux, with_sums);
     else
         hipLaunchKernelGGL(k_band<BAND_BOUNDS>, dim3(nbands), dim3(kBlock), lds_bytes, s, bands, P, c,
-                           cnt, hi, clo, res, bd, label, aux, with_sums);
+                           cnt, hi, clo, res, bd, label, label16, aux, with_sums);
 }
 
 void launch_touch(hipStream_t s, const int32_t* X, const int32_t* Y, const int32_t* prev,
@@ -736,19 +855,22 @@ void launch_mark_rects(hipStream_t s, const int32_t* rects, int nrects, const De
 }
 
 void launch_dilate_to_label(hipStream_t s, const uint32_t* mask_bits, const DevCanvas& c, int r,
-                            uint32_t* label)
+                            void* label, int label16)
 {
     int wpr = (c.W + 31) / 32;
     int64_t n = (int64_t)wpr * c.H;
     int blocks = (int)((n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_dilate_to_label, dim3(blocks), dim3(kBlock), 0, s, mask_bits, c.W, c.H, wpr, r,
-                       label);
+    hipLaunchKernelGGL(k_dilate_to_label, dim3(blocks), dim3(kBlock), 0, s, mask_bits, c.W, c.H, c.LW, wpr, r,
+                       label, label16);
 }
 
-void launch_count_labeled(hipStream_t s, const uint32_t* label, size_t n, unsigned long long* out)
+void launch_tile_labels(hipStream_t s, const void* label, int label16, const DevCanvas& c, uint32_t* tiles,
+                        unsigned long long* labeled)
 {
-    int blocks = (int)std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
-    hipLaunchKernelGGL(k_count_labeled, dim3(blocks), dim3(kBlock), 0, s, label, n, out);
+    const int TW = c.LW >> kTileShift, TH = (c.H + (1 << kTileShift) - 1) >> kTileShift;
+    const int nvec = c.LW / (label16 ? 8 : 4);
+    hipLaunchKernelGGL(k_tile_labels, dim3((unsigned)((nvec + kBlock - 1) / kBlock), (unsigned)TH), dim3(kBlock), 0,
+                       s, label, label16, c.LW, c.H, TW, tiles, labeled);
 }
 
 void launch_points(hipStream_t s, int mode, PointArgs a)

@@ -13,7 +13,9 @@ constexpr int kSub = 256;          // subpixel units per pixel of the fixed-poin
 constexpr int kSubShift = 8;
 constexpr uint32_t kLabelNone = 0xFFFFFFFFu;    // pixel center in no polygon
 constexpr uint32_t kLabelBorder = 0xFFFFFFFEu;  // pixel handled by the exact test
+constexpr uint32_t kLabelMixed = 0xFFFFFFFDu;   // tile map: the tile holds several labels
 constexpr uint32_t kLabelMaxId = 0xFFFFFFF0u;
+constexpr int kTileShift = 4;                     // label tiles of 16 x 16 pixels
 
 // Columns of a point table, as seen by a kernel.
 struct DevCols {
@@ -41,6 +43,7 @@ struct DevQuery {
 // Canvas geometry (same numbers as the host plan).
 struct DevCanvas {
     int32_t W, H;
+    int32_t LW;         // row stride of label maps (W rounded up to 16)
     float minx, miny, maxx, maxy;
     float sx, sy;
 };
@@ -93,7 +96,10 @@ struct PointArgs {
     DevCanvas cv;
     DevGrid g;
     DevPolys P;
-    const uint32_t* label;
+    const void* label;    // label map, 16- or 32-bit ids
+    int label16;
+    const uint32_t* tiles; // per 16x16 tile: its single label, or kLabelMixed
+    int tiles_w;           // tiles per canvas row
     DevResult res;
     int lds_np;           // polygons accumulated in LDS (0: global atomics)
     uint32_t* cnt;        // canvas (PT_SPLAT)
@@ -114,17 +120,21 @@ inline size_t band_lds_bytes(int mode, int nrows, int wpr)
 
 // ---- launchers (kernels.hip) ----
 void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);
+void launch_snap(hipStream_t s, const float* xy, int64_t nv, double minx, double miny, double fxs, double fys,
+                 int32_t* X, int32_t* Y);
 void launch_band(hipStream_t s, int mode, const Band* bands, int nbands, const DevPolys& P,
                  const DevCanvas& c, const uint32_t* cnt, const long long* chi,
-                 const unsigned long long* clo, DevResult res, long long* bounds, uint32_t* label,
-                 unsigned long long* aux, size_t lds_bytes, int with_sums);
-void launch_count_labeled(hipStream_t s, const uint32_t* label, size_t n, unsigned long long* out);
+                 const unsigned long long* clo, DevResult res, long long* bounds, void* label,
+                 int label16, unsigned long long* aux, size_t lds_bytes, int with_sums);
+// Tile map (16x16 tiles); `labeled` (may be null) receives the count of labeled pixels.
+void launch_tile_labels(hipStream_t s, const void* label, int label16, const DevCanvas& c, uint32_t* tiles,
+                        unsigned long long* labeled);
 void launch_touch(hipStream_t s, const int32_t* X, const int32_t* Y, const int32_t* prev,
                   int64_t nverts, const DevCanvas& c, uint32_t* mask_bits);
 void launch_mark_rects(hipStream_t s, const int32_t* rects, int nrects, const DevCanvas& c,
                        uint32_t* mask_bits);
 void launch_dilate_to_label(hipStream_t s, const uint32_t* mask_bits, const DevCanvas& c, int r,
-                            uint32_t* label);
+                            void* label, int label16);
 // mode PT_*; a.start and a.per_block are filled in by the launcher.
 void launch_points(hipStream_t s, int mode, PointArgs a);
 

@@ -18,6 +18,9 @@ struct PolySet {
     std::vector<int32_t> off;
     double bbox[4] = {0, 0, 0, 0};  // minx, miny, maxx, maxy (double of the float bounds)
     float bboxf[4] = {0, 0, 0, 0};
+    std::vector<float> pbox;        // per-polygon float bounds: minx, miny, maxx, maxy
+    std::vector<int32_t> prev;      // predecessor vertex of every vertex (edge list)
+    void derive();                  // fills pbox and prev from xy/off
 };
 
 // Canvas geometry for a bbox and accuracy (GLFunction.cpp:26-30,
@@ -33,11 +36,11 @@ struct CanvasPlan {
 int make_plan(const double bbox[4], double eps_m, double ground_res, CanvasPlan* p);
 DevCanvas dev_canvas(const CanvasPlan& p);
 
-// Polygons on one canvas: snapped vertices, per-polygon pixel regions and the
-// bands the scan-conversion kernels run on.
+// Polygons on one canvas: per-polygon pixel regions and the bands the
+// scan-conversion kernels run on. The snapped vertices themselves are
+// computed on the device (the snap is monotone, so a region follows from the
+// snapped float bounds of its polygon).
 struct PolyRaster {
-    std::vector<int32_t> X, Y;        // snapped canvas coordinates (1/256 pixel)
-    std::vector<int32_t> prev;        // predecessor vertex of every vertex (edge list)
     std::vector<int32_t> region;      // c0, c1, r0, r1 per polygon (inclusive)
     std::vector<Band> bands_sum;      // BAND_SUM / BAND_LABEL
     std::vector<Band> bands_bounds;   // BAND_BOUNDS (full region width per band)
@@ -45,6 +48,9 @@ struct PolyRaster {
     bool bounds_ok = true;            // every region fits a BAND_BOUNDS band
 };
 void rasterize_polys(const PolySet& ps, const CanvasPlan& plan, PolyRaster* out);
+// Snap one coordinate to the fixed-point canvas grid (round half to even).
+int64_t snap_x(const CanvasPlan& plan, float x);
+int64_t snap_y(const CanvasPlan& plan, float y);
 
 // CPUJoin grid index (CPUJoin/GridIndexF.cpp:60-155): cell = y*xs + x.
 // `deficient` lists cells whose candidate list misses a polygon that meets
