@@ -47,7 +47,11 @@ __device__ __forceinline__ T cmp_op(int op, T a, T b)
     return op == RJ_EQ ? a == b : op == RJ_LT ? a < b : op == RJ_LTE ? a <= b : op == RJ_GT ? a > b : a >= b;
 }
 
-constexpr int kRows = 8;  // consecutive rows per thread and iteration of a point pass
+#ifndef RJ_POINT_ROWS
+#define RJ_POINT_ROWS 8
+#endif
+constexpr int kRows = RJ_POINT_ROWS;  // consecutive rows per thread and iteration of a point pass
+static_assert(kRows == 4 || kRows == 8 || kRows == 16, "rows per thread: 4, 8 or 16");
 
 // Typed comparison of 32-bit column patterns: Float columns compare as
 // floats, Int as signed and Uint (and u8) as unsigned integers; the
@@ -93,12 +97,20 @@ __device__ __forceinline__ void load_rows(const DevCols& C, int col, int64_t i0,
     if (t == RJ_ATTR_UINT8) {
         const uint8_t* p = static_cast<const uint8_t*>(C.col[col]);
         if (full) {
-            const uint2 w = *reinterpret_cast<const uint2*>(p + i0);
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                v[r] = (w.x >> (8 * r)) & 0xFFu;
-                v[r + 4] = (w.y >> (8 * r)) & 0xFFu;
+            uint32_t w[kRows / 4];
+            if (kRows == 4) {
+                w[0] = *reinterpret_cast<const uint32_t*>(p + i0);
+            } else if (kRows == 8) {
+                const uint2 u = *reinterpret_cast<const uint2*>(p + i0);
+                w[0] = u.x; w[kRows / 4 > 1 ? 1 : 0] = u.y;
+            } else {
+                const uint4 u = *reinterpret_cast<const uint4*>(p + i0);
+                w[0] = u.x; w[kRows / 4 > 1 ? 1 : 0] = u.y;
+                w[kRows / 4 > 2 ? 2 : 0] = u.z;
+                w[kRows / 4 > 3 ? 3 : 0] = u.w;
             }
+#pragma unroll
+            for (int r = 0; r < kRows; r++) v[r] = (w[r >> 2] >> (8 * (r & 3))) & 0xFFu;
         } else {
 #pragma unroll
             for (int r = 0; r < kRows; r++) v[r] = (i0 + r < end) ? (uint32_t)p[i0 + r] : 0u;
@@ -106,13 +118,38 @@ __device__ __forceinline__ void load_rows(const DevCols& C, int col, int64_t i0,
     } else {
         const uint32_t* p = static_cast<const uint32_t*>(C.col[col]);
         if (full) {
-            const uint4 a = *reinterpret_cast<const uint4*>(p + i0);
-            const uint4 b = *reinterpret_cast<const uint4*>(p + i0 + 4);
-            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-            v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+#pragma unroll
+            for (int k = 0; k < kRows / 4; k++) {
+                const uint4 a = *reinterpret_cast<const uint4*>(p + i0 + 4 * k);
+                v[4 * k] = a.x; v[4 * k + 1] = a.y; v[4 * k + 2] = a.z; v[4 * k + 3] = a.w;
+            }
         } else {
 #pragma unroll
             for (int r = 0; r < kRows; r++) v[r] = (i0 + r < end) ? p[i0 + r] : 0u;
+        }
+    }
+}
+
+// x,y of rows [i0, i0+kRows): vector loads when the group is inside the
+// table, guarded loads otherwise; nothing when i0 >= b1.
+__device__ __forceinline__ void load_xy(const float* xy, int64_t i0, int64_t b1, int64_t end, float (&px)[kRows],
+                                        float (&py)[kRows])
+{
+    if (i0 >= b1) return;
+    if (i0 + kRows <= end) {
+        const float4* p4 = reinterpret_cast<const float4*>(xy + 2 * i0);
+#pragma unroll
+        for (int k = 0; k < kRows / 2; k++) {
+            const float4 f = p4[k];
+            px[2 * k] = f.x; py[2 * k] = f.y;
+            px[2 * k + 1] = f.z; py[2 * k + 1] = f.w;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < kRows; r++) {
+            const bool in = i0 + r < end;
+            px[r] = in ? xy[2 * (i0 + r)] : 0.0f;
+            py[r] = in ? xy[2 * (i0 + r) + 1] : 0.0f;
         }
     }
 }
@@ -636,27 +673,22 @@ __global__ __launch_bounds__(kBlock) void k_points(PointArgs a)
     uint32_t cc = 0;
     unsigned long long ch = 0, cl = 0;
     const int agg_t = AVG ? a.cols.type[q.slot_col[q.agg_slot < 0 ? 0 : q.agg_slot]] : RJ_ATTR_UINT;
-    for (int64_t base = b0; base < b1; base += (int64_t)blockDim.x * kRows) {
+    // x,y of the next row group are loaded one iteration ahead: the tile
+    // gather needs only x,y and then overlaps the column loads of its group.
+    const int64_t stride = (int64_t)blockDim.x * kRows;
+    float nx[kRows], ny[kRows];
+    load_xy(a.xy, b0 + (int64_t)threadIdx.x * kRows, b1, a.end, nx, ny);
+    for (int64_t base = b0; base < b1; base += stride) {
         const int64_t i0 = base + (int64_t)threadIdx.x * kRows;
+        float px[kRows], py[kRows];
+#pragma unroll
+        for (int r = 0; r < kRows; r++) {
+            px[r] = nx[r];
+            py[r] = ny[r];
+        }
+        load_xy(a.xy, i0 + stride, b1, a.end, nx, ny);
         if (i0 >= b1) continue;
         const bool full = (i0 + kRows <= a.end);
-        float px[kRows], py[kRows];
-        if (full) {
-            const float4* p4 = reinterpret_cast<const float4*>(a.xy + 2 * i0);
-#pragma unroll
-            for (int k = 0; k < kRows / 2; k++) {
-                const float4 f = p4[k];
-                px[2 * k] = f.x; py[2 * k] = f.y;
-                px[2 * k + 1] = f.z; py[2 * k + 1] = f.w;
-            }
-        } else {
-#pragma unroll
-            for (int r = 0; r < kRows; r++) {
-                const bool in = i0 + r < a.end;
-                px[r] = in ? a.xy[2 * (i0 + r)] : 0.0f;
-                py[r] = in ? a.xy[2 * (i0 + r) + 1] : 0.0f;
-            }
-        }
         bool ok[kRows];
 #pragma unroll
         for (int r = 0; r < kRows; r++) ok[r] = (i0 + r >= lo_row) && (i0 + r < b1);
