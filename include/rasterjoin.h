@@ -53,6 +53,8 @@ typedef struct rj_constraint {
 
 typedef struct rj_ctx rj_ctx;
 typedef struct rj_table rj_table;
+typedef struct rj_polyset rj_polyset;
+typedef struct rj_server rj_server;
 
 /* GLHandler::getInstance(gpuMemMB, inMemory), GLHandler.cpp:144-149 (the
  * budget keeps the reference's 100 MB reserve, GLHandler.cpp:128-132). */
@@ -87,6 +89,40 @@ int rj_table_destroy(rj_table* t);
 /* Use rows [row_begin, row_end) of a table as the context's points. */
 int rj_bind_table(rj_ctx* ctx, rj_table* t, int64_t row_begin, int64_t row_end);
 
+/* Time-bucketed tables (replacing the backend's time partitions,
+ * DataHandler::executeQuery, DataHandler.cpp:86-96, and the 10-day x 1 km
+ * partitions of BackendIndex/main.cpp:23-71). With time_col >= 0 (a Uint
+ * column) rows are ordered by (bucket, Morton key), bucket = (t - time0) /
+ * bucket_seconds clamped to the table's buckets, so a query whose constraints
+ * bound that column reads only the rows of its buckets; its constraints are
+ * still applied exactly. */
+typedef struct rj_table_options {
+    int32_t organize;        /* 1: Morton order (within each bucket) */
+    int32_t time_col;        /* -1: no buckets */
+    uint32_t time0;          /* start of bucket 0 */
+    uint32_t bucket_seconds; /* bucket width, > 0 when time_col >= 0 */
+} rj_table_options;
+int rj_table_create_ex(rj_ctx* ctx, int64_t n, const float* xy, int32_t ncols, const void* const* cols,
+                       const int32_t* types, const rj_table_options* opt, rj_table** out);
+/* Rows of a bucketed table that may hold times in [t_lo, t_hi] (all rows
+ * without buckets). */
+int rj_table_time_rows(const rj_table* t, uint32_t t_lo, uint32_t t_hi, int64_t* row_begin, int64_t* row_end);
+
+/* Resident polygon collections. A collection keeps its snapped polygons,
+ * label maps (per accuracy) and grid indexes (per resolution) in device
+ * memory across queries; cached structures of collections not in use are
+ * evicted when the memory budget runs short. Several collections per
+ * context; a collection is used only with the context that created it.
+ * Input as rj_set_polygons. */
+int rj_polyset_create(rj_ctx* ctx, const float* xy, const int32_t* off, int32_t np, const double* bbox,
+                      rj_polyset** out);
+int rj_polyset_count(const rj_polyset* ps, int32_t* np);
+/* Device bytes the collection holds now (polygons + cached structures). */
+int rj_polyset_bytes(const rj_polyset* ps, int64_t* bytes);
+int rj_polyset_destroy(rj_polyset* ps);
+/* Use a collection as the context's polygons (instead of rj_set_polygons). */
+int rj_bind_polyset(rj_ctx* ctx, rj_polyset* ps);
+
 /* DataHandler::setQueryConstraints (DataHandler.cpp:64-66); at most 10. */
 int rj_set_constraints(rj_ctx* ctx, const rj_constraint* c, int32_t n);
 /* DataHandler::setAggregation (DataHandler.cpp:68-71): type RJ_AGG_*, col = column index. */
@@ -113,6 +149,81 @@ int rj_error_bounds(rj_ctx* ctx, int64_t* out4np);
  * line (minimum over executions, ms). */
 int rj_time_stats(rj_ctx* ctx, int32_t fn, char* buf, size_t len);
 
+/* One self-contained query (no reference counterpart: the reference runs one
+ * query at a time through GLHandler's state). row_end < 0 reads to the end of
+ * the table; index_res 0 leaves the grid resolution to the engine (256 unless
+ * rj_autotune chose another with the same results); ground_res 0 means 1. */
+typedef struct rj_query {
+    rj_polyset* polys;
+    int32_t fn;        /* RJ_RASTER, RJ_INDEX, RJ_HYBRID or RJ_ERRORBOUNDS */
+    int32_t agg_type;  /* RJ_AGG_* */
+    int32_t agg_col;
+    int32_t ncons;
+    rj_constraint cons[RJ_MAX_CONSTRAINTS];
+    double eps_m;
+    double ground_res;
+    int32_t index_res;
+    int32_t reserved;
+    int64_t row_begin, row_end;
+} rj_query;
+
+/* Where a query's results go; every pointer may be NULL. */
+typedef struct rj_result {
+    int64_t* counts;   /* np */
+    double* sums;      /* np: aggregate sums */
+    int64_t* sum_hi;   /* np: exact sums = hi + lo * 2^-32 */
+    uint64_t* sum_lo;  /* np */
+    int64_t* bounds;   /* 4*np, RJ_ERRORBOUNDS only */
+    int32_t status;    /* RJ_OK or the query's error code */
+    int32_t path;      /* RJ_PATH_*: how it ran */
+} rj_result;
+enum { RJ_PATH_SINGLE = 0, RJ_PATH_FUSED = 1 };
+
+/* One query with the single-query kernels. */
+int rj_execute_query(rj_ctx* ctx, rj_table* t, const rj_query* q, rj_result* out);
+/* A batch of queries over one table: queries that can share a scan run in
+ * fused passes (up to 64 queries and 4 distinct columns per pass, one read of
+ * the union of their rows); errorbounds, the canvas path and queries over
+ * overlapping polygons run alone. Results equal rj_execute_query's exactly.
+ * Returns RJ_OK when the batch ran; out[i].status holds each query's code. */
+int rj_execute_batch(rj_ctx* ctx, rj_table* t, int32_t nq, const rj_query* qs, rj_result* out);
+/* Autotuner of the fused pass: times label tile sizes (8/16/32 pixels), rows
+ * per thread, workgroup counts, chunk sizes and, for queries with index_res 0,
+ * grid resolutions on this batch, checks that every candidate returns the
+ * same results, and keeps the fastest in the context. chosen (may be NULL):
+ * 5 values, tile shift, rows per thread, workgroups, chunks per workgroup,
+ * automatic grid resolution. */
+int rj_autotune(rj_ctx* ctx, rj_table* t, int32_t nq, const rj_query* qs, int32_t* chosen);
+
+/* Serving loop (no reference counterpart; the nearest is the queue left as a
+ * TODO at GLHandler.cpp:195). A thread admits submitted queries while device
+ * memory and pass slots allow (memory model: the label maps, grids and results
+ * a query would add), runs them in fused passes that overlap with the
+ * preparation of the next, and fills each query's rj_result. Buffers named by
+ * the rj_result must stay valid until rj_server_poll reports the query done.
+ * While a server runs, its context must not be used by other threads except
+ * through rj_server_* and rj_polyset_create. */
+typedef struct rj_server_config {
+    int32_t max_batch;    /* queries per fused pass, 1..64 (0: 64) */
+    int32_t max_pending;  /* queued queries before rj_server_submit refuses (0: 65536) */
+    int64_t mem_mb;       /* device memory for admitted queries (0: the context budget) */
+} rj_server_config;
+typedef struct rj_server_stats {
+    int64_t submitted, completed, failed, batches;
+    double mean_batch;       /* queries per fused pass */
+    double p50_ms, p99_ms, max_ms;  /* submit -> result latency of completed queries */
+    double qps;              /* completed / (last completion - first submit) */
+} rj_server_stats;
+int rj_server_create(rj_ctx* ctx, rj_table* t, const rj_server_config* cfg, rj_server** out);
+int rj_server_submit(rj_server* s, const rj_query* q, rj_result* out, uint64_t* ticket);
+/* 1: done (rj_result filled), 0: still pending after wait_ms, < 0: error. */
+int rj_server_poll(rj_server* s, uint64_t ticket, int32_t wait_ms);
+int rj_server_stats_get(rj_server* s, rj_server_stats* st);
+/* Resets the latency record (counters stay). */
+int rj_server_stats_reset(rj_server* s);
+/* Finishes every admitted query, then stops the thread. */
+int rj_server_destroy(rj_server* s);
+
 /* Canvas geometry of the current polygons + accuracy (RasterJoin.cpp:51-78). */
 int rj_canvas_info(rj_ctx* ctx, int32_t* W, int32_t* H);
 
@@ -124,9 +235,16 @@ int rj_canvas_info(rj_ctx* ctx, int32_t* W, int32_t* H);
  *   RJ_OPT_CACHE:           1 (default) keeps polygon-side structures (snapped
  *                           polygons, label maps, grid index) across executes;
  *                           0 rebuilds them on every execute as the reference does
- *   RJ_OPT_ORGANIZE:        1 (default) rj_set_points orders rows spatially */
+ *   RJ_OPT_ORGANIZE:        1 (default) rj_set_points orders rows spatially
+ *   RJ_OPT_TILE_SHIFT:      label tile side 2^v pixels, v in 3..5 (default 4)
+ *   RJ_OPT_PASS_WORKGROUPS: workgroups of a single-query point pass (0: 4096)
+ *   RJ_OPT_BATCH_ROWS:      rows per thread of fused passes, 4 or 8
+ *   RJ_OPT_BATCH_WORKGROUPS: persistent workgroups of fused passes (64..65536)
+ *   RJ_OPT_BATCH_CHUNKS:    chunks per workgroup fused passes aim at (1..64)
+ *   RJ_OPT_GRAPH:           1 (default) fused passes replay a captured HIP graph */
 enum { RJ_OPT_RASTER_PATH = 1, RJ_OPT_TIMING = 2, RJ_OPT_HYBRID_DILATION = 3, RJ_OPT_CACHE = 4,
-       RJ_OPT_ORGANIZE = 5 };
+       RJ_OPT_ORGANIZE = 5, RJ_OPT_TILE_SHIFT = 6, RJ_OPT_PASS_WORKGROUPS = 7, RJ_OPT_BATCH_ROWS = 8,
+       RJ_OPT_BATCH_WORKGROUPS = 9, RJ_OPT_BATCH_CHUNKS = 10, RJ_OPT_GRAPH = 11 };
 int rj_set_option(rj_ctx* ctx, int32_t key, int64_t value);
 
 /* Per-kernel device time of the last execute (timing on): name i, total ms and

@@ -1,0 +1,1422 @@
+// engine.cpp -- device memory, resident polygon structures, the single-query
+// join (GLFunction::execute per function type) and fused batches.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+namespace rj {
+size_t organize_scratch_bytes(int64_t n);
+int organize_rows(hipStream_t s, int64_t n, float* xy, void** cols, const int32_t* types, int ncols,
+                  const float bbox[4], const uint32_t* time, uint32_t t0, uint32_t span, uint32_t nb,
+                  int64_t* off_dev, void* scratch);
+}  // namespace rj
+
+using namespace rj;
+
+namespace {
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+
+// label-map tiles: columns LW >> tshift, rows ceil(H / 2^tshift); the tile
+// buffer ends with two u64 overlap counters
+size_t ntiles(const DevCanvas& c, int tshift)
+{
+    return (size_t)(c.LW >> tshift) * (size_t)((c.H + (1 << tshift) - 1) >> tshift);
+}
+size_t tiles_bytes(const DevCanvas& c, int tshift) { return (ntiles(c, tshift) * 4 + 7) / 8 * 8 + 16; }
+unsigned long long* tiles_aux(const DevBuf& tiles, const DevCanvas& c, int tshift)
+{
+    return reinterpret_cast<unsigned long long*>(tiles.as<char>() + (ntiles(c, tshift) * 4 + 7) / 8 * 8);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// buffers, tables, collections
+// ---------------------------------------------------------------------------
+
+void rj::DevBuf::release()
+{
+    if (p) {
+        (void)hipFree(p);
+        if (budget) budget->used -= (int64_t)bytes;
+    }
+    p = nullptr;
+    bytes = 0;
+    budget.reset();
+}
+
+size_t rj::Canvas::device_bytes() const
+{
+    return d_pc.bytes + d_XY.bytes + d_lab.bytes + d_tiles.bytes + d_hlab.bytes + d_htiles.bytes + d_mask.bytes +
+           d_rects.bytes;
+}
+
+size_t rj_polyset::device_bytes() const
+{
+    size_t b = d_xy.bytes + d_off.bytes + d_prev.bytes;
+    for (auto& c : canvases) b += c.second->device_bytes();
+    for (auto& g : grids) b += g.second->device_bytes();
+    return b;
+}
+
+static size_t type_bytes(int t) { return t == RJ_ATTR_UINT8 ? 1 : 4; }
+
+rj_table::~rj_table()
+{
+    if (xy) (void)hipFree(xy);
+    for (int c = 0; c < ncols; c++)
+        if (cols[c]) (void)hipFree(cols[c]);
+}
+
+size_t rj_table::bytes() const
+{
+    size_t b = (size_t)n * 8;
+    for (int c = 0; c < ncols; c++) b += (size_t)n * type_bytes(types[c]);
+    return b;
+}
+
+void rj_table::time_rows(uint32_t lo, uint32_t hi, int64_t* b, int64_t* e) const
+{
+    *b = 0;
+    *e = n;
+    if (time_col < 0 || bucket_off.size() < 2) return;
+    if (lo > hi) {
+        *e = 0;
+        return;
+    }
+    const int64_t nb = (int64_t)bucket_off.size() - 1;
+    auto bucket = [&](uint32_t t) -> int64_t {
+        if (t < t0) return 0;
+        return std::min<int64_t>((t - t0) / span, nb - 1);
+    };
+    *b = bucket_off[bucket(lo)];
+    *e = bucket_off[bucket(hi) + 1];
+}
+
+// ---------------------------------------------------------------------------
+// context basics
+// ---------------------------------------------------------------------------
+
+rj_ctx::~rj_ctx()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (own_table) delete table;
+    if (own_polys) polysets.erase(own_polys.get());
+    own_polys.reset();
+    for (auto* P : polysets) P->ctx = nullptr;  // caller-owned collections may outlive the context
+    for (auto& S : slots) {
+        if (S.graph) (void)hipGraphExecDestroy(S.graph);
+        if (S.h_block) (void)hipHostFree(S.h_block);
+        if (S.h_res) (void)hipHostFree(S.h_res);
+        if (S.done) (void)hipEventDestroy(S.done);
+    }
+    if (h_res) (void)hipHostFree(h_res);
+    if (h_stage) (void)hipHostFree(h_stage);
+    if (stage_ev) (void)hipEventDestroy(stage_ev);
+    for (auto e : pool) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+int rj_ctx::fail(int code, const std::string& m)
+{
+    err = m;
+    return code;
+}
+
+int rj_ctx::hip_fail(hipError_t e, const char* what)
+{
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return (e == hipErrorOutOfMemory) ? RJ_ERR_NOMEM : RJ_ERR_DEVICE;
+}
+
+// Device allocation counted against the budget; reuses a large-enough buffer.
+// When the budget is short, cached polygon structures not in use are evicted
+// (least recently used first).
+int rj_ctx::alloc(DevBuf& b, size_t bytes, const char* what)
+{
+    if (bytes == 0) bytes = 8;
+    if (b.p && b.bytes >= bytes) return RJ_OK;
+    b.release();
+    if (budget->limit > 0 && budget->used + (int64_t)bytes > budget->limit) make_room((int64_t)bytes);
+    if (budget->limit > 0 && budget->used + (int64_t)bytes > budget->limit)
+        return fail(RJ_ERR_NOMEM, std::string("memory budget exhausted allocating ") + what + " (" +
+                                      std::to_string(bytes) + " bytes, " + std::to_string(budget->used) + " of " +
+                                      std::to_string(budget->limit) + " in use)");
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        (void)hipGetLastError();
+        return hip_fail(e, what);
+    }
+    b.bytes = bytes;
+    b.budget = budget;
+    budget->used += (int64_t)bytes;
+    return RJ_OK;
+}
+
+int rj_ctx::upload(DevBuf& b, const void* src, size_t bytes, const char* what)
+{
+    int rc = alloc(b, bytes, what);
+    if (rc) return rc;
+    if (bytes == 0) return RJ_OK;
+    hipError_t e = hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) return hip_fail(e, what);
+    return RJ_OK;
+}
+
+int rj_ctx::pinned(void** p, size_t* cur, size_t bytes, const char* what)
+{
+    if (*p && *cur >= bytes) return RJ_OK;
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    *cur = 0;
+    hipError_t e = hipHostMalloc(p, std::max<size_t>(bytes, 64));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return hip_fail(e, what);
+    }
+    *cur = std::max<size_t>(bytes, 64);
+    return RJ_OK;
+}
+
+hipEvent_t rj_ctx::event()
+{
+    if (pool_used == pool.size()) {
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        pool.push_back(e);
+    }
+    return pool[pool_used++];
+}
+
+void rj_ctx::collect_times(double* pt_ms, double* poly_ms)
+{
+    kstats.clear();
+    *pt_ms = *poly_ms = 0;
+    for (auto& l : launches) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, l.a, l.b);
+        bool found = false;
+        for (auto& k : kstats)
+            if (k.name == l.name) {
+                k.ms += ms;
+                k.launches++;
+                found = true;
+            }
+        if (!found) kstats.push_back({l.name, ms, 1});
+        if (l.name.rfind("points", 0) == 0 || l.name.rfind("splat", 0) == 0 || l.name.rfind("batch", 0) == 0)
+            *pt_ms += ms;
+        else
+            *poly_ms += ms;
+    }
+    launches.clear();
+    pool_used = 0;
+}
+
+// Evict cached canvases and grids of collections that no batch in flight
+// uses, least recently used first, until `bytes` more fit the budget.
+int rj_ctx::make_room(int64_t bytes)
+{
+    if (budget->limit <= 0) return RJ_OK;
+    while (budget->used + bytes > budget->limit) {
+        rj_polyset* bestP = nullptr;
+        double bestKey = 0;
+        int bestRes = 0;
+        bool bestCanvas = false;
+        uint64_t bestUse = UINT64_MAX;
+        for (rj_polyset* P : polysets) {
+            if (P->pins > 0) continue;
+            for (auto& c : P->canvases)
+                if (c.second->last_use < bestUse && c.second->device_bytes() > 0) {
+                    bestUse = c.second->last_use;
+                    bestP = P;
+                    bestKey = c.first;
+                    bestCanvas = true;
+                }
+            for (auto& g : P->grids)
+                if (g.second->last_use < bestUse && g.second->device_bytes() > 0) {
+                    bestUse = g.second->last_use;
+                    bestP = P;
+                    bestRes = g.first;
+                    bestCanvas = false;
+                }
+        }
+        if (!bestP) return RJ_ERR_NOMEM;
+        (void)hipStreamSynchronize(stream);  // nothing queued may still read it
+        if (bestCanvas) bestP->canvases.erase(bestKey);
+        else bestP->grids.erase(bestRes);
+    }
+    return RJ_OK;
+}
+
+void rj_ctx::forget_polyset(rj_polyset* P)
+{
+    polysets.erase(P);
+    if (polys == P) polys = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// polygon collections
+// ---------------------------------------------------------------------------
+
+int rj_ctx::make_polyset(const float* xy, const int32_t* off, int32_t np, const double* bbox,
+                         std::unique_ptr<rj_polyset>* out)
+{
+    if (np <= 0 || !xy || !off) return fail(RJ_ERR_ARG, "empty polygon collection");
+    if (off[0] != 0) return fail(RJ_ERR_ARG, "offsets must start at 0");
+    for (int p = 0; p < np; p++)
+        if (off[p + 1] - off[p] < 3) return fail(RJ_ERR_ARG, "polygon " + std::to_string(p) + " has < 3 vertices");
+    auto P = std::make_unique<rj_polyset>();
+    PolySet& ps = P->ps;
+    ps.np = np;
+    ps.off.assign(off, off + np + 1);
+    ps.xy.assign(xy, xy + 2 * (size_t)off[np]);
+    float mn[2] = {xy[0], xy[1]}, mx[2] = {xy[0], xy[1]};
+    for (int64_t k = 0; k < off[np]; k++) {
+        for (int d = 0; d < 2; d++) {
+            float v = xy[2 * k + d];
+            if (!std::isfinite(v)) return fail(RJ_ERR_ARG, "non-finite polygon vertex");
+            mn[d] = std::min(mn[d], v);
+            mx[d] = std::max(mx[d], v);
+        }
+    }
+    if (bbox) {
+        for (int d = 0; d < 4; d++) ps.bbox[d] = bbox[d];
+    } else {
+        ps.bbox[0] = mn[0]; ps.bbox[1] = mn[1]; ps.bbox[2] = mx[0]; ps.bbox[3] = mx[1];
+    }
+    for (int d = 0; d < 4; d++) ps.bboxf[d] = (float)ps.bbox[d];
+    if (!(ps.bbox[2] > ps.bbox[0]) || !(ps.bbox[3] > ps.bbox[1])) return fail(RJ_ERR_ARG, "degenerate bbox");
+    ps.derive();
+    int rc = upload(P->d_xy, ps.xy.data(), ps.xy.size() * 4, "polygons");
+    if (!rc) rc = upload(P->d_off, ps.off.data(), ps.off.size() * 4, "polygon offsets");
+    if (!rc) rc = upload(P->d_prev, ps.prev.data(), ps.prev.size() * 4, "edge list");
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) rc = hip_fail(e, "polygon upload");
+    }
+    if (rc) return rc;
+    P->ctx = this;
+    polysets.insert(P.get());
+    *out = std::move(P);
+    return RJ_OK;
+}
+
+// Plan, snapped polygons and scan-conversion bands of a collection on the
+// canvas of one accuracy (GLFunction::setMaxCellSize, RasterJoin::updateBuffers).
+int rj_ctx::canvas_for(rj_polyset* P, double eps_m, double gres, Canvas** out)
+{
+    CanvasPlan plan;
+    int rc = make_plan(P->ps.bbox, eps_m, gres, &plan);
+    if (rc) return fail(rc, "canvas too large or bad accuracy for this polygon bbox");
+    auto it = P->canvases.find(plan.cell);
+    Canvas* C;
+    if (it != P->canvases.end()) {
+        C = it->second.get();
+        C->last_use = ++clock;
+        if (cache) {
+            *out = C;
+            return RJ_OK;
+        }
+    } else {
+        auto nc = std::make_unique<Canvas>();
+        C = nc.get();
+        P->canvases[plan.cell] = std::move(nc);
+    }
+    C->last_use = ++clock;
+    C->plan = plan;
+    C->lab_built = false;
+    C->lab_state = 0;
+    C->hyb_res = -1;
+    double t0 = now_ms();
+    rasterize_polys(P->ps, plan, &C->pr);
+    cur_setup += now_ms() - t0;
+    t0 = now_ms();
+    // regions and bands: one pinned staging buffer, one copy
+    const PolyRaster& pr = C->pr;
+    const size_t parts[3] = {pr.region.size() * 4, pr.bands_sum.size() * sizeof(Band),
+                             pr.bands_bounds.size() * sizeof(Band)};
+    const void* srcs[3] = {pr.region.data(), pr.bands_sum.data(), pr.bands_bounds.data()};
+    size_t offs[3], total = 0;
+    for (int k = 0; k < 3; k++) {
+        offs[k] = total;
+        total += al256(parts[k]);
+    }
+    if (stage_busy) {
+        (void)hipEventSynchronize(stage_ev);
+        stage_busy = false;
+    }
+    if ((rc = pinned(&h_stage, &h_stage_bytes, total, "pinned staging"))) return rc;
+    for (int k = 0; k < 3; k++)
+        if (parts[k]) std::memcpy((char*)h_stage + offs[k], srcs[k], parts[k]);
+    if ((rc = alloc(C->d_pc, total, "polygon regions"))) {
+        P->canvases.erase(plan.cell);
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(C->d_pc.p, h_stage, total, hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) return hip_fail(e, "polygon regions");
+    if (!stage_ev) (void)hipEventCreateWithFlags(&stage_ev, hipEventDisableTiming);
+    (void)hipEventRecord(stage_ev, stream);
+    stage_busy = true;
+    char* base = C->d_pc.as<char>();
+    C->region = (int32_t*)(base + offs[0]);
+    C->bands_sum = (Band*)(base + offs[1]);
+    C->bands_bounds = (Band*)(base + offs[2]);
+    const int64_t nv = P->ps.off[P->ps.np];
+    if ((rc = alloc(C->d_XY, (size_t)nv * 8, "snapped vertices"))) {
+        P->canvases.erase(plan.cell);
+        return rc;
+    }
+    C->X = C->d_XY.as<int32_t>();
+    C->Y = C->X + nv;
+    const float* pxy = P->d_xy.as<float>();
+    int32_t *X = C->X, *Y = C->Y;
+    if ((rc = launch("snap", [&] { launch_snap(stream, pxy, nv, plan.minx, plan.miny, plan.fxs, plan.fys, X, Y); })))
+        return rc;
+    cur_polymem += now_ms() - t0;
+    *out = C;
+    return RJ_OK;
+}
+
+// CPUJoin grid index of a collection (GridIndexF::buildGrid) at one resolution.
+int rj_ctx::grid_for(rj_polyset* P, int res, Grid** out)
+{
+    auto it = P->grids.find(res);
+    if (it != P->grids.end() && cache) {
+        it->second->last_use = ++clock;
+        *out = it->second.get();
+        return RJ_OK;
+    }
+    Grid* G;
+    if (it != P->grids.end()) {
+        G = it->second.get();
+    } else {
+        auto ng = std::make_unique<Grid>();
+        G = ng.get();
+        P->grids[res] = std::move(ng);
+    }
+    G->last_use = ++clock;
+    double t0 = now_ms();
+    build_grid(P->ps, res, &G->g);
+    cur_polyindex += now_ms() - t0;
+    int rc;
+    if ((rc = upload(G->d_cell_off, G->g.cell_off.data(), G->g.cell_off.size() * 4, "grid cells")) ||
+        (rc = upload(G->d_cand, G->g.cand.data(), G->g.cand.size() * 4, "grid candidates"))) {
+        P->grids.erase(res);
+        return rc;
+    }
+    for (auto& c : P->canvases)
+        if (c.second->hyb_res == res) c.second->hyb_res = -1;
+    *out = G;
+    return RJ_OK;
+}
+
+// Label map with plain stores and its tile map; with `count` the two overlap
+// counters at the end of the tile buffer receive the coverage summed over
+// polygons and the number of labeled pixels (equal iff no pixel is in two
+// polygons). Returns without waiting.
+int rj_ctx::build_labels(rj_polyset* P, Canvas* C, void* lab, int label16, DevBuf& tiles, int tshift, bool count)
+{
+    const DevCanvas cv = dev_canvas(C->plan);
+    const size_t npx = (size_t)cv.LW * cv.H;
+    const size_t words = label16 ? npx / 2 : npx;
+    int rc = alloc(tiles, tiles_bytes(cv, tshift), "label tiles");
+    if (rc) return rc;
+    unsigned long long* aux = tiles_aux(tiles, cv, tshift);
+    const DevPolys dp = dev_polys(P, C);
+    if ((rc = launch("fill_label", [&] {
+             launch_fill_u32(stream, static_cast<uint32_t*>(lab), words, 0xFFFFFFFFu);
+             launch_fill_u32(stream, (uint32_t*)aux, 4, 0);
+         })))
+        return rc;
+    if ((rc = launch("band_label", [&] {
+             launch_band(stream, BAND_LABEL, C->bands_sum, (int)C->pr.bands_sum.size(), dp, cv, nullptr, nullptr,
+                         nullptr, DevResult{}, nullptr, lab, label16, count ? aux : nullptr, C->pr.lds_sum, 0);
+         })))
+        return rc;
+    uint32_t* t = tiles.as<uint32_t>();
+    return launch("tile_labels",
+                  [&] { launch_tile_labels(stream, lab, label16, cv, tshift, t, count ? aux + 1 : nullptr); });
+}
+
+int rj_ctx::raster_labels(rj_polyset* P, Canvas* C, bool* fresh)
+{
+    *fresh = false;
+    if (C->lab_built && cache && C->lab_tshift == opt_tshift) return RJ_OK;
+    const DevCanvas cv = dev_canvas(C->plan);
+    const size_t npx = (size_t)cv.LW * cv.H;
+    const int l16 = P->ps.np < (int)0xFFF0 ? 1 : 0;
+    int rc = alloc(C->d_lab, npx * (l16 ? 2 : 4) + 16, "label map");
+    if (rc) return rc;
+    if ((rc = build_labels(P, C, C->d_lab.p, l16, C->d_tiles, opt_tshift, true))) return rc;
+    C->lab_built = true;
+    C->lab16 = l16;
+    C->lab_tshift = opt_tshift;
+    C->lab_state = 0;  // confirmed when the counters are read
+    *fresh = true;
+    return RJ_OK;
+}
+
+int rj_ctx::hybrid_radius(const CanvasPlan& plan) const
+{
+    if (hybrid_dilation > 0) return hybrid_dilation;
+    double m = std::max(std::max(std::fabs(plan.minx), std::fabs(plan.maxx)),
+                        std::max(std::fabs(plan.miny), std::fabs(plan.maxy)));
+    double err = 16.0 * m * std::ldexp(1.0, -24);
+    double pw = std::min((plan.maxx - plan.minx) / plan.W, (plan.maxy - plan.miny) / plan.H);
+    return std::max(1, (int)std::ceil(2.0 * err / pw));
+}
+
+// Hybrid label map: polygon labels, with every pixel within the dilation
+// radius of an edge (and every pixel of a grid cell with an incomplete
+// candidate list) marked for the exact test (HybridJoin::drawOutline + the
+// 3x3 dilation of hybrid.glsl:174-185).
+int rj_ctx::hybrid_labels(rj_polyset* P, Canvas* C, Grid* G)
+{
+    if (C->hyb_res == G->g.xs && cache && C->hyb_tshift == opt_tshift) return RJ_OK;
+    const int r = hybrid_radius(C->plan);
+    if (r > 31) return fail(RJ_ERR_UNSUPPORTED, "accuracy too fine for the float margin of the exact test");
+    const DevCanvas cv = dev_canvas(C->plan);
+    const size_t npx = (size_t)cv.LW * cv.H;
+    const size_t wpr = ((size_t)cv.W + 31) / 32;
+    int rc = alloc(C->d_mask, wpr * cv.H * 4, "border mask");
+    if (rc) return rc;
+    std::vector<int32_t> rects;
+    deficient_rects(G->g, C->plan, &rects);
+    if (!rects.empty() && (rc = upload(C->d_rects, rects.data(), rects.size() * 4, "deficient cells"))) return rc;
+    const DevPolys dp = dev_polys(P, C);
+    uint32_t* mask = C->d_mask.as<uint32_t>();
+    int l16 = P->ps.np < (int)0xFFF0 ? 1 : 0;
+    if ((rc = alloc(C->d_hlab, npx * (l16 ? 2 : 4) + 16, "hybrid label map"))) return rc;
+    if ((rc = build_labels(P, C, C->d_hlab.p, l16, C->d_htiles, opt_tshift, true))) return rc;
+    unsigned long long a[2] = {0, 0};
+    hipError_t e = hipMemcpyAsync(a, tiles_aux(C->d_htiles, cv, opt_tshift), 16, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "overlap test");
+    if (a[0] != a[1]) {
+        // pixels in two or more polygons go to the exact test (32-bit map, CAS)
+        l16 = 0;
+        if ((rc = alloc(C->d_hlab, npx * 4 + 16, "hybrid label map"))) return rc;
+        uint32_t* lab32 = C->d_hlab.as<uint32_t>();
+        if ((rc = launch("fill_label", [&] { launch_fill_u32(stream, lab32, npx, kLabelNone); }))) return rc;
+        if ((rc = launch("band_label_atomic", [&] {
+                 launch_band(stream, BAND_LABEL_ATOMIC, C->bands_sum, (int)C->pr.bands_sum.size(), dp, cv, nullptr,
+                             nullptr, nullptr, DevResult{}, nullptr, lab32, 0, nullptr, C->pr.lds_sum, 0);
+             })))
+            return rc;
+    }
+    void* lab = C->d_hlab.p;
+    const int32_t* prev = P->d_prev.as<int32_t>();
+    const int64_t nv = P->ps.off[P->ps.np];
+    if ((rc = launch("fill_mask", [&] { launch_fill_u32(stream, mask, wpr * cv.H, 0); }))) return rc;
+    if ((rc = launch("touch", [&] { launch_touch(stream, C->X, C->Y, prev, nv, cv, mask); }))) return rc;
+    if (!rects.empty()) {
+        const int32_t* dr = C->d_rects.as<int32_t>();
+        if ((rc = launch("mark_cells", [&] { launch_mark_rects(stream, dr, (int)(rects.size() / 4), cv, mask); })))
+            return rc;
+    }
+    if ((rc = launch("dilate", [&] { launch_dilate_to_label(stream, mask, cv, r, lab, l16); }))) return rc;
+    uint32_t* t = C->d_htiles.as<uint32_t>();
+    if ((rc = launch("tile_labels", [&] { launch_tile_labels(stream, lab, l16, cv, opt_tshift, t, nullptr); })))
+        return rc;
+    C->hyb16 = l16;
+    C->hyb_res = G->g.xs;
+    C->hyb_tshift = opt_tshift;
+    return RJ_OK;
+}
+
+// Device bytes the polygon structures of a query would add (nothing when
+// they are resident): the memory model of admission.
+int64_t rj_ctx::estimate_bytes(const QuerySpec& s) const
+{
+    const int np = s.P->ps.np;
+    int64_t b = (int64_t)np * 7 * 8 + 64;
+    if (s.fn == RJ_INDEX || s.fn == RJ_HYBRID) {
+        if (!s.P->grids.count(s.index_res)) {
+            const int64_t cells = (int64_t)s.index_res * s.index_res;
+            b += cells * 4 + (int64_t)np * 64 * 4;
+        }
+    }
+    if (s.fn != RJ_INDEX) {
+        CanvasPlan plan;
+        if (make_plan(s.P->ps.bbox, s.eps, s.ground_res, &plan)) return b;
+        auto it = s.P->canvases.find(plan.cell);
+        const Canvas* C = it == s.P->canvases.end() ? nullptr : it->second.get();
+        const int64_t LW = (plan.W + kLabelRowAlign - 1) & ~(kLabelRowAlign - 1);
+        const int64_t npx = LW * plan.H;
+        const int64_t lw = np < 0xFFF0 ? 2 : 4;
+        if (!C) b += (int64_t)s.P->ps.off[np] * 8 + (int64_t)np * 1024;
+        if (s.fn == RJ_RASTER && (!C || !C->lab_built)) b += npx * lw + npx / 64;
+        if (s.fn == RJ_HYBRID && (!C || C->hyb_res != s.index_res)) b += npx * lw + npx / 64 + npx / 8;
+        if (s.fn == RJ_ERRORBOUNDS) b += (int64_t)plan.W * plan.H * 4;
+    }
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+// queries
+// ---------------------------------------------------------------------------
+
+DevCols rj_ctx::dev_cols(const rj_table* t) const
+{
+    DevCols c;
+    std::memset(&c, 0, sizeof(c));
+    for (int k = 0; k < t->ncols; k++) {
+        c.col[k] = t->cols[k];
+        c.type[k] = t->types[k];
+    }
+    return c;
+}
+
+DevPolys rj_ctx::dev_polys(const rj_polyset* P, const Canvas* C) const
+{
+    DevPolys d;
+    std::memset(&d, 0, sizeof(d));
+    d.np = P->ps.np;
+    d.xy = P->d_xy.as<float>();
+    d.off = P->d_off.as<int32_t>();
+    if (C) {
+        d.X = C->X;
+        d.Y = C->Y;
+        d.region = C->region;
+    }
+    return d;
+}
+
+DevGrid rj_ctx::dev_grid(const Grid* G) const
+{
+    DevGrid g;
+    g.xs = G->g.xs; g.ys = G->g.ys;
+    g.minx = G->g.minx; g.miny = G->g.miny; g.width = G->g.width; g.height = G->g.height;
+    g.cell_off = G->d_cell_off.as<int32_t>();
+    g.cand = G->d_cand.as<int32_t>();
+    return g;
+}
+
+// Filters and aggregate of a query on a table's columns (GLFunction::setupPoints,
+// GLFunction.cpp:228-323: the columns a query reads, at most 5; at most 10
+// constraints).
+int rj_ctx::build_query(const rj_table* t, const QuerySpec& s, DevQuery* q)
+{
+    std::memset(q, 0, sizeof(*q));
+    q->agg_slot = -1;
+    if ((int)s.cons.size() > RJ_MAX_CONSTRAINTS) return fail(RJ_ERR_LIMIT, "at most 10 constraints");
+    auto slot_of = [&](int col) {
+        for (int k = 0; k < q->nslots; k++)
+            if (q->slot_col[k] == col) return k;
+        if (q->nslots == kMaxSlots) return -1;
+        q->slot_col[q->nslots] = col;
+        return q->nslots++;
+    };
+    for (size_t k = 0; k < s.cons.size(); k++) {
+        const rj_constraint& c = s.cons[k];
+        if (c.attrib_id < 0 || c.attrib_id >= t->ncols)
+            return fail(RJ_ERR_ARG, "constraint column " + std::to_string(c.attrib_id) + " out of range");
+        if (c.type < RJ_EQ || c.type > RJ_GTE) return fail(RJ_ERR_ARG, "bad constraint type");
+        q->con_col[k] = c.attrib_id;
+        q->con_op[k] = c.type;
+        q->con_val[k] = c.val.uval;
+        q->con_slot[k] = slot_of(c.attrib_id);
+        if (q->con_slot[k] < 0) return fail(RJ_ERR_LIMIT, "a query may use at most 5 attributes");
+    }
+    q->ncons = (int)s.cons.size();
+    q->agg_type = s.agg_type;
+    q->agg_col = 0;
+    if (s.agg_type != RJ_AGG_COUNT && s.fn != RJ_ERRORBOUNDS) {
+        if (s.agg_col < 0 || s.agg_col >= t->ncols) return fail(RJ_ERR_ARG, "aggregation column out of range");
+        q->agg_col = s.agg_col;
+        q->agg_slot = slot_of(s.agg_col);
+        if (q->agg_slot < 0) return fail(RJ_ERR_LIMIT, "a query may use at most 5 attributes");
+    } else {
+        q->agg_type = RJ_AGG_COUNT;  // errorbounds counts only (RasterJoinBounds.cpp:124)
+    }
+    return RJ_OK;
+}
+
+// Rows a query has to read: its row range, cut to the time buckets its
+// constraints on the table's time column allow (the constraints themselves
+// are still applied, so the result is the same as over the whole range).
+void rj_ctx::narrow_rows(const rj_table* t, const QuerySpec& s, int64_t* b, int64_t* e) const
+{
+    if (t->time_col < 0 || t->bucket_off.size() < 2) return;
+    uint32_t lo = 0, hi = UINT32_MAX;
+    bool none = false;
+    for (const rj_constraint& c : s.cons) {
+        if (c.attrib_id != t->time_col) continue;
+        const uint32_t v = c.val.uval;
+        switch (c.type) {
+        case RJ_EQ: lo = std::max(lo, v); hi = std::min(hi, v); break;
+        case RJ_LT: if (v == 0) none = true; else hi = std::min(hi, v - 1); break;
+        case RJ_LTE: hi = std::min(hi, v); break;
+        case RJ_GT: if (v == UINT32_MAX) none = true; else lo = std::max(lo, v + 1); break;
+        case RJ_GTE: lo = std::max(lo, v); break;
+        default: break;
+        }
+    }
+    if (none || lo > hi) {
+        *e = *b;
+        return;
+    }
+    int64_t r0, r1;
+    t->time_rows(lo, hi, &r0, &r1);
+    *b = std::max(*b, r0);
+    *e = std::min(*e, r1);
+    if (*e < *b) *e = *b;
+}
+
+int rj_ctx::ensure_canvas_buffers(const CanvasPlan& plan, bool sums)
+{
+    const size_t npx = (size_t)plan.W * plan.H;
+    int rc = alloc(d_cnt, npx * 4, "canvas count");
+    if (rc) return rc;
+    if (sums) {
+        if ((rc = alloc(d_chi, npx * 8, "canvas sum hi"))) return rc;
+        if ((rc = alloc(d_clo, npx * 8, "canvas sum lo"))) return rc;
+    }
+    return launch("fill_canvas", [&] {
+        launch_fill_u32(stream, d_cnt.as<uint32_t>(), npx, 0);
+        if (sums) {
+            launch_fill_u32(stream, d_chi.as<uint32_t>(), 2 * npx, 0);
+            launch_fill_u32(stream, d_clo.as<uint32_t>(), 2 * npx, 0);
+        }
+    });
+}
+
+// One query with the single-query kernels: GLFunction::execute of the
+// function type (RasterJoin, IndexJoin, HybridJoin, RasterJoinBounds).
+int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
+{
+    const int fn = s.fn;
+    if (fn < RJ_RASTER || fn > RJ_ERRORBOUNDS) return fail(RJ_ERR_UNSUPPORTED, "unknown function type");
+    if (!s.P) return fail(RJ_ERR_STATE, "no polygons set");
+    if (!t) return fail(RJ_ERR_STATE, "no points set");
+    rj_polyset* P = s.P;
+    DevQuery q;
+    int rc = build_query(t, s, &q);
+    if (rc) return rc;
+    // the collection's structures may not be evicted while this query runs
+    struct Pin {
+        rj_polyset* P;
+        explicit Pin(rj_polyset* p) : P(p) { P->pins++; }
+        ~Pin() { P->pins--; }
+    } pin(P);
+    const bool avg = q.agg_type != RJ_AGG_COUNT;
+    const int np = P->ps.np;
+    int64_t rb0 = s.rb, re0 = s.re;
+    narrow_rows(t, s, &rb0, &re0);
+    const size_t res_words = (size_t)np * 7;
+    if ((rc = alloc(d_res, (res_words + 2) * 8, "results"))) return rc;
+    unsigned long long* res_base = d_res.as<unsigned long long>();
+    DevResult res{res_base, (long long*)(res_base + np), res_base + 2 * (size_t)np};
+    long long* bnd = (long long*)(res_base + 3 * (size_t)np);
+    auto clear_results = [&] {
+        return launch("fill_results", [&] { launch_fill_u32(stream, (uint32_t*)res_base, res_words * 2, 0); });
+    };
+    if ((rc = clear_results())) return rc;
+
+    PointArgs pa;
+    std::memset(&pa, 0, sizeof(pa));
+    pa.xy = static_cast<const float*>(t->xy);
+    pa.begin = rb0;
+    pa.end = re0;
+    pa.cols = dev_cols(t);
+    pa.q = q;
+    pa.res = res;
+    pa.lds_np = (np <= 2048) ? np : 0;
+    pa.cap = opt_cap;
+    Canvas* C = nullptr;
+    Grid* G = nullptr;
+    const unsigned long long* check_aux = nullptr;  // overlap counters of a fresh label map
+    out->path = RJ_PATH_SINGLE;
+
+    auto canvas_pass = [&](bool sums) -> int {
+        int r2 = ensure_canvas_buffers(C->plan, sums);
+        if (r2) return r2;
+        PointArgs sp = pa;
+        sp.cv = dev_canvas(C->plan);
+        sp.cnt = d_cnt.as<uint32_t>();
+        sp.chi = sums ? d_chi.as<unsigned long long>() : nullptr;
+        sp.clo = sums ? d_clo.as<unsigned long long>() : nullptr;
+        return launch("splat", [&] { launch_points(stream, PT_SPLAT, sp); });
+    };
+    auto canvas_join = [&]() -> int {
+        int r2 = canvas_pass(avg);
+        if (r2) return r2;
+        const DevCanvas cv = dev_canvas(C->plan);
+        const DevPolys dp = dev_polys(P, C);
+        return launch("band_sum", [&] {
+            launch_band(stream, BAND_SUM, C->bands_sum, (int)C->pr.bands_sum.size(), dp, cv, d_cnt.as<uint32_t>(),
+                        d_chi.as<long long>(), d_clo.as<unsigned long long>(), res, nullptr, nullptr, 0, nullptr,
+                        C->pr.lds_sum, avg ? 1 : 0);
+        });
+    };
+
+    if (fn == RJ_INDEX) {
+        if ((rc = grid_for(P, s.index_res, &G))) return rc;
+        pa.g = dev_grid(G);
+        pa.P = dev_polys(P, nullptr);
+        if ((rc = launch("points_index", [&] { launch_points(stream, PT_INDEX, pa); }))) return rc;
+    } else if (fn == RJ_HYBRID) {
+        if ((rc = canvas_for(P, s.eps, s.ground_res, &C))) return rc;
+        if ((rc = grid_for(P, s.index_res, &G))) return rc;
+        if ((rc = hybrid_labels(P, C, G))) return rc;
+        pa.cv = dev_canvas(C->plan);
+        pa.g = dev_grid(G);
+        pa.P = dev_polys(P, C);
+        pa.label = C->d_hlab.p;
+        pa.label16 = C->hyb16;
+        pa.tshift = C->hyb_tshift;
+        pa.tiles = C->d_htiles.as<uint32_t>();
+        pa.tiles_w = pa.cv.LW >> pa.tshift;
+        if ((rc = launch("points_hybrid", [&] { launch_points(stream, PT_LABEL_HYBRID, pa); }))) return rc;
+    } else if (fn == RJ_RASTER) {
+        if ((rc = canvas_for(P, s.eps, s.ground_res, &C))) return rc;
+        const bool known_overlap = C->lab_built && cache && C->lab_state == 2;
+        if (raster_path == 2 && known_overlap)
+            return fail(RJ_ERR_UNSUPPORTED, "polygons overlap: the label-map path needs disjoint polygons");
+        bool use_label = raster_path != 1 && !known_overlap;
+        if (use_label) {
+            bool fresh = false;
+            if ((rc = raster_labels(P, C, &fresh))) return rc;
+            const DevCanvas cv = dev_canvas(C->plan);
+            pa.cv = cv;
+            pa.P = dev_polys(P, C);
+            pa.label = C->d_lab.p;
+            pa.label16 = C->lab16;
+            pa.tshift = C->lab_tshift;
+            pa.tiles = C->d_tiles.as<uint32_t>();
+            pa.tiles_w = cv.LW >> pa.tshift;
+            if ((rc = launch("points_label", [&] { launch_points(stream, PT_LABEL, pa); }))) return rc;
+            if (C->lab_state == 0) check_aux = tiles_aux(C->d_tiles, cv, C->lab_tshift);
+        } else {
+            if ((rc = canvas_join())) return rc;
+        }
+    } else {
+        if ((rc = canvas_for(P, s.eps, s.ground_res, &C))) return rc;
+        if (!C->pr.bounds_ok) return fail(RJ_ERR_UNSUPPORTED, "a polygon is too wide for the error-bounds kernel");
+        const DevCanvas cv = dev_canvas(C->plan);
+        const DevPolys dp = dev_polys(P, C);
+        if ((rc = canvas_pass(false))) return rc;
+        if ((rc = launch("band_bounds", [&] {
+                 launch_band(stream, BAND_BOUNDS, C->bands_bounds, (int)C->pr.bands_bounds.size(), dp, cv,
+                             d_cnt.as<uint32_t>(), nullptr, nullptr, res, bnd, nullptr, 0, nullptr,
+                             C->pr.lds_bounds, 0);
+             })))
+            return rc;
+    }
+
+    // one pinned copy of [results | bounds] (+ the overlap counters)
+    const size_t res_bytes = (res_words + 2) * 8;
+    if ((rc = pinned((void**)&h_res, &h_res_bytes, res_bytes, "pinned results"))) return rc;
+    hipError_t e = hipMemcpyAsync(h_res, res_base, res_words * 8, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess && check_aux)
+        e = hipMemcpyAsync(h_res + res_words, check_aux, 16, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "execute");
+    if (check_aux) {
+        C->lab_state = (h_res[res_words] != h_res[res_words + 1]) ? 2 : 1;
+        if (C->lab_state == 2) {
+            // the fresh label map has pixels in two polygons: redo on the canvas
+            if (raster_path == 2)
+                return fail(RJ_ERR_UNSUPPORTED, "polygons overlap: the label-map path needs disjoint polygons");
+            if ((rc = clear_results())) return rc;
+            if ((rc = canvas_join())) return rc;
+            e = hipMemcpyAsync(h_res, res_base, res_words * 8, hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) return hip_fail(e, "execute");
+        }
+    }
+    out->counts.assign((const int64_t*)h_res, (const int64_t*)h_res + np);
+    out->hi.assign((const int64_t*)h_res + np, (const int64_t*)h_res + 2 * np);
+    out->lo.assign(h_res + 2 * np, h_res + 3 * np);
+    if (fn == RJ_ERRORBOUNDS) out->bounds.assign((const int64_t*)h_res + 3 * np, (const int64_t*)h_res + 7 * np);
+    else out->bounds.clear();
+    if (!avg) {
+        std::fill(out->hi.begin(), out->hi.end(), 0);
+        std::fill(out->lo.begin(), out->lo.end(), 0);
+    }
+    return RJ_OK;
+}
+
+int rj_ctx::execute(int fn, int64_t* out_counts, double* out_sums, int32_t* legacy)
+{
+    if (fn < RJ_RASTER || fn > RJ_ERRORBOUNDS) return fail(RJ_ERR_UNSUPPORTED, "unknown function type");
+    if (!polys) return fail(RJ_ERR_STATE, "no polygons set");
+    if (!table) return fail(RJ_ERR_STATE, "no points set");
+    const double t_start = now_ms();
+    cur_ptmem = cur_polymem = cur_setup = cur_polyindex = 0;
+    launches.clear();
+    pool_used = 0;
+    QuerySpec s;
+    s.P = polys;
+    s.fn = fn;
+    s.agg_type = agg_type;
+    s.agg_col = agg_col;
+    s.cons = cons;
+    s.eps = eps;
+    s.ground_res = ground_res;
+    s.index_res = index_res;
+    s.rb = rb;
+    s.re = re;
+    int rc = run_single(table, s, &last);
+    if (rc) return rc;
+    last_fn = fn;
+    const int np = polys->ps.np;
+    const bool avg = agg_type != RJ_AGG_COUNT && fn != RJ_ERRORBOUNDS;
+    for (int p = 0; p < np; p++) {
+        if (out_counts) out_counts[p] = last.counts[p];
+        const double sm = avg ? fixed_to_double(last.hi[p], last.lo[p]) : 0.0;
+        if (out_sums) out_sums[p] = sm;
+        if (legacy) {
+            legacy[p] = (int32_t)last.counts[p];
+            const double hundreds = std::trunc(sm / 100.0);
+            legacy[np + p] = (int32_t)hundreds;
+            legacy[2 * np + p] = (int32_t)((sm - 100.0 * hundreds) * 10.0);
+        }
+    }
+    double pt_ms = 0, poly_ms = 0;
+    if (timing) collect_times(&pt_ms, &poly_ms);
+    Times& T = times[fn];
+    T.execute.push_back(now_ms() - t_start);
+    T.ptMem.push_back(cur_ptmem);
+    T.ptRender.push_back(pt_ms);
+    T.polyMem.push_back(cur_polymem);
+    T.polyRender.push_back(poly_ms);
+    T.setup.push_back(cur_setup);
+    T.tri.push_back(0.0);
+    T.polyIndex.push_back(cur_polyindex);
+    T.backend.push_back(0.0);
+    T.pts = re - rb;
+    T.np = np;
+    T.passes = (fn == RJ_INDEX || fn == RJ_HYBRID) ? index_res : 1;
+    T.ncons = (int)cons.size();
+    return RJ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fused batches
+// ---------------------------------------------------------------------------
+
+int spec_check(const rj_ctx* ctx, const rj_table* t, const rj_query* q, QuerySpec* s, std::string* why)
+{
+    auto bad = [&](int code, const char* m) {
+        *why = m;
+        return code;
+    };
+    if (!q || !q->polys) return bad(RJ_ERR_ARG, "query without a polygon collection");
+    if (q->polys->ctx != ctx) return bad(RJ_ERR_ARG, "polygon collection of another context");
+    if (q->fn < RJ_RASTER || q->fn > RJ_ERRORBOUNDS) return bad(RJ_ERR_UNSUPPORTED, "unknown function type");
+    if (q->ncons < 0 || q->ncons > RJ_MAX_CONSTRAINTS) return bad(RJ_ERR_LIMIT, "at most 10 constraints");
+    if (q->agg_type < RJ_AGG_COUNT || q->agg_type > RJ_AGG_SUM) return bad(RJ_ERR_ARG, "bad aggregation");
+    if (q->agg_type != RJ_AGG_COUNT && (q->agg_col < 0 || q->agg_col >= t->ncols))
+        return bad(RJ_ERR_ARG, "aggregation column out of range");
+    for (int k = 0; k < q->ncons; k++)
+        if (q->cons[k].attrib_id < 0 || q->cons[k].attrib_id >= t->ncols || q->cons[k].type < RJ_EQ ||
+            q->cons[k].type > RJ_GTE)
+            return bad(RJ_ERR_ARG, "bad constraint");
+    if (q->fn != RJ_INDEX && !(q->eps_m > 0)) return bad(RJ_ERR_ARG, "accuracy must be > 0");
+    if (q->ground_res < 0 || q->index_res < 0 || q->index_res > 16384) return bad(RJ_ERR_ARG, "bad query field");
+    s->P = q->polys;
+    s->fn = q->fn;
+    s->agg_type = q->agg_type;
+    s->agg_col = q->agg_col;
+    s->cons.assign(q->cons, q->cons + q->ncons);
+    s->eps = q->eps_m > 0 ? q->eps_m : 100.0;
+    s->ground_res = q->ground_res > 0 ? q->ground_res : 1.0;
+    s->index_res = q->index_res > 0 ? q->index_res : ctx->opt_auto_res;
+    s->rb = q->row_begin;
+    s->re = q->row_end < 0 ? t->n : q->row_end;
+    if (s->rb < 0 || s->re < s->rb || s->re > t->n) return bad(RJ_ERR_ARG, "bad row range");
+    return RJ_OK;
+}
+
+int spec_from_query(rj_ctx* ctx, const rj_table* t, const rj_query* q, QuerySpec* s)
+{
+    std::string why;
+    const int rc = spec_check(ctx, t, q, s, &why);
+    if (rc) ctx->err = why;
+    return rc;
+}
+
+// Columns a query reads, in a fixed order (constraints, then the aggregate).
+static void query_columns(const QuerySpec& s, std::vector<int>* cols)
+{
+    cols->clear();
+    auto add = [&](int c) {
+        if (std::find(cols->begin(), cols->end(), c) == cols->end()) cols->push_back(c);
+    };
+    for (auto& c : s.cons) add(c.attrib_id);
+    if (s.agg_type != RJ_AGG_COUNT && s.fn != RJ_ERRORBOUNDS) add(s.agg_col);
+}
+
+// Polygon structures of every query of a batch; sorts the queries into those
+// the fused pass runs and those that run alone (errorbounds, the canvas path).
+int rj_ctx::batch_prepare(rj_table* t, const QuerySpec* qs, int nq, std::vector<int>* fused,
+                          std::vector<int>* alone, std::vector<int>* status)
+{
+    fused->clear();
+    alone->clear();
+    status->assign(nq, RJ_OK);
+    std::vector<std::pair<int, Canvas*>> checks;  // queries on fresh label maps
+    for (int i = 0; i < nq; i++) qs[i].P->pins++;  // released by the caller once the batch is done
+    for (int i = 0; i < nq; i++) {
+        const QuerySpec& s = qs[i];
+        DevQuery dq;
+        int rc = build_query(t, s, &dq);
+        if (rc) {
+            (*status)[i] = rc;
+            continue;
+        }
+        std::vector<int> cols;
+        query_columns(s, &cols);
+        if (s.fn == RJ_ERRORBOUNDS || (s.fn == RJ_RASTER && raster_path == 1) || (int)cols.size() > kBatchSlots) {
+            alone->push_back(i);
+            continue;
+        }
+        Canvas* C = nullptr;
+        Grid* G = nullptr;
+        if (s.fn == RJ_INDEX) {
+            rc = grid_for(s.P, s.index_res, &G);
+        } else if (s.fn == RJ_HYBRID) {
+            rc = canvas_for(s.P, s.eps, s.ground_res, &C);
+            if (!rc) rc = grid_for(s.P, s.index_res, &G);
+            if (!rc) rc = hybrid_labels(s.P, C, G);
+        } else {
+            rc = canvas_for(s.P, s.eps, s.ground_res, &C);
+            if (!rc && C->lab_built && C->lab_state == 2) {
+                alone->push_back(i);
+                continue;
+            }
+            bool fresh = false;
+            if (!rc) rc = raster_labels(s.P, C, &fresh);
+            if (!rc && C->lab_state == 0) checks.push_back({i, C});
+        }
+        if (rc) {
+            (*status)[i] = rc;
+            continue;
+        }
+        fused->push_back(i);
+    }
+    if (!checks.empty()) {
+        // one wait for the overlap counters of every fresh label map
+        std::vector<unsigned long long> a(2 * checks.size());
+        hipError_t e = hipSuccess;
+        for (size_t k = 0; k < checks.size() && e == hipSuccess; k++) {
+            Canvas* C = checks[k].second;
+            e = hipMemcpyAsync(&a[2 * k], tiles_aux(C->d_tiles, dev_canvas(C->plan), C->lab_tshift), 16,
+                               hipMemcpyDeviceToHost, stream);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) {
+            for (int i = 0; i < nq; i++) qs[i].P->pins--;
+            return hip_fail(e, "overlap test");
+        }
+        for (size_t k = 0; k < checks.size(); k++) {
+            Canvas* C = checks[k].second;
+            if (C->lab_state == 0) C->lab_state = (a[2 * k] != a[2 * k + 1]) ? 2 : 1;
+        }
+        std::vector<int> keep;
+        for (int i : *fused) {
+            bool overl = false;
+            for (auto& c : checks)
+                if (c.first == i && c.second->lab_state == 2) overl = true;
+            if (overl) alone->push_back(i);
+            else keep.push_back(i);
+        }
+        *fused = keep;
+        if (raster_path == 2)
+            for (int i : *alone)
+                if (qs[i].fn == RJ_RASTER) (*status)[i] = RJ_ERR_UNSUPPORTED;
+    }
+    return RJ_OK;
+}
+
+// Fill a pass slot's block for `members` and launch the fused pass (and the
+// copy of its results) on the stream. Returns without waiting.
+int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::vector<int>& members)
+{
+    PassSlot& S = slots[slot];
+    int rc;
+    if (!S.h_block) {
+        size_t cur = 0;
+        if ((rc = pinned((void**)&S.h_block, &cur, sizeof(BatchBlock), "pinned batch block"))) return rc;
+    }
+    // the block, then the pass's work counter
+    const size_t ctr_off = al256(sizeof(BatchBlock));
+    if ((rc = alloc(S.d_block, ctr_off + 256, "batch block"))) return rc;
+    uint32_t* counter = reinterpret_cast<uint32_t*>(S.d_block.as<char>() + ctr_off);
+    if (!S.done) (void)hipEventCreateWithFlags(&S.done, hipEventDisableTiming);
+    BatchBlock& B = *S.h_block;
+    std::memset(&B, 0, sizeof(B));
+    // batch slots: the columns of all members
+    std::vector<int> cols;
+    for (int i : members) {
+        std::vector<int> qc;
+        query_columns(qs[i], &qc);
+        for (int c : qc)
+            if (std::find(cols.begin(), cols.end(), c) == cols.end()) cols.push_back(c);
+    }
+    if ((int)cols.size() > kBatchSlots) return fail(RJ_ERR_LIMIT, "a fused pass reads at most 4 columns");
+    auto slot_of = [&](int c) { return (int)(std::find(cols.begin(), cols.end(), c) - cols.begin()); };
+    B.nslots = (int)cols.size();
+    for (size_t k = 0; k < cols.size(); k++) {
+        B.col[k] = t->cols[cols[k]];
+        B.type[k] = t->types[cols[k]];
+    }
+    // results: 3 np words per member
+    S.members = members;
+    S.res_off.assign(members.size(), 0);
+    int64_t words = 0;
+    for (size_t j = 0; j < members.size(); j++) {
+        S.res_off[j] = words;
+        words += 3 * (int64_t)qs[members[j]].P->ps.np;
+    }
+    S.res_words = words;
+    if ((rc = alloc(S.d_res, (size_t)words * 8, "batch results"))) return rc;
+    if ((rc = pinned((void**)&S.h_res, &S.h_res_cap, (size_t)words * 8, "pinned batch results"))) return rc;
+    unsigned long long* dres = S.d_res.as<unsigned long long>();
+    B.res = dres;
+    B.res_words = words;
+    B.nq = (int)members.size();
+    // queries
+    std::vector<int64_t> qb(members.size()), qe(members.size());
+    for (size_t j = 0; j < members.size(); j++) {
+        const QuerySpec& s = qs[members[j]];
+        BatchQuery& Q = B.q[j];
+        rj_polyset* P = s.P;
+        const int np = P->ps.np;
+        Q.avg = (s.agg_type != RJ_AGG_COUNT) ? 1 : 0;
+        Q.agg_slot = Q.avg ? slot_of(s.agg_col) : 0;
+        Q.agg_type = Q.avg ? t->types[s.agg_col] : RJ_ATTR_UINT;
+        Q.ncons = (int)s.cons.size();
+        for (int k = 0; k < Q.ncons; k++) {
+            Q.con_slot[k] = slot_of(s.cons[k].attrib_id);
+            Q.con_type[k] = t->types[s.cons[k].attrib_id];
+            Q.con_op[k] = s.cons[k].type;
+            Q.con_val[k] = s.cons[k].val.uval;
+        }
+        Q.P = dev_polys(P, nullptr);
+        Grid* G = nullptr;
+        if (s.fn != RJ_RASTER) {
+            auto gi = P->grids.find(s.index_res);
+            if (gi == P->grids.end()) return fail(RJ_ERR_STATE, "grid of a fused query is not resident");
+            G = gi->second.get();
+        }
+        if (s.fn == RJ_INDEX) {
+            Q.mode = PT_INDEX;
+            Q.g = dev_grid(G);
+        } else {
+            CanvasPlan plan;
+            make_plan(P->ps.bbox, s.eps, s.ground_res, &plan);
+            auto ci = P->canvases.find(plan.cell);
+            if (ci == P->canvases.end()) return fail(RJ_ERR_STATE, "canvas of a fused query is not resident");
+            Canvas* C = ci->second.get();
+            Q.cv = dev_canvas(C->plan);
+            if (s.fn == RJ_HYBRID) {
+                Q.mode = PT_LABEL_HYBRID;
+                Q.g = dev_grid(G);
+                Q.label = C->d_hlab.p;
+                Q.label16 = C->hyb16;
+                Q.tshift = C->hyb_tshift;
+                Q.tiles = C->d_htiles.as<uint32_t>();
+            } else {
+                Q.mode = PT_LABEL;
+                Q.label = C->d_lab.p;
+                Q.label16 = C->lab16;
+                Q.tshift = C->lab_tshift;
+                Q.tiles = C->d_tiles.as<uint32_t>();
+            }
+            Q.tiles_w = Q.cv.LW >> Q.tshift;
+        }
+        unsigned long long* r = dres + S.res_off[j];
+        Q.res = DevResult{r, (long long*)(r + np), r + 2 * (size_t)np};
+        qb[j] = s.rb;
+        qe[j] = s.re;
+        narrow_rows(t, s, &qb[j], &qe[j]);
+    }
+    // segments: elementary intervals of the row ranges with a constant query set
+    std::vector<int64_t> cut;
+    for (size_t j = 0; j < members.size(); j++)
+        if (qe[j] > qb[j]) {
+            cut.push_back(qb[j]);
+            cut.push_back(qe[j]);
+        }
+    std::sort(cut.begin(), cut.end());
+    cut.erase(std::unique(cut.begin(), cut.end()), cut.end());
+    int64_t union_rows = 0;
+    std::vector<BatchSeg> segs;
+    for (size_t k = 0; k + 1 < cut.size(); k++) {
+        unsigned long long m = 0;
+        for (size_t j = 0; j < members.size(); j++)
+            if (qb[j] <= cut[k] && qe[j] >= cut[k + 1] && qe[j] > qb[j]) m |= 1ull << j;
+        if (!m) continue;
+        if (!segs.empty() && segs.back().mask == m && segs.back().r1 == cut[k]) {
+            segs.back().r1 = cut[k + 1];
+        } else {
+            segs.push_back(BatchSeg{cut[k], cut[k + 1], m, 0});
+        }
+        union_rows += cut[k + 1] - cut[k];
+    }
+    const int rows = opt_batch_rows;
+    const int64_t group = 256LL * rows;
+    int64_t chunk = union_rows / std::max<int64_t>(1, (int64_t)opt_batch_wg * opt_batch_chunks);
+    chunk = std::max<int64_t>(group * 4, (chunk + group - 1) / group * group);
+    int64_t nchunks = 0;
+    for (auto& g : segs) {
+        g.chunk0 = nchunks;
+        nchunks += (g.r1 - g.r0 + chunk - 1) / chunk;
+    }
+    B.nseg = (int)segs.size();
+    for (size_t k = 0; k < segs.size(); k++) B.seg[k] = segs[k];
+    B.nchunks = nchunks;
+    B.chunk_rows = chunk;
+
+    const float* xy = static_cast<const float*>(t->xy);
+    const int nwg = opt_batch_wg;
+    const BatchBlock* dblock = S.d_block.as<BatchBlock>();
+    if (opt_graph) {
+        // the pass as one captured HIP graph: block upload, result clear and
+        // the fused kernel; everything that varies lives in the block
+        if (!S.graph || S.graph_key[0] != xy || S.graph_key[1] != dblock || S.graph_rows != rows ||
+            S.graph_wg != nwg || S.graph_nrows != t->n) {
+            if (S.graph) (void)hipGraphExecDestroy(S.graph);
+            S.graph = nullptr;
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) return hip_fail(e, "graph capture");
+            (void)hipMemcpyAsync(S.d_block.p, S.h_block, sizeof(BatchBlock), hipMemcpyHostToDevice, stream);
+            launch_batch(stream, dblock, counter, xy, t->n, rows, nwg);
+            e = hipStreamEndCapture(stream, &g);
+            if (e != hipSuccess) return hip_fail(e, "graph capture");
+            e = hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                S.graph = nullptr;
+                return hip_fail(e, "graph instantiate");
+            }
+            S.graph_key[0] = xy;
+            S.graph_key[1] = dblock;
+            S.graph_rows = rows;
+            S.graph_wg = nwg;
+            S.graph_nrows = t->n;
+        }
+        if ((rc = launch("batch", [&] { (void)hipGraphLaunch(S.graph, stream); }))) return rc;
+    } else {
+        hipError_t e = hipMemcpyAsync(S.d_block.p, S.h_block, sizeof(BatchBlock), hipMemcpyHostToDevice, stream);
+        if (e != hipSuccess) return hip_fail(e, "batch block");
+        if ((rc = launch("batch", [&] { launch_batch(stream, dblock, counter, xy, t->n, rows, nwg); }))) return rc;
+    }
+    if (words) {
+        hipError_t e = hipMemcpyAsync(S.h_res, S.d_res.p, (size_t)words * 8, hipMemcpyDeviceToHost, stream);
+        if (e != hipSuccess) return hip_fail(e, "batch results");
+    }
+    (void)hipEventRecord(S.done, stream);
+    S.busy = true;
+    return RJ_OK;
+}
+
+// Wait for a pass slot and decode its results.
+int rj_ctx::batch_finish(int slot, const QuerySpec* qs, QueryOut* outs)
+{
+    PassSlot& S = slots[slot];
+    if (!S.busy) return RJ_OK;
+    hipError_t e = hipEventSynchronize(S.done);
+    S.busy = false;
+    if (e != hipSuccess) return hip_fail(e, "fused pass");
+    for (size_t j = 0; j < S.members.size(); j++) {
+        const QuerySpec& s = qs[S.members[j]];
+        QueryOut& o = outs[S.members[j]];
+        const int np = s.P->ps.np;
+        const unsigned long long* r = S.h_res + S.res_off[j];
+        o.counts.assign((const int64_t*)r, (const int64_t*)r + np);
+        o.hi.assign((const int64_t*)r + np, (const int64_t*)r + 2 * np);
+        o.lo.assign(r + 2 * np, r + 3 * np);
+        o.bounds.clear();
+        o.path = RJ_PATH_FUSED;
+    }
+    return RJ_OK;
+}
+
+// Group fused queries into passes of at most 64 queries reading at most 4
+// columns (first fit, in query order).
+static void group_passes(const QuerySpec* qs, const std::vector<int>& fused, int max_batch,
+                         std::vector<std::vector<int>>* passes)
+{
+    passes->clear();
+    std::vector<std::vector<int>> pcols;
+    for (int i : fused) {
+        std::vector<int> qc;
+        query_columns(qs[i], &qc);
+        bool placed = false;
+        for (size_t p = 0; p < passes->size() && !placed; p++) {
+            if ((int)(*passes)[p].size() >= max_batch) continue;
+            std::vector<int> u = pcols[p];
+            for (int c : qc)
+                if (std::find(u.begin(), u.end(), c) == u.end()) u.push_back(c);
+            if ((int)u.size() > kBatchSlots) continue;
+            (*passes)[p].push_back(i);
+            pcols[p] = u;
+            placed = true;
+        }
+        if (!placed) {
+            passes->push_back({i});
+            pcols.push_back(qc);
+        }
+    }
+}
+
+int rj_ctx::execute_batch(rj_table* t, const QuerySpec* qs, int nq, QueryOut* outs, int* status)
+{
+    launches.clear();
+    pool_used = 0;
+    std::vector<int> fused, alone, st;
+    int rc = batch_prepare(t, qs, nq, &fused, &alone, &st);
+    if (rc) return rc;
+    std::vector<std::vector<int>> passes;
+    group_passes(qs, fused, kBatchMax, &passes);
+    for (size_t p = 0; p < passes.size() && !rc; p++) {
+        const int slot = (int)(p & 1);
+        rc = batch_finish(slot, qs, outs);
+        if (!rc) rc = batch_launch(slot, t, qs, passes[p]);
+    }
+    for (int slot = 0; slot < 2; slot++) {
+        const int r2 = batch_finish(slot, qs, outs);
+        if (!rc) rc = r2;
+    }
+    for (size_t k = 0; k < alone.size() && !rc; k++) {
+        const int i = alone[k];
+        if (st[i] != RJ_OK) continue;
+        st[i] = run_single(t, qs[i], &outs[i]);
+    }
+    for (int i = 0; i < nq; i++) qs[i].P->pins--;
+    if (rc) return rc;
+    for (int i = 0; i < nq; i++) status[i] = st[i];
+    if (timing) {
+        double pt = 0, poly = 0;
+        collect_times(&pt, &poly);
+    }
+    return RJ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner
+// ---------------------------------------------------------------------------
+
+// Times the fused pass of a batch over a grid of launch shapes and label tile
+// sizes (and, for queries with the automatic grid resolution, grid
+// resolutions), checks that every candidate returns the results of the first
+// one exactly, and keeps the fastest. chosen: tile shift, rows per thread,
+// workgroups, chunks per workgroup, automatic grid resolution.
+int rj_ctx::autotune(rj_table* t, const QuerySpec* qs_in, int nq, const std::vector<int>& auto_res_q,
+                     int32_t* chosen)
+{
+    if (nq <= 0) return fail(RJ_ERR_ARG, "nothing to tune");
+    std::vector<QuerySpec> qs(qs_in, qs_in + nq);
+    const bool saved_timing = timing;
+    timing = true;
+    const int base_tshift = opt_tshift, base_rows = opt_batch_rows, base_wg = opt_batch_wg,
+              base_chunks = opt_batch_chunks, base_res = opt_auto_res;
+    std::vector<QueryOut> ref(nq), cur(nq);
+    std::vector<int> st(nq);
+    bool have_ref = false;
+    double best = 1e30;
+    int32_t bestc[5] = {base_tshift, base_rows, base_wg, base_chunks, base_res};
+    auto run = [&](double* ms) -> int {
+        double m = 1e30;
+        for (int k = 0; k < 3; k++) {
+            int rc = execute_batch(t, qs.data(), nq, cur.data(), st.data());
+            if (rc) return rc;
+            double sum = 0;
+            for (auto& ks : kstats)
+                if (ks.name == "batch") sum += ks.ms;
+            m = std::min(m, sum);
+        }
+        for (int i = 0; i < nq; i++)
+            if (st[i]) return fail(st[i], "autotune: a query failed");
+        *ms = m;
+        return RJ_OK;
+    };
+    auto same = [&]() {
+        for (int i = 0; i < nq; i++)
+            if (cur[i].counts != ref[i].counts || cur[i].hi != ref[i].hi || cur[i].lo != ref[i].lo) return false;
+        return true;
+    };
+    int rc = RJ_OK;
+    const int tsh[3] = {3, 4, 5};
+    const int rws[2] = {4, 8};
+    const int wgs[3] = {1024, 2048, 4096};
+    const int chs[3] = {2, 4, 8};
+    for (int a = 0; a < 3 && !rc; a++) {
+        opt_tshift = tsh[a];
+        for (int b = 0; b < 2 && !rc; b++)
+            for (int c = 0; c < 3 && !rc; c++)
+                for (int d = 0; d < 3 && !rc; d++) {
+                    opt_batch_rows = rws[b];
+                    opt_batch_wg = wgs[c];
+                    opt_batch_chunks = chs[d];
+                    double ms = 0;
+                    if ((rc = run(&ms))) break;
+                    if (!have_ref) {
+                        ref = cur;
+                        have_ref = true;
+                    } else if (!same()) {
+                        rc = fail(RJ_ERR_DEVICE, "autotune: results differ between launch shapes");
+                        break;
+                    }
+                    if (ms < best) {
+                        best = ms;
+                        bestc[0] = tsh[a]; bestc[1] = rws[b]; bestc[2] = wgs[c]; bestc[3] = chs[d];
+                    }
+                }
+    }
+    opt_tshift = bestc[0];
+    opt_batch_rows = bestc[1];
+    opt_batch_wg = bestc[2];
+    opt_batch_chunks = bestc[3];
+    if (!rc && !auto_res_q.empty()) {
+        // grid resolution of queries that leave it to the engine: a
+        // resolution is kept only where the results equal those at 256
+        const int res_c[4] = {128, 256, 512, 1024};
+        std::vector<double> ms_r(4, 1e30);
+        std::vector<std::vector<QueryOut>> outs_r(4);
+        for (int k = 0; k < 4 && !rc; k++) {
+            for (int i : auto_res_q) qs[i].index_res = res_c[k];
+            if ((rc = run(&ms_r[k]))) break;
+            outs_r[k] = cur;
+        }
+        int pick = 1;
+        for (int k = 0; k < 4 && !rc; k++) {
+            bool eq = true;
+            for (int i = 0; i < nq; i++)
+                if (outs_r[k][i].counts != outs_r[1][i].counts || outs_r[k][i].hi != outs_r[1][i].hi ||
+                    outs_r[k][i].lo != outs_r[1][i].lo)
+                    eq = false;
+            if (eq && ms_r[k] < ms_r[pick]) pick = k;
+        }
+        bestc[4] = res_c[pick];
+        opt_auto_res = bestc[4];
+    }
+    timing = saved_timing;
+    if (rc) {
+        opt_tshift = base_tshift;
+        opt_batch_rows = base_rows;
+        opt_batch_wg = base_wg;
+        opt_batch_chunks = base_chunks;
+        opt_auto_res = base_res;
+        return rc;
+    }
+    for (int k = 0; k < 5; k++)
+        if (chosen) chosen[k] = bestc[k];
+    return RJ_OK;
+}

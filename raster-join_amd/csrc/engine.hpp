@@ -1,0 +1,294 @@
+// engine.hpp -- the join engine behind the C ABI: device buffers under a
+// memory budget, resident point tables, resident polygon collections with
+// their per-canvas and per-grid structures, single-query execution and fused
+// batches.
+//
+// One rj_ctx plays the role of the reference's GLHandler + GLFunction objects
+// (RasterJoin/join/GLHandler.cpp, GLFunction.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "rasterjoin.h"
+#include "rj_impl.hpp"
+
+namespace rj {
+
+struct Budget {
+    int64_t limit = 0;  // bytes; <= 0: unlimited
+    int64_t used = 0;
+};
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    std::shared_ptr<Budget> budget;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release();
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// Polygons of one collection on one canvas (one accuracy).
+struct Canvas {
+    CanvasPlan plan;
+    PolyRaster pr;
+    DevBuf d_pc, d_XY;  // regions + bands; snapped vertices X then Y
+    int32_t *X = nullptr, *Y = nullptr, *region = nullptr;
+    Band *bands_sum = nullptr, *bands_bounds = nullptr;
+    // raster label map. lab_state: 0 not checked, 1 polygons disjoint on the
+    // canvas, 2 some pixel in two polygons (the label map cannot be used)
+    bool lab_built = false;
+    int lab_state = 0;
+    int lab16 = 0;
+    int lab_tshift = kTileShiftDefault;
+    DevBuf d_lab, d_tiles;  // tiles buffer ends with 2 overlap counters
+    // hybrid label map (border pixels marked); depends on the grid resolution
+    // through the cells whose candidate lists are incomplete
+    int hyb_res = -1;
+    int hyb16 = 0;
+    int hyb_tshift = kTileShiftDefault;
+    DevBuf d_hlab, d_htiles, d_mask, d_rects;
+    uint64_t last_use = 0;
+    size_t device_bytes() const;
+};
+
+struct Grid {
+    GridIndex g;
+    DevBuf d_cell_off, d_cand;
+    uint64_t last_use = 0;
+    size_t device_bytes() const { return d_cell_off.bytes + d_cand.bytes; }
+};
+
+// One query as the engine runs it.
+struct QuerySpec {
+    rj_polyset* P = nullptr;
+    int fn = RJ_RASTER;
+    int agg_type = RJ_AGG_COUNT, agg_col = -1;
+    std::vector<rj_constraint> cons;
+    double eps = 100.0, ground_res = 1.0;
+    int index_res = 256;
+    int64_t rb = 0, re = 0;  // rows of the table (before time narrowing)
+};
+
+// Results of one query: counts, exact sums hi + lo * 2^-32, error bounds.
+struct QueryOut {
+    std::vector<int64_t> counts, hi, bounds;
+    std::vector<uint64_t> lo;
+    int path = 0;  // RJ_PATH_*
+};
+
+}  // namespace rj
+
+struct rj_ctx;
+
+// A resident polygon collection (rj_polyset in the C ABI).
+struct rj_polyset {
+    rj_ctx* ctx = nullptr;
+    rj::PolySet ps;
+    rj::DevBuf d_xy, d_off, d_prev;
+    std::map<double, std::unique_ptr<rj::Canvas>> canvases;  // by pixel side
+    std::map<int, std::unique_ptr<rj::Grid>> grids;          // by resolution
+    int pins = 0;  // in use by a batch in flight: not evicted
+    size_t device_bytes() const;
+};
+
+// A resident point table (rj_table in the C ABI).
+struct rj_table {
+    int device = 0;
+    int64_t n = 0;
+    int32_t ncols = 0;
+    int32_t types[RJ_MAX_COLUMNS] = {0};
+    void* xy = nullptr;
+    void* cols[RJ_MAX_COLUMNS] = {nullptr};
+    float bbox[4] = {0, 0, 0, 0};
+    bool organized = false;
+    // time buckets: rows are ordered by (bucket, Morton key); bucket b holds
+    // rows [bucket_off[b], bucket_off[b+1]). bucket(t) = (t - t0) / span,
+    // clamped to [0, nbuckets-1] (so it is monotone in t).
+    int32_t time_col = -1;
+    uint32_t t0 = 0, span = 0;
+    std::vector<int64_t> bucket_off;
+    ~rj_table();
+    size_t bytes() const;
+    // Rows that may hold times in [lo, hi] (all rows without buckets).
+    void time_rows(uint32_t lo, uint32_t hi, int64_t* b, int64_t* e) const;
+};
+
+struct rj_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::shared_ptr<rj::Budget> budget = std::make_shared<rj::Budget>();
+    int inmem = 0;
+    std::string err;
+    std::recursive_mutex mu;  // entry points and the serving thread
+
+    // polygons: rj_set_polygons (owned) or a bound collection
+    std::unique_ptr<rj_polyset> own_polys;
+    rj_polyset* polys = nullptr;
+    std::set<rj_polyset*> polysets;  // every collection of this context
+
+    // points
+    rj_table* table = nullptr;
+    bool own_table = false;
+    int64_t rb = 0, re = 0;
+
+    // query state of the GLHandler-style interface
+    std::vector<rj_constraint> cons;
+    int agg_type = RJ_AGG_COUNT, agg_col = -1;
+    double eps = 100.0, ground_res = 1.0;
+    int index_res = 256;
+
+    // options
+    int raster_path = 0;
+    bool timing = false;
+    int hybrid_dilation = 0;
+    bool cache = true;
+    bool organize = true;
+    int opt_cap = 0;                       // workgroups of a single-query pass (0: 4096)
+    int opt_tshift = rj::kTileShiftDefault;  // label tile side 2^tshift
+    int opt_batch_rows = 8;                // rows per thread in fused passes
+    int opt_batch_wg = 2048;               // persistent workgroups of a fused pass
+    int opt_batch_chunks = 4;              // chunks per workgroup a fused pass aims at
+    bool opt_graph = true;                 // fused passes replay a captured HIP graph
+    int opt_auto_res = 256;                // grid resolution of queries that leave it open
+
+    // canvas (canvas path, errorbounds)
+    rj::DevBuf d_cnt, d_chi, d_clo;
+    // single-query results
+    rj::DevBuf d_res;
+    unsigned long long* h_res = nullptr;  // pinned
+    size_t h_res_bytes = 0;
+    void* h_stage = nullptr;  // pinned staging of polygon uploads
+    size_t h_stage_bytes = 0;
+    hipEvent_t stage_ev = nullptr;
+    bool stage_busy = false;
+    rj::QueryOut last;
+    int last_fn = -1;
+
+    // fused passes: two slots (one being filled while the other runs)
+    struct PassSlot {
+        rj::DevBuf d_block, d_res;
+        rj::BatchBlock* h_block = nullptr;  // pinned
+        unsigned long long* h_res = nullptr;  // pinned
+        size_t h_res_cap = 0;
+        hipGraphExec_t graph = nullptr;
+        const void* graph_key[2] = {nullptr, nullptr};
+        int graph_rows = 0, graph_wg = 0;
+        int64_t graph_nrows = 0;
+        hipEvent_t done = nullptr;
+        std::vector<int> members;  // query indices of the pass
+        std::vector<int64_t> res_off;
+        int64_t res_words = 0;
+        bool busy = false;
+    } slots[2];
+
+    // timing
+    struct Launch {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<Launch> launches;
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+    struct KStat {
+        std::string name;
+        double ms;
+        int launches;
+    };
+    std::vector<KStat> kstats;
+    struct Times {
+        std::vector<double> execute, ptMem, ptRender, polyMem, polyRender, setup, tri, polyIndex, backend;
+        int64_t pts = 0;
+        int np = 0;
+        int passes = 1;
+        int ncons = 0;
+    } times[4];
+    double cur_ptmem = 0, cur_polymem = 0, cur_setup = 0, cur_polyindex = 0;
+    uint64_t clock = 0;  // LRU stamps of cached structures
+
+    ~rj_ctx();
+
+    int fail(int code, const std::string& m);
+    int hip_fail(hipError_t e, const char* what);
+    int alloc(rj::DevBuf& b, size_t bytes, const char* what);
+    int upload(rj::DevBuf& b, const void* src, size_t bytes, const char* what);
+    int pinned(void** p, size_t* cur, size_t bytes, const char* what);
+    hipEvent_t event();
+    template <class F>
+    int launch(const char* name, F f)
+    {
+        hipEvent_t a = nullptr, b = nullptr;
+        if (timing) {
+            a = event();
+            b = event();
+            (void)hipEventRecord(a, stream);
+        }
+        f();
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, name);
+        if (timing) {
+            (void)hipEventRecord(b, stream);
+            launches.push_back({name, a, b});
+        }
+        return RJ_OK;
+    }
+    void collect_times(double* pt_ms, double* poly_ms);
+
+    // polygon structures (cached per collection)
+    int make_polyset(const float* xy, const int32_t* off, int32_t np, const double* bbox,
+                     std::unique_ptr<rj_polyset>* out);
+    int canvas_for(rj_polyset* P, double eps_m, double gres, rj::Canvas** out);
+    int grid_for(rj_polyset* P, int res, rj::Grid** out);
+    int raster_labels(rj_polyset* P, rj::Canvas* C, bool* fresh);
+    int hybrid_labels(rj_polyset* P, rj::Canvas* C, rj::Grid* G);
+    int build_labels(rj_polyset* P, rj::Canvas* C, void* lab, int label16, rj::DevBuf& tiles, int tshift,
+                     bool count);
+    int hybrid_radius(const rj::CanvasPlan& plan) const;
+    int make_room(int64_t bytes);
+    void forget_polyset(rj_polyset* P);
+
+    // queries
+    int build_query(const rj_table* t, const rj::QuerySpec& s, rj::DevQuery* q);
+    void narrow_rows(const rj_table* t, const rj::QuerySpec& s, int64_t* b, int64_t* e) const;
+    rj::DevCols dev_cols(const rj_table* t) const;
+    rj::DevPolys dev_polys(const rj_polyset* P, const rj::Canvas* C) const;
+    rj::DevGrid dev_grid(const rj::Grid* G) const;
+    int ensure_canvas_buffers(const rj::CanvasPlan& plan, bool sums);
+    int64_t estimate_bytes(const rj::QuerySpec& s) const;
+
+    // the single-query path (plain kernels); out receives the results
+    int run_single(rj_table* t, const rj::QuerySpec& s, rj::QueryOut* out);
+    // GLHandler::executeFunction on the context's state
+    int execute(int fn, int64_t* out_counts, double* out_sums, int32_t* legacy);
+
+    // fused batches
+    // builds the polygon structures of a batch and pins its collections (the
+    // caller unpins them once the batch is done)
+    int batch_prepare(rj_table* t, const rj::QuerySpec* qs, int nq, std::vector<int>* fused,
+                      std::vector<int>* alone, std::vector<int>* status);
+    int batch_launch(int slot, rj_table* t, const rj::QuerySpec* qs, const std::vector<int>& members);
+    int batch_finish(int slot, const rj::QuerySpec* qs, rj::QueryOut* outs);
+    // every query of the batch: fused passes, then the queries that run alone;
+    // status[i] receives each query's own code
+    int execute_batch(rj_table* t, const rj::QuerySpec* qs, int nq, rj::QueryOut* outs, int* status);
+    // autoq: queries whose grid resolution is left to the tuner
+    int autotune(rj_table* t, const rj::QuerySpec* qs, int nq, const std::vector<int>& autoq, int32_t* chosen);
+};
+
+// QuerySpec from the C ABI query record (spec_check reports the error in
+// `why` and leaves the context untouched).
+int spec_check(const rj_ctx* ctx, const rj_table* t, const rj_query* q, rj::QuerySpec* s, std::string* why);
+int spec_from_query(rj_ctx* ctx, const rj_table* t, const rj_query* q, rj::QuerySpec* s);
+// Copy a query's results to the caller's rj_result buffers.
+void rj_fill_result(const rj::QueryOut& o, int np, int fn, int agg, rj_result* r);

@@ -56,7 +56,7 @@ DevCanvas dev_canvas(const CanvasPlan& p)
 {
     DevCanvas c;
     c.W = p.W; c.H = p.H;
-    c.LW = (p.W + 15) & ~15;
+    c.LW = (p.W + kLabelRowAlign - 1) & ~(kLabelRowAlign - 1);
     c.minx = p.minx_f; c.miny = p.miny_f; c.maxx = p.maxx_f; c.maxy = p.maxy_f;
     c.sx = p.sx_f; c.sy = p.sy_f;
     return c;

@@ -15,7 +15,8 @@ constexpr uint32_t kLabelNone = 0xFFFFFFFFu;    // pixel center in no polygon
 constexpr uint32_t kLabelBorder = 0xFFFFFFFEu;  // pixel handled by the exact test
 constexpr uint32_t kLabelMixed = 0xFFFFFFFDu;   // tile map: the tile holds several labels
 constexpr uint32_t kLabelMaxId = 0xFFFFFFF0u;
-constexpr int kTileShift = 4;                     // label tiles of 16 x 16 pixels
+constexpr int kTileShiftDefault = 4;              // label tiles of 16 x 16 pixels (3..5 allowed)
+constexpr int kLabelRowAlign = 32;               // label-map rows are padded to 32 pixels
 
 // Columns of a point table, as seen by a kernel.
 struct DevCols {
@@ -43,7 +44,7 @@ struct DevQuery {
 // Canvas geometry (same numbers as the host plan).
 struct DevCanvas {
     int32_t W, H;
-    int32_t LW;         // row stride of label maps (W rounded up to 16)
+    int32_t LW;         // row stride of label maps (W rounded up to kLabelRowAlign)
     float minx, miny, maxx, maxy;
     float sx, sy;
 };
@@ -98,13 +99,64 @@ struct PointArgs {
     DevPolys P;
     const void* label;    // label map, 16- or 32-bit ids
     int label16;
-    const uint32_t* tiles; // per 16x16 tile: its single label, or kLabelMixed
-    int tiles_w;           // tiles per canvas row
+    const uint32_t* tiles; // per tile of 2^tshift pixels square: its single label, or kLabelMixed
+    int tiles_w;           // tiles per canvas row (LW >> tshift)
+    int tshift;
     DevResult res;
     int lds_np;           // polygons accumulated in LDS (0: global atomics)
     uint32_t* cnt;        // canvas (PT_SPLAT)
     unsigned long long* chi;
     unsigned long long* clo;
+    int cap;              // workgroups of the pass (0: 4096)
+};
+
+// ---- fused passes over many queries (batch.hip) ----
+
+constexpr int kBatchMax = 64;    // queries of one fused pass (one bit each in a segment mask)
+constexpr int kBatchSlots = 4;   // distinct columns one fused pass reads
+constexpr int kBatchSegs = 2 * kBatchMax + 1;
+
+// One query of a fused pass. Filters and the aggregate name batch slots.
+struct BatchQuery {
+    int32_t mode;      // PT_LABEL, PT_LABEL_HYBRID or PT_INDEX
+    int32_t avg;       // 1: sum the aggregate column
+    int32_t agg_slot;
+    int32_t agg_type;  // column type of the aggregate
+    int32_t ncons;
+    int32_t con_slot[RJ_MAX_CONSTRAINTS];
+    int32_t con_type[RJ_MAX_CONSTRAINTS];
+    int32_t con_op[RJ_MAX_CONSTRAINTS];
+    uint32_t con_val[RJ_MAX_CONSTRAINTS];
+    int32_t label16, tshift, tiles_w, pad;
+    const void* label;
+    const uint32_t* tiles;
+    DevCanvas cv;
+    DevGrid g;
+    DevPolys P;
+    DevResult res;
+};
+
+// Rows [r0, r1) read by the queries of `mask`; split into chunks of
+// chunk_rows rows numbered from chunk0.
+struct BatchSeg {
+    int64_t r0, r1;
+    unsigned long long mask;
+    int64_t chunk0;
+};
+
+// Everything a fused pass reads, copied to the device in one piece.
+struct BatchBlock {
+    int32_t nseg;
+    int32_t nslots;
+    int32_t nq;
+    int64_t nchunks;
+    int64_t chunk_rows;
+    unsigned long long* res;  // result words of all queries, cleared before the pass
+    int64_t res_words;
+    const void* col[kBatchSlots];
+    int32_t type[kBatchSlots];
+    BatchSeg seg[kBatchSegs];
+    BatchQuery q[kBatchMax];
 };
 
 // LDS bytes a band of `nrows` x `wpr` words needs in each mode.
@@ -126,9 +178,10 @@ void launch_band(hipStream_t s, int mode, const Band* bands, int nbands, const D
                  const DevCanvas& c, const uint32_t* cnt, const long long* chi,
                  const unsigned long long* clo, DevResult res, long long* bounds, void* label,
                  int label16, unsigned long long* aux, size_t lds_bytes, int with_sums);
-// Tile map (16x16 tiles); `labeled` (may be null) receives the count of labeled pixels.
-void launch_tile_labels(hipStream_t s, const void* label, int label16, const DevCanvas& c, uint32_t* tiles,
-                        unsigned long long* labeled);
+// Tile map (tiles of 2^tshift pixels square, tshift 3..5); `labeled` (may be
+// null) receives the count of labeled pixels.
+void launch_tile_labels(hipStream_t s, const void* label, int label16, const DevCanvas& c, int tshift,
+                        uint32_t* tiles, unsigned long long* labeled);
 void launch_touch(hipStream_t s, const int32_t* X, const int32_t* Y, const int32_t* prev,
                   int64_t nverts, const DevCanvas& c, uint32_t* mask_bits);
 void launch_mark_rects(hipStream_t s, const int32_t* rects, int nrects, const DevCanvas& c,
@@ -137,5 +190,10 @@ void launch_dilate_to_label(hipStream_t s, const uint32_t* mask_bits, const DevC
                             void* label, int label16);
 // mode PT_*; a.start and a.per_block are filled in by the launcher.
 void launch_points(hipStream_t s, int mode, PointArgs a);
+// Fused pass: clears the results named by the block and the work counter,
+// then `nwg` persistent workgroups take chunks from the counter until none is
+// left; rows = rows per thread (4 or 8). The block is read-only on the device.
+void launch_batch(hipStream_t s, const BatchBlock* dev_block, uint32_t* counter, const float* xy, int64_t nrows,
+                  int rows, int nwg);
 
 }  // namespace rj

@@ -27,6 +27,10 @@ EQ, LT, LTE, GT, GTE = 0, 1, 2, 3, 4
 COUNT, AVG, SUM = 0, 1, 2
 # options
 OPT_RASTER_PATH, OPT_TIMING, OPT_HYBRID_DILATION, OPT_CACHE, OPT_ORGANIZE = 1, 2, 3, 4, 5
+OPT_TILE_SHIFT, OPT_PASS_WORKGROUPS, OPT_BATCH_ROWS, OPT_BATCH_WORKGROUPS, OPT_BATCH_CHUNKS, OPT_GRAPH = \
+    6, 7, 8, 9, 10, 11
+# how a query ran
+PATH_SINGLE, PATH_FUSED = 0, 1
 
 ERRORS = {0: "ok", -1: "bad argument", -2: "bad state", -3: "out of memory",
           -4: "device error", -5: "limit exceeded", -6: "unsupported"}
@@ -58,6 +62,35 @@ class VoronoiParams(C.Structure):
                 ("spacing_lo", C.c_double), ("spacing_hi", C.c_double), ("jitter", C.c_double)]
 
 
+class TableOptions(C.Structure):
+    _fields_ = [("organize", C.c_int32), ("time_col", C.c_int32), ("time0", C.c_uint32),
+                ("bucket_seconds", C.c_uint32)]
+
+
+class Query(C.Structure):
+    """rj_query: one self-contained query."""
+    _fields_ = [("polys", C.c_void_p), ("fn", C.c_int32), ("agg_type", C.c_int32),
+                ("agg_col", C.c_int32), ("ncons", C.c_int32), ("cons", Constraint * 10),
+                ("eps_m", C.c_double), ("ground_res", C.c_double), ("index_res", C.c_int32),
+                ("reserved", C.c_int32), ("row_begin", C.c_int64), ("row_end", C.c_int64)]
+
+
+class Result(C.Structure):
+    _fields_ = [("counts", C.c_void_p), ("sums", C.c_void_p), ("sum_hi", C.c_void_p),
+                ("sum_lo", C.c_void_p), ("bounds", C.c_void_p), ("status", C.c_int32),
+                ("path", C.c_int32)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("max_batch", C.c_int32), ("max_pending", C.c_int32), ("mem_mb", C.c_int64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("batches", C.c_int64), ("mean_batch", C.c_double), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("max_ms", C.c_double), ("qps", C.c_double)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -78,6 +111,15 @@ def lib():
             "rj_table_destroy": [vp], "rj_bind_table": [vp, vp, i64, i64],
             "rj_gen_points": [vp, i64, i64, vp, vp, vp, vp], "rj_gen_voronoi": [vp, vp],
             "rj_polys_size": [vp, vp, vp], "rj_polys_copy": [vp, vp, vp],
+            "rj_table_create_ex": [vp, i64, vp, i32, vp, vp, vp, vp],
+            "rj_table_time_rows": [vp, C.c_uint32, C.c_uint32, vp, vp],
+            "rj_polyset_create": [vp, vp, vp, i32, vp, vp], "rj_polyset_count": [vp, vp],
+            "rj_polyset_bytes": [vp, vp], "rj_polyset_destroy": [vp], "rj_bind_polyset": [vp, vp],
+            "rj_execute_query": [vp, vp, vp, vp], "rj_execute_batch": [vp, vp, i32, vp, vp],
+            "rj_autotune": [vp, vp, i32, vp, vp],
+            "rj_server_create": [vp, vp, vp, vp], "rj_server_submit": [vp, vp, vp, vp],
+            "rj_server_poll": [vp, C.c_uint64, i32], "rj_server_stats_get": [vp, vp],
+            "rj_server_stats_reset": [vp], "rj_server_destroy": [vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -167,17 +209,33 @@ def make_constraints(cons):
 class Table:
     """A resident device point table (rj_table)."""
 
-    def __init__(self, handler, xy, cols=(), types=(), organize=True):
+    def __init__(self, handler, xy, cols=(), types=(), organize=True, time_col=-1, time0=0,
+                 bucket_seconds=0):
+        """time_col >= 0 (a UINT column) orders the rows by (time bucket,
+        Morton key) with buckets of bucket_seconds starting at time0."""
         self.h = C.c_void_p()
         xy = np.ascontiguousarray(xy, np.float32)
         cols = [np.ascontiguousarray(c) for c in cols]
         colptr = (C.c_void_p * max(1, len(cols)))(*[_p(c) for c in cols])
         typ = np.array(list(types) or [0], np.int32)
-        handler._check(lib().rj_table_create(handler.ctx, len(xy) // 2, _p(xy), len(cols),
-                                             C.cast(colptr, C.c_void_p), _p(typ), int(organize),
-                                             C.byref(self.h)))
+        if time_col >= 0:
+            opt = TableOptions(int(organize), time_col, time0, bucket_seconds)
+            handler._check(lib().rj_table_create_ex(handler.ctx, len(xy) // 2, _p(xy), len(cols),
+                                                    C.cast(colptr, C.c_void_p), _p(typ),
+                                                    C.byref(opt), C.byref(self.h)))
+        else:
+            handler._check(lib().rj_table_create(handler.ctx, len(xy) // 2, _p(xy), len(cols),
+                                                 C.cast(colptr, C.c_void_p), _p(typ), int(organize),
+                                                 C.byref(self.h)))
         self.n = len(xy) // 2
         self.types = list(types)
+
+    def time_rows(self, t_lo, t_hi):
+        b, e = C.c_int64(), C.c_int64()
+        rc = lib().rj_table_time_rows(self.h, t_lo, t_hi, C.byref(b), C.byref(e))
+        if rc:
+            raise RJError(rc, "rj_table_time_rows")
+        return b.value, e.value
 
     def read(self, row0=0, n=None):
         n = self.n - row0 if n is None else n
@@ -196,6 +254,119 @@ class Table:
     def close(self):
         if self.h:
             lib().rj_table_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class PolySet:
+    """A resident polygon collection (rj_polyset) of one Handler."""
+
+    def __init__(self, handler, xy, off, bbox=None):
+        self.h = C.c_void_p()
+        xy = np.ascontiguousarray(xy, np.float32)
+        off = np.ascontiguousarray(off, np.int32)
+        b = None if bbox is None else np.ascontiguousarray(bbox, np.float64)
+        handler._check(lib().rj_polyset_create(handler.ctx, _p(xy), _p(off), len(off) - 1, _p(b),
+                                               C.byref(self.h)))
+        self.np = len(off) - 1
+        self.handler = handler
+
+    def device_bytes(self):
+        b = C.c_int64()
+        lib().rj_polyset_bytes(self.h, C.byref(b))
+        return b.value
+
+    def close(self):
+        # a collection outlives its context on the C side, so it can always be destroyed
+        if self.h:
+            lib().rj_polyset_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def make_query(polys, fn, agg=COUNT, agg_col=-1, cons=(), eps=10.0, ground_res=0.0, index_res=0,
+               rows=(0, -1)):
+    """An rj_query record for a PolySet."""
+    q = Query()
+    q.polys = polys.h
+    q.fn = fn
+    q.agg_type = agg
+    q.agg_col = agg_col
+    arr, n = make_constraints(cons)
+    q.ncons = n
+    for i in range(n):
+        q.cons[i] = arr[i]
+    q.eps_m = eps
+    q.ground_res = ground_res
+    q.index_res = index_res
+    q.row_begin, q.row_end = rows
+    return q
+
+
+class QueryResult:
+    """Output buffers of one query (counts, sums, exact sums, bounds)."""
+
+    def __init__(self, np_):
+        self.counts = np.zeros(np_, np.int64)
+        self.sums = np.zeros(np_, np.float64)
+        self.hi = np.zeros(np_, np.int64)
+        self.lo = np.zeros(np_, np.uint64)
+        self.bounds = np.zeros(4 * np_, np.int64)
+        self.rec = Result(_p(self.counts), _p(self.sums), _p(self.hi), _p(self.lo), _p(self.bounds),
+                          0, 0)
+
+    @property
+    def status(self):
+        return self.rec.status
+
+    @property
+    def path(self):
+        return self.rec.path
+
+
+class Server:
+    """Serving loop (rj_server) over one resident table."""
+
+    def __init__(self, handler, table, max_batch=0, max_pending=0, mem_mb=0):
+        self.h = C.c_void_p()
+        cfg = ServerConfig(max_batch, max_pending, mem_mb)
+        handler._check(lib().rj_server_create(handler.ctx, table.h, C.byref(cfg), C.byref(self.h)))
+        self._inflight = {}
+
+    def submit(self, q, np_):
+        res = QueryResult(np_)
+        t = C.c_uint64()
+        rc = lib().rj_server_submit(self.h, C.byref(q), C.byref(res.rec), C.byref(t))
+        if rc:
+            raise RJError(rc, "rj_server_submit")
+        self._inflight[t.value] = res
+        return t.value
+
+    def poll(self, ticket, wait_ms=0):
+        """The QueryResult once done, else None."""
+        rc = lib().rj_server_poll(self.h, ticket, int(wait_ms))
+        if rc == 0:
+            return None
+        res = self._inflight.pop(ticket)
+        if rc < 0:
+            raise RJError(rc, f"query {ticket} failed")
+        return res
+
+    def stats(self):
+        st = ServerStats()
+        lib().rj_server_stats_get(self.h, C.byref(st))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def reset_stats(self):
+        lib().rj_server_stats_reset(self.h)
+
+    def close(self):
+        if self.h:
+            lib().rj_server_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -241,6 +412,10 @@ class Handler:
         typ = np.array(list(types) or [0], np.int32)
         self._check(lib().rj_set_points(self.ctx, len(xy) // 2, _p(xy), len(cols),
                                         C.cast(colptr, C.c_void_p), _p(typ)))
+
+    def bind_polyset(self, ps):
+        self._check(lib().rj_bind_polyset(self.ctx, ps.h))
+        self.np = ps.np
 
     def bind_table(self, table, row_begin=0, row_end=None):
         row_end = table.n if row_end is None else row_end
@@ -309,3 +484,30 @@ class Handler:
 
     def sync(self):
         self._check(lib().rj_sync(self.ctx))
+
+    # --- self-contained queries ---
+    def execute_query(self, table, q, np_):
+        res = QueryResult(np_)
+        rc = lib().rj_execute_query(self.ctx, table.h, C.byref(q), C.byref(res.rec))
+        self._check(rc)
+        return res
+
+    def execute_batch(self, table, queries, nps):
+        """queries: list of Query; nps: polygon count of each. Returns a list of
+        QueryResult (check .status)."""
+        n = len(queries)
+        qa = (Query * max(1, n))(*queries)
+        outs = [QueryResult(k) for k in nps]
+        ra = (Result * max(1, n))(*[o.rec for o in outs])
+        self._check(lib().rj_execute_batch(self.ctx, table.h, n, C.cast(qa, C.c_void_p),
+                                           C.cast(ra, C.c_void_p)))
+        for i, o in enumerate(outs):
+            o.rec = ra[i]
+        return outs
+
+    def autotune(self, table, queries):
+        n = len(queries)
+        qa = (Query * max(1, n))(*queries)
+        chosen = np.zeros(5, np.int32)
+        self._check(lib().rj_autotune(self.ctx, table.h, n, C.cast(qa, C.c_void_p), _p(chosen)))
+        return dict(zip(("tile_shift", "rows", "workgroups", "chunks", "grid_res"), chosen.tolist()))
