@@ -1,21 +1,31 @@
 #!/usr/bin/env python3
 """Benchmark of the raster join on one MI355X.
 
-Workload: raster join at eps = 10 m, count() and avg(fare), over 100M points
-(70% Gaussian mixture / 30% uniform in a 50 km box) and 260 Voronoi polygons
-with jittered shared edges; the points are resident in HBM (a spatially
-organized table) before the timed region starts. One step = one complete
-join: the polygon label map is rebuilt from the polygons (no caching across
-steps) and the points are streamed once.
+Headline (`value`): raster join at eps = 10 m, count() and avg(fare), over
+100M points (70% Gaussian mixture / 30% uniform in a 50 km box) and 260
+Voronoi polygons with jittered shared edges; the points are resident in HBM
+(a spatially organized table) before the timed region starts. One step = one
+complete join: the polygon label map is rebuilt from the polygons (no caching
+across steps) and the points are streamed once.
 
-Prints one JSON line with the throughput, the roofline of the point pass
-(HIP events over the timed region) and a CPU baseline (the oracle's CPUJoin
-restatement on the host cores, on a bounded sample of the same points).
+The same JSON line also carries, on a resident 1B-point table (x, y, time,
+fare, passengers; rows in (day bucket, Morton) order):
+  raster_count_1B  raster count at eps = 10 m over all 1B points (roofline)
+  batch            64 queries (raster + hybrid, polygon sets of 10-2200,
+                   windows selecting 1M-1B points, eps in {5, 10, 50, 100})
+                   as fused passes vs one query at a time, identical results
+  serving          Poisson arrivals at 50-500 qps (60/30/10 raster/hybrid/
+                   index, 10-5000 polygons, windows 1 hour - 1 year): p50/p99
+                   latency and sustained queries/s; plus the saturated rate of
+                   the serving loop vs one query at a time
+and a CPU baseline (the oracle's CPUJoin restatement on the host cores, on a
+bounded sample of the headline points).
 """
 import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -29,6 +39,10 @@ import rasterjoin as rj  # noqa: E402
 
 METRIC = "G points joined/s at ε=10m + served queries/s with p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
+T0 = 1230768000
+YEAR = 31536000
+TYPES = (rj.UINT, rj.FLOAT, rj.UINT8)   # time, fare, passengers
+ROW_BYTES = 8 + 4 + 4 + 1
 
 
 def log(*a):
@@ -59,97 +73,57 @@ def cpu_baseline(xy, off, pts, fare, target_s=10.0, max_n=None):
                       f"avg(fare), min of 2 runs, {best:.2f} s"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--points", type=int, default=100_000_000)
-    ap.add_argument("--eps", type=float, default=10.0)
-    ap.add_argument("--path", type=int, default=0, help="0 auto, 1 canvas, 2 label map")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--calib", action="store_true",
-                    help="first run a plain 1 GiB device copy (HBM counter calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(points would shard by row range with one all-reduce of the per-polygon sums)",
-              file=sys.stderr)
-        sys.exit(2)
-
-    if args.calib:
-        import torch
-        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log("calibration copy: 1 GiB read + 1 GiB written")
-        del src, dst
-
-    n = args.points
-    cp = rj.CONFIG_POLYS["voronoi260"]
+def timed_steps(h, fn, steps, warmup):
+    for _ in range(warmup):
+        h.execute_function(fn)
+    h.sync()
+    ktot = {}
     t0 = time.perf_counter()
-    xy, off = rj.gen_voronoi(cp["nsites"], seed=cp["seed"], spacing=cp["spacing"], jitter=cp["jitter"])
-    pts, tm, fare, pas = rj.gen_points(n, params=rj.point_params(seed=7))
-    log(f"generated {n} points, {len(off) - 1} polygons in {time.perf_counter() - t0:.1f} s")
+    for _ in range(steps):
+        h.execute_function(fn)
+        for k, (ms, cnt) in h.kernel_times().items():
+            a = ktot.setdefault(k, [0.0, 0])
+            a[0] += ms
+            a[1] += cnt
+    h.sync()
+    return time.perf_counter() - t0, ktot
 
-    h = rj.Handler()
+
+def headline(h, pts, cols, xy, off, args):
+    """Config 2: raster eps=10 m, count + avg(fare), 100M points."""
+    n = len(pts) // 2
     t0 = time.perf_counter()
-    table = rj.Table(h, pts, (tm, fare, pas), (rj.UINT, rj.FLOAT, rj.UINT8), organize=True)
+    table = rj.Table(h, pts, cols, TYPES, organize=True)
     h.sync()
     load_s = time.perf_counter() - t0
-    log(f"resident table loaded and organized in {load_s:.2f} s")
+    log(f"headline table ({n} rows) loaded and organized in {load_s:.2f} s")
     h.set_polygons(xy, off)
     h.bind_table(table)
+    h.set_query_constraints(())
     h.set_aggregation(rj.AVG, 1)
     h.set_accuracy_distance(args.eps)
     h.set_option(rj.OPT_RASTER_PATH, args.path)
     h.set_option(rj.OPT_CACHE, 0)
     h.set_option(rj.OPT_TIMING, 1)
     W, H = h.canvas_size()
-
-    for _ in range(args.warmup):
-        h.execute_function(rj.RASTER)
-    h.sync()
-    ktot = {}
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        h.execute_function(rj.RASTER)
-        for k, (ms, cnt) in h.kernel_times().items():
-            a = ktot.setdefault(k, [0.0, 0])
-            a[0] += ms
-            a[1] += cnt
-    h.sync()
-    dt = time.perf_counter() - t0
-    counts, sums = h.execute_function(rj.RASTER)
-    value = n * args.steps / dt / 1e9
+    dt, ktot = timed_steps(h, rj.RASTER, args.steps, args.warmup)
+    counts, _ = h.execute_function(rj.RASTER)
     path = "label" if "points_label" in ktot else "canvas"
     log("kernels (ms total, launches):", {k: (round(v[0], 3), v[1]) for k, v in ktot.items()})
     log("timing line (GLHandler::printTimeStats layout, min over executes, ms):",
         h.print_time_stats(rj.RASTER))
-
-    # Roofline of the point pass: per launch it reads x,y (8 B) and fare
-    # (4 B) of every point and, at most once, each 4-byte label of the map.
+    # Roofline of the point pass: per launch it reads x,y (8 B) and fare (4 B)
+    # of every point and, at most once, each 2-byte label of the map.
     kname = "points_label" if path == "label" else "splat"
     k_ms = ktot[kname][0] / ktot[kname][1]
-    if path == "label":
-        alg_bytes = n * 12 + W * H * 4
-    else:
-        alg_bytes = n * 12 + W * H * 20
+    LW = (W + 31) // 32 * 32
+    alg_bytes = n * 12 + (LW * H * 2 if path == "label" else W * H * 20)
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
-    out = {
-        "metric": METRIC,
-        "value": round(value, 4),
-        "unit": "G points/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(dt / args.steps * 1e3, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32+int64",
-        "data": "synthetic",
+    h.set_option(rj.OPT_CACHE, 1)
+    h.set_option(rj.OPT_TIMING, 0)
+    table.close()
+    return {
+        "value": n * args.steps / dt / 1e9, "ms_per_step": dt / args.steps * 1e3,
         "config": {"workload": "raster join eps=10 m, count()+avg(fare), 100M points, 260 Voronoi "
                                "polygons, resident organized table",
                    "points": n, "polygons": len(off) - 1, "eps_m": args.eps, "canvas": [W, H],
@@ -157,11 +131,296 @@ def main():
                    "points_in_polygons": int(counts.sum())},
         "roofline": {"kernel": kname, "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel_ms": round(k_ms, 4), "alg_bytes": int(alg_bytes)},
+                     "traffic": profiled_traffic(), "kernel_ms": round(k_ms, 4),
+                     "alg_bytes": int(alg_bytes)},
         "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in ktot.items()},
     }
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the headline point pass as measured by the
+    rocprofv3 FETCH_SIZE/WRITE_SIZE runs summarized under profiles/."""
+    p = os.path.join(ROOT, "profiles", "points_label_100M_eps10_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("traffic_bytes")
+
+
+def count_1b(h, table, xy, off, args):
+    """Raster count at eps = 10 m over every row of the 1B table."""
+    n = table.n
+    h.set_polygons(xy, off)
+    h.bind_table(table)
+    h.set_query_constraints(())
+    h.set_aggregation(rj.COUNT, -1)
+    h.set_accuracy_distance(10.0)
+    h.set_option(rj.OPT_TIMING, 1)
+    W, H = h.canvas_size()
+    dt, ktot = timed_steps(h, rj.RASTER, max(3, args.steps // 2), 1)
+    steps = max(3, args.steps // 2)
+    k_ms = ktot["points_label"][0] / ktot["points_label"][1]
+    LW = (W + 31) // 32 * 32
+    alg = n * 8 + LW * H * 2
+    ach = alg / (k_ms * 1e-3) / 1e9
+    h.set_option(rj.OPT_TIMING, 0)
+    return {"workload": f"raster count eps=10 m over {n} points (Morton-ordered location table), "
+                        "260 polygons, label map cached",
+            "G_points_per_s": round(n * steps / dt / 1e9, 2), "ms_per_join": round(dt / steps * 1e3, 3),
+            "roofline": {"kernel": "points_label", "bound": "hbm", "achieved": round(ach, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                         "kernel_ms": round(k_ms, 4), "alg_bytes": int(alg)}}
+
+
+POLY_LIB = {  # name: (sites, seed, spacing, jitter)
+    "v10": (10, 101, (400.0, 400.0), 30.0),
+    "v40": (40, 102, (150.0, 150.0), 30.0),
+    "v260": (260, 1, (60.0, 60.0), 30.0),
+    "v900": (900, 104, (40.0, 40.0), 30.0),
+    "t2200": (2200, 3, (2.2, 140.0), 30.0),
+    "v5000": (5000, 106, (25.0, 25.0), 20.0),
+}
+
+
+def poly_library(h, names):
+    out = {}
+    for k in names:
+        s, seed, sp, jit = POLY_LIB[k]
+        xy, off = rj.gen_voronoi(s, seed=seed, spacing=sp, jitter=jit)
+        out[k] = rj.PolySet(h, xy, off)
+    return out
+
+
+def window_rows_frac(rng, lo_frac, hi_frac):
+    f = float(np.exp(rng.uniform(np.log(lo_frac), np.log(hi_frac))))
+    length = max(1, int(f * YEAR))
+    start = T0 + int(rng.uniform(0, max(0.0, 1.0 - f)) * YEAR)
+    return [(0, rj.GTE, start), (0, rj.LTE, start + length - 1)]
+
+
+def batch_queries(lib, n, seed, nrows):
+    """Config 4: raster + hybrid, polygon sets of 10-2200, windows selecting
+    1M-1B points, eps in {5, 10, 50, 100}."""
+    rng = np.random.default_rng(seed)
+    names = ["v10", "v40", "v260", "v900", "t2200"]
+    qs, nps = [], []
+    for _ in range(n):
+        ps = lib[names[int(rng.integers(len(names)))]]
+        fn = rj.RASTER if rng.random() < 0.7 else rj.HYBRID
+        eps = float((5.0, 10.0, 50.0, 100.0)[int(rng.integers(4))])
+        cons = window_rows_frac(rng, 1e6 / nrows, 1.0)
+        agg = rj.AVG if rng.random() < 0.5 else rj.COUNT
+        qs.append(rj.make_query(ps, fn, agg, 1 if agg == rj.AVG else -1, cons, eps=eps, index_res=256))
+        nps.append(ps.np)
+    return qs, nps
+
+
+def serving_queries(lib, n, seed):
+    """Config 5: 60/30/10 raster/hybrid/index, 10-5000 polygons, windows
+    log-uniform from 1 hour to 1 year, an optional passengers filter."""
+    rng = np.random.default_rng(seed)
+    names = list(POLY_LIB)
+    qs, nps = [], []
+    for _ in range(n):
+        ps = lib[names[int(rng.integers(len(names)))]]
+        u = rng.random()
+        fn = rj.RASTER if u < 0.6 else (rj.HYBRID if u < 0.9 else rj.INDEX)
+        eps = float((10.0, 20.0, 50.0, 100.0)[int(rng.integers(4))])
+        cons = window_rows_frac(rng, 3600.0 / YEAR, 1.0)
+        if rng.random() < 0.3:
+            cons.append((2, rj.LTE, int(rng.integers(1, 7))))
+        agg = rj.AVG if rng.random() < 0.5 else rj.COUNT
+        qs.append(rj.make_query(ps, fn, agg, 1 if agg == rj.AVG else -1, cons, eps=eps, index_res=256))
+        nps.append(ps.np)
+    return qs, nps
+
+
+def same(a, b):
+    return (np.array_equal(a.counts, b.counts) and np.array_equal(a.hi, b.hi)
+            and np.array_equal(a.lo, b.lo))
+
+
+def batch_section(h, table, lib, args):
+    qs, nps = batch_queries(lib, 64, seed=41, nrows=table.n)
+    # warm: polygon structures of every query resident for both paths
+    single = [h.execute_query(table, q, k) for q, k in zip(qs, nps)]
+    t0 = time.perf_counter()
+    chosen = h.autotune(table, qs)
+    tune_s = time.perf_counter() - t0
+    log(f"autotuner chose {chosen} in {tune_s:.1f} s")
+    # warm again: the tuner's last candidates left other label tile sizes built
+    single = [h.execute_query(table, q, k) for q, k in zip(qs, nps)]
+    h.execute_batch(table, qs, nps)
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        single = [h.execute_query(table, q, k) for q, k in zip(qs, nps)]
+    seq_s = (time.perf_counter() - t0) / reps
+    h.set_option(rj.OPT_TIMING, 1)
+    h.execute_batch(table, qs, nps)
+    kt = h.kernel_times()
+    h.set_option(rj.OPT_TIMING, 0)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = h.execute_batch(table, qs, nps)
+    bat_s = (time.perf_counter() - t0) / reps
+    ok = all(o.status == 0 and same(a, o) for a, o in zip(single, out))
+    fused = sum(o.path == rj.PATH_FUSED for o in out)
+    # rows each query reads and the union the fused pass reads
+    rngs = []
+    for q in qs:
+        lo, hi = q.cons[0].val, q.cons[1].val
+        rngs.append(table.time_rows(lo, hi))
+    total = sum(e - b for b, e in rngs)
+    cut = sorted(set([b for b, _ in rngs] + [e for _, e in rngs]))
+    union = sum(c1 - c0 for c0, c1 in zip(cut, cut[1:]) if any(b <= c0 and e >= c1 for b, e in rngs))
+    bms = kt.get("batch", (0.0, 0))[0]
+    res = {"workload": "64 queries, raster 70% / hybrid 30%, polygon sets 10-2200, windows 1M-1B points, "
+                       "eps in {5,10,50,100}, resident 1B-row table",
+           "queries": len(qs), "fused": fused, "identical_results": ok,
+           "batch_ms": round(bat_s * 1e3, 3), "one_at_a_time_ms": round(seq_s * 1e3, 3),
+           "batch_qps": round(len(qs) / bat_s, 1), "one_at_a_time_qps": round(len(qs) / seq_s, 1),
+           "speedup": round(seq_s / bat_s, 2), "rows_read_one_at_a_time": int(total),
+           "rows_read_fused": int(union), "autotuned": chosen, "autotune_s": round(tune_s, 2)}
+    if bms > 0:
+        ach = union * ROW_BYTES / (bms * 1e-3) / 1e9
+        res["fused_pass"] = {"kernel": "batch", "ms": round(bms, 3), "achieved": round(ach, 1),
+                             "unit": "GB/s", "peak": HBM_PEAK_GBS, "frac": round(ach / HBM_PEAK_GBS, 4),
+                             "alg_bytes": int(union * ROW_BYTES)}
+    return res
+
+
+def serve(h, table, qs, nps, rate, seed, max_batch=64):
+    srv = rj.Server(h, table, max_batch=max_batch)
+    rng = np.random.default_rng(seed)
+    tickets = []
+
+    def client():
+        t_next = time.perf_counter()
+        for q, k in zip(qs, nps):
+            if rate:
+                t_next += rng.exponential(1.0 / rate)
+                d = t_next - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+            tickets.append((srv.submit(q, k), k))
+
+    th = threading.Thread(target=client)
+    t0 = time.perf_counter()
+    th.start()
+    th.join()
+    outs = [srv.poll(tk, wait_ms=600000) for tk, _ in tickets]
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    return outs, st, wall
+
+
+def serving_section(h, table, lib, args):
+    nq = args.serve_queries
+    qs, nps = serving_queries(lib, nq, seed=43)
+    t0 = time.perf_counter()
+    alone = [h.execute_query(table, q, k) for q, k in zip(qs, nps)]  # warm + reference results
+    t0 = time.perf_counter()
+    alone = [h.execute_query(table, q, k) for q, k in zip(qs, nps)]
+    one_s = time.perf_counter() - t0
+    rates = []
+    for rate in (50, 100, 200, 500):
+        n = min(nq, max(50, int(rate * args.serve_seconds)))
+        outs, st, wall = serve(h, table, qs[:n], nps[:n], rate, seed=rate)
+        ok = all(o is not None and same(a, o) for a, o in zip(alone[:n], outs))
+        rates.append({"offered_qps": rate, "queries": n, "served_qps": round(st["qps"], 1),
+                      "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                      "mean_batch": round(st["mean_batch"], 2), "identical_results": ok})
+        log(f"serving at {rate} qps: {rates[-1]}")
+    outs, st, wall = serve(h, table, qs, nps, 0, seed=1)
+    ok = all(o is not None and same(a, o) for a, o in zip(alone, outs))
+    sat = {"queries": nq, "served_qps": round(nq / wall, 1), "p50_ms": round(st["p50_ms"], 3),
+           "p99_ms": round(st["p99_ms"], 3), "mean_batch": round(st["mean_batch"], 2),
+           "one_at_a_time_qps": round(nq / one_s, 1), "speedup": round(one_s / wall, 2),
+           "identical_results": ok}
+    log(f"saturated serving: {sat}")
+    return {"workload": "Poisson arrivals, 60/30/10 raster/hybrid/index, 10-5000 polygons, windows "
+                        "1 h - 1 yr, 30% with a passengers filter, resident 1B-row table",
+            "rates": rates, "saturated": sat}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--points", type=int, default=100_000_000, help="headline points")
+    ap.add_argument("--big", type=int, default=1_000_000_000, help="rows of the resident table (0: skip)")
+    ap.add_argument("--eps", type=float, default=10.0)
+    ap.add_argument("--path", type=int, default=0, help="0 auto, 1 canvas, 2 label map")
+    ap.add_argument("--serve-queries", type=int, default=400)
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--headline-only", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(points would shard by row range with one all-reduce of the per-polygon sums)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    big = 0 if args.headline_only else max(args.big, 0)
+    nrows = max(big, args.points)
+    cp = rj.CONFIG_POLYS["voronoi260"]
+    t0 = time.perf_counter()
+    xy, off = rj.gen_voronoi(cp["nsites"], seed=cp["seed"], spacing=cp["spacing"], jitter=cp["jitter"])
+    pts, tm, fare, pas = rj.gen_points(nrows, params=rj.point_params(seed=7))
+    log(f"generated {nrows} points, {len(off) - 1} polygons in {time.perf_counter() - t0:.1f} s")
+    n = args.points
+
+    h = rj.Handler()
+    head = headline(h, pts[:2 * n], (tm[:n], fare[:n], pas[:n]), xy, off, args)
+    out = {
+        "metric": METRIC,
+        "value": round(head["value"], 4),
+        "unit": "G points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+int64",
+        "data": "synthetic",
+        "config": head["config"],
+        "roofline": head["roofline"],
+        "kernels_ms_per_step": head["kernels_ms_per_step"],
+    }
+    if big:
+        t0 = time.perf_counter()
+        mt = rj.Table(h, pts[:2 * big], (), (), organize=True)
+        h.sync()
+        log(f"resident {big}-row location table (Morton order) in {time.perf_counter() - t0:.1f} s")
+        out["raster_count_1B"] = count_1b(h, mt, xy, off, args)
+        log("raster count 1B:", out["raster_count_1B"])
+        t0 = time.perf_counter()
+        table = rj.Table(h, pts[:2 * big], (tm[:big], fare[:big], pas[:big]), TYPES, time_col=0,
+                         time0=T0, bucket_seconds=86400)
+        h.sync()
+        h.bind_table(table)  # the context no longer points at the location table
+        mt.close()
+        log(f"resident {big}-row table (day buckets, Morton order) in {time.perf_counter() - t0:.1f} s")
+        lib = poly_library(h, list(POLY_LIB))
+        out["batch"] = batch_section(h, table, lib, args)
+        log("batch:", out["batch"])
+        srv = serving_section(h, table, lib, args)
+        out["serving"] = srv
+        top = max(srv["rates"], key=lambda r: r["offered_qps"])
+        out["served_qps"] = top["served_qps"]
+        out["p50_ms"] = top["p50_ms"]
+        out["p99_ms"] = top["p99_ms"]
+        for ps in lib.values():
+            ps.close()
+        table.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(xy, off, pts, fare)
+        out["cpu_baseline"] = cpu_baseline(xy, off, pts[:2 * n], fare[:n])
     h.close()
     print(json.dumps(out), flush=True)
 

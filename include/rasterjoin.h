@@ -182,8 +182,8 @@ enum { RJ_PATH_SINGLE = 0, RJ_PATH_FUSED = 1 };
 /* One query with the single-query kernels. */
 int rj_execute_query(rj_ctx* ctx, rj_table* t, const rj_query* q, rj_result* out);
 /* A batch of queries over one table: queries that can share a scan run in
- * fused passes (up to 64 queries and 4 distinct columns per pass, one read of
- * the union of their rows); errorbounds, the canvas path and queries over
+ * fused passes (up to 64 queries per pass, one read of the union of their
+ * rows); errorbounds, the canvas path and queries over
  * overlapping polygons run alone. Results equal rj_execute_query's exactly.
  * Returns RJ_OK when the batch ran; out[i].status holds each query's code. */
 int rj_execute_batch(rj_ctx* ctx, rj_table* t, int32_t nq, const rj_query* qs, rj_result* out);
@@ -191,7 +191,7 @@ int rj_execute_batch(rj_ctx* ctx, rj_table* t, int32_t nq, const rj_query* qs, r
  * per thread, workgroup counts, chunk sizes and, for queries with index_res 0,
  * grid resolutions on this batch, checks that every candidate returns the
  * same results, and keeps the fastest in the context. chosen (may be NULL):
- * 5 values, tile shift, rows per thread, workgroups, chunks per workgroup,
+ * 5 values, tile shift, rows per thread, workgroups, rows per chunk,
  * automatic grid resolution. */
 int rj_autotune(rj_ctx* ctx, rj_table* t, int32_t nq, const rj_query* qs, int32_t* chosen);
 
@@ -240,11 +240,11 @@ int rj_canvas_info(rj_ctx* ctx, int32_t* W, int32_t* H);
  *   RJ_OPT_PASS_WORKGROUPS: workgroups of a single-query point pass (0: 4096)
  *   RJ_OPT_BATCH_ROWS:      rows per thread of fused passes, 4 or 8
  *   RJ_OPT_BATCH_WORKGROUPS: persistent workgroups of fused passes (64..65536)
- *   RJ_OPT_BATCH_CHUNKS:    chunks per workgroup fused passes aim at (1..64)
+ *   RJ_OPT_BATCH_CHUNK_ROWS: rows per chunk of fused passes (1024 .. 2^22)
  *   RJ_OPT_GRAPH:           1 (default) fused passes replay a captured HIP graph */
 enum { RJ_OPT_RASTER_PATH = 1, RJ_OPT_TIMING = 2, RJ_OPT_HYBRID_DILATION = 3, RJ_OPT_CACHE = 4,
        RJ_OPT_ORGANIZE = 5, RJ_OPT_TILE_SHIFT = 6, RJ_OPT_PASS_WORKGROUPS = 7, RJ_OPT_BATCH_ROWS = 8,
-       RJ_OPT_BATCH_WORKGROUPS = 9, RJ_OPT_BATCH_CHUNKS = 10, RJ_OPT_GRAPH = 11 };
+       RJ_OPT_BATCH_WORKGROUPS = 9, RJ_OPT_BATCH_CHUNK_ROWS = 10, RJ_OPT_GRAPH = 11 };
 int rj_set_option(rj_ctx* ctx, int32_t key, int64_t value);
 
 /* Per-kernel device time of the last execute (timing on): name i, total ms and

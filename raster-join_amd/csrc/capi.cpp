@@ -439,9 +439,9 @@ int rj_set_option(rj_ctx* ctx, int32_t key, int64_t value)
         if (value < 64 || value > 65536) return ctx->fail(RJ_ERR_ARG, "workgroups 64 .. 65536");
         ctx->opt_batch_wg = (int)value;
         return RJ_OK;
-    case RJ_OPT_BATCH_CHUNKS:
-        if (value < 1 || value > 64) return ctx->fail(RJ_ERR_ARG, "chunks per workgroup 1 .. 64");
-        ctx->opt_batch_chunks = (int)value;
+    case RJ_OPT_BATCH_CHUNK_ROWS:
+        if (value < 1024 || value > (1 << 22)) return ctx->fail(RJ_ERR_ARG, "chunk rows 1024 .. 2^22");
+        ctx->opt_batch_chunk_rows = (int)value;
         return RJ_OK;
     case RJ_OPT_GRAPH: ctx->opt_graph = value != 0; return RJ_OK;
     default: return ctx->fail(RJ_ERR_ARG, "unknown option");

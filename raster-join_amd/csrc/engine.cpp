@@ -27,6 +27,24 @@ double now_ms()
 
 size_t al256(size_t b) { return (b + 255) / 256 * 256; }
 
+// Largest collection whose per-polygon sums a workgroup keeps in LDS indexed
+// by polygon (16 KiB); larger ones use the hashed slots.
+int lds_direct_cap(bool avg) { return avg ? 16384 / 24 : 16384 / 8; }
+
+// Accuracy that gives a query its canvas. The hybrid join is exact on any
+// canvas (border pixels take the exact test), so like the reference's outline
+// pass (HybridJoin::updateBuffers, HybridJoin.cpp:60-75) it uses a fixed
+// resolution, 8192 pixels on the long side of the collection's bbox, whatever
+// the accuracy: coarse accuracies would otherwise send most points to the
+// exact test.
+double canvas_eps(const QuerySpec& s)
+{
+    if (s.fn != RJ_HYBRID) return s.eps;
+    const double* b = s.P->ps.bbox;
+    const double ext = std::max(b[2] - b[0], b[3] - b[1]);
+    return ext / 8191.999 * std::sqrt(2.0) * s.ground_res;
+}
+
 // label-map tiles: columns LW >> tshift, rows ceil(H / 2^tshift); the tile
 // buffer ends with two u64 overlap counters
 size_t ntiles(const DevCanvas& c, int tshift)
@@ -410,12 +428,26 @@ int rj_ctx::grid_for(rj_polyset* P, int res, Grid** out)
     double t0 = now_ms();
     build_grid(P->ps, res, &G->g);
     cur_polyindex += now_ms() - t0;
+    // row bands in one buffer: band_lo | band_hi | pband | band_off
+    const GridIndex& gi = G->g;
+    const size_t np = gi.band_lo.size();
+    std::vector<int32_t> bands;
+    bands.reserve(3 * np + 1 + gi.band_off.size());
+    bands.insert(bands.end(), gi.band_lo.begin(), gi.band_lo.end());
+    bands.insert(bands.end(), gi.band_hi.begin(), gi.band_hi.end());
+    bands.insert(bands.end(), gi.pband.begin(), gi.pband.end());
+    bands.insert(bands.end(), gi.band_off.begin(), gi.band_off.end());
     int rc;
-    if ((rc = upload(G->d_cell_off, G->g.cell_off.data(), G->g.cell_off.size() * 4, "grid cells")) ||
-        (rc = upload(G->d_cand, G->g.cand.data(), G->g.cand.size() * 4, "grid candidates"))) {
+    if ((rc = upload(G->d_cell_off, gi.cell_off.data(), gi.cell_off.size() * 4, "grid cells")) ||
+        (rc = upload(G->d_cand, gi.cand.data(), gi.cand.size() * 4, "grid candidates")) ||
+        (rc = upload(G->d_bands, bands.data(), bands.size() * 4, "grid row bands")) ||
+        (rc = upload(G->d_edges, gi.edges.data(), gi.edges.size() * 4, "grid band edges"))) {
         P->grids.erase(res);
         return rc;
     }
+    // the host vectors above are read by the copies: wait before they go away
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "grid upload");
     for (auto& c : P->canvases)
         if (c.second->hyb_res == res) c.second->hyb_res = -1;
     *out = G;
@@ -550,7 +582,7 @@ int64_t rj_ctx::estimate_bytes(const QuerySpec& s) const
     }
     if (s.fn != RJ_INDEX) {
         CanvasPlan plan;
-        if (make_plan(s.P->ps.bbox, s.eps, s.ground_res, &plan)) return b;
+        if (make_plan(s.P->ps.bbox, canvas_eps(s), s.ground_res, &plan)) return b;
         auto it = s.P->canvases.find(plan.cell);
         const Canvas* C = it == s.P->canvases.end() ? nullptr : it->second.get();
         const int64_t LW = (plan.W + kLabelRowAlign - 1) & ~(kLabelRowAlign - 1);
@@ -601,6 +633,13 @@ DevGrid rj_ctx::dev_grid(const Grid* G) const
     g.minx = G->g.minx; g.miny = G->g.miny; g.width = G->g.width; g.height = G->g.height;
     g.cell_off = G->d_cell_off.as<int32_t>();
     g.cand = G->d_cand.as<int32_t>();
+    const int32_t* b = G->d_bands.as<int32_t>();
+    const size_t np = G->g.band_lo.size();
+    g.band_lo = b;
+    g.band_hi = b + np;
+    g.pband = b + 2 * np;
+    g.band_off = b + 3 * np + 1;
+    g.edges = G->d_edges.as<float4>();
     return g;
 }
 
@@ -733,7 +772,7 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
     pa.cols = dev_cols(t);
     pa.q = q;
     pa.res = res;
-    pa.lds_np = (np <= 2048) ? np : 0;
+    pa.lds_np = lds_direct_cap(avg);
     pa.cap = opt_cap;
     Canvas* C = nullptr;
     Grid* G = nullptr;
@@ -768,7 +807,7 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
         pa.P = dev_polys(P, nullptr);
         if ((rc = launch("points_index", [&] { launch_points(stream, PT_INDEX, pa); }))) return rc;
     } else if (fn == RJ_HYBRID) {
-        if ((rc = canvas_for(P, s.eps, s.ground_res, &C))) return rc;
+        if ((rc = canvas_for(P, canvas_eps(s), s.ground_res, &C))) return rc;
         if ((rc = grid_for(P, s.index_res, &G))) return rc;
         if ((rc = hybrid_labels(P, C, G))) return rc;
         pa.cv = dev_canvas(C->plan);
@@ -781,7 +820,7 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
         pa.tiles_w = pa.cv.LW >> pa.tshift;
         if ((rc = launch("points_hybrid", [&] { launch_points(stream, PT_LABEL_HYBRID, pa); }))) return rc;
     } else if (fn == RJ_RASTER) {
-        if ((rc = canvas_for(P, s.eps, s.ground_res, &C))) return rc;
+        if ((rc = canvas_for(P, canvas_eps(s), s.ground_res, &C))) return rc;
         const bool known_overlap = C->lab_built && cache && C->lab_state == 2;
         if (raster_path == 2 && known_overlap)
             return fail(RJ_ERR_UNSUPPORTED, "polygons overlap: the label-map path needs disjoint polygons");
@@ -803,7 +842,7 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
             if ((rc = canvas_join())) return rc;
         }
     } else {
-        if ((rc = canvas_for(P, s.eps, s.ground_res, &C))) return rc;
+        if ((rc = canvas_for(P, canvas_eps(s), s.ground_res, &C))) return rc;
         if (!C->pr.bounds_ok) return fail(RJ_ERR_UNSUPPORTED, "a polygon is too wide for the error-bounds kernel");
         const DevCanvas cv = dev_canvas(C->plan);
         const DevPolys dp = dev_polys(P, C);
@@ -949,17 +988,6 @@ int spec_from_query(rj_ctx* ctx, const rj_table* t, const rj_query* q, QuerySpec
     return rc;
 }
 
-// Columns a query reads, in a fixed order (constraints, then the aggregate).
-static void query_columns(const QuerySpec& s, std::vector<int>* cols)
-{
-    cols->clear();
-    auto add = [&](int c) {
-        if (std::find(cols->begin(), cols->end(), c) == cols->end()) cols->push_back(c);
-    };
-    for (auto& c : s.cons) add(c.attrib_id);
-    if (s.agg_type != RJ_AGG_COUNT && s.fn != RJ_ERRORBOUNDS) add(s.agg_col);
-}
-
 // Polygon structures of every query of a batch; sorts the queries into those
 // the fused pass runs and those that run alone (errorbounds, the canvas path).
 int rj_ctx::batch_prepare(rj_table* t, const QuerySpec* qs, int nq, std::vector<int>* fused,
@@ -978,9 +1006,7 @@ int rj_ctx::batch_prepare(rj_table* t, const QuerySpec* qs, int nq, std::vector<
             (*status)[i] = rc;
             continue;
         }
-        std::vector<int> cols;
-        query_columns(s, &cols);
-        if (s.fn == RJ_ERRORBOUNDS || (s.fn == RJ_RASTER && raster_path == 1) || (int)cols.size() > kBatchSlots) {
+        if (s.fn == RJ_ERRORBOUNDS || (s.fn == RJ_RASTER && raster_path == 1)) {
             alone->push_back(i);
             continue;
         }
@@ -989,11 +1015,11 @@ int rj_ctx::batch_prepare(rj_table* t, const QuerySpec* qs, int nq, std::vector<
         if (s.fn == RJ_INDEX) {
             rc = grid_for(s.P, s.index_res, &G);
         } else if (s.fn == RJ_HYBRID) {
-            rc = canvas_for(s.P, s.eps, s.ground_res, &C);
+            rc = canvas_for(s.P, canvas_eps(s), s.ground_res, &C);
             if (!rc) rc = grid_for(s.P, s.index_res, &G);
             if (!rc) rc = hybrid_labels(s.P, C, G);
         } else {
-            rc = canvas_for(s.P, s.eps, s.ground_res, &C);
+            rc = canvas_for(s.P, canvas_eps(s), s.ground_res, &C);
             if (!rc && C->lab_built && C->lab_state == 2) {
                 alone->push_back(i);
                 continue;
@@ -1059,21 +1085,6 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
     if (!S.done) (void)hipEventCreateWithFlags(&S.done, hipEventDisableTiming);
     BatchBlock& B = *S.h_block;
     std::memset(&B, 0, sizeof(B));
-    // batch slots: the columns of all members
-    std::vector<int> cols;
-    for (int i : members) {
-        std::vector<int> qc;
-        query_columns(qs[i], &qc);
-        for (int c : qc)
-            if (std::find(cols.begin(), cols.end(), c) == cols.end()) cols.push_back(c);
-    }
-    if ((int)cols.size() > kBatchSlots) return fail(RJ_ERR_LIMIT, "a fused pass reads at most 4 columns");
-    auto slot_of = [&](int c) { return (int)(std::find(cols.begin(), cols.end(), c) - cols.begin()); };
-    B.nslots = (int)cols.size();
-    for (size_t k = 0; k < cols.size(); k++) {
-        B.col[k] = t->cols[cols[k]];
-        B.type[k] = t->types[cols[k]];
-    }
     // results: 3 np words per member
     S.members = members;
     S.res_off.assign(members.size(), 0);
@@ -1089,58 +1100,53 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
     B.res = dres;
     B.res_words = words;
     B.nq = (int)members.size();
-    // queries
+    // per query: the arguments of its single-query pass over the table
     std::vector<int64_t> qb(members.size()), qe(members.size());
     for (size_t j = 0; j < members.size(); j++) {
         const QuerySpec& s = qs[members[j]];
-        BatchQuery& Q = B.q[j];
         rj_polyset* P = s.P;
         const int np = P->ps.np;
-        Q.avg = (s.agg_type != RJ_AGG_COUNT) ? 1 : 0;
-        Q.agg_slot = Q.avg ? slot_of(s.agg_col) : 0;
-        Q.agg_type = Q.avg ? t->types[s.agg_col] : RJ_ATTR_UINT;
-        Q.ncons = (int)s.cons.size();
-        for (int k = 0; k < Q.ncons; k++) {
-            Q.con_slot[k] = slot_of(s.cons[k].attrib_id);
-            Q.con_type[k] = t->types[s.cons[k].attrib_id];
-            Q.con_op[k] = s.cons[k].type;
-            Q.con_val[k] = s.cons[k].val.uval;
-        }
-        Q.P = dev_polys(P, nullptr);
+        PointArgs& a = B.q[j];
+        if ((rc = build_query(t, s, &a.q))) return rc;
+        a.xy = static_cast<const float*>(t->xy);
+        a.begin = 0;
+        a.end = t->n;
+        a.cols = dev_cols(t);
+        a.P = dev_polys(P, nullptr);
         Grid* G = nullptr;
         if (s.fn != RJ_RASTER) {
             auto gi = P->grids.find(s.index_res);
             if (gi == P->grids.end()) return fail(RJ_ERR_STATE, "grid of a fused query is not resident");
             G = gi->second.get();
+            a.g = dev_grid(G);
         }
         if (s.fn == RJ_INDEX) {
-            Q.mode = PT_INDEX;
-            Q.g = dev_grid(G);
+            B.mode[j] = PT_INDEX;
         } else {
             CanvasPlan plan;
-            make_plan(P->ps.bbox, s.eps, s.ground_res, &plan);
+            make_plan(P->ps.bbox, canvas_eps(s), s.ground_res, &plan);
             auto ci = P->canvases.find(plan.cell);
             if (ci == P->canvases.end()) return fail(RJ_ERR_STATE, "canvas of a fused query is not resident");
             Canvas* C = ci->second.get();
-            Q.cv = dev_canvas(C->plan);
+            a.cv = dev_canvas(C->plan);
+            a.P = dev_polys(P, C);
             if (s.fn == RJ_HYBRID) {
-                Q.mode = PT_LABEL_HYBRID;
-                Q.g = dev_grid(G);
-                Q.label = C->d_hlab.p;
-                Q.label16 = C->hyb16;
-                Q.tshift = C->hyb_tshift;
-                Q.tiles = C->d_htiles.as<uint32_t>();
+                B.mode[j] = PT_LABEL_HYBRID;
+                a.label = C->d_hlab.p;
+                a.label16 = C->hyb16;
+                a.tshift = C->hyb_tshift;
+                a.tiles = C->d_htiles.as<uint32_t>();
             } else {
-                Q.mode = PT_LABEL;
-                Q.label = C->d_lab.p;
-                Q.label16 = C->lab16;
-                Q.tshift = C->lab_tshift;
-                Q.tiles = C->d_tiles.as<uint32_t>();
+                B.mode[j] = PT_LABEL;
+                a.label = C->d_lab.p;
+                a.label16 = C->lab16;
+                a.tshift = C->lab_tshift;
+                a.tiles = C->d_tiles.as<uint32_t>();
             }
-            Q.tiles_w = Q.cv.LW >> Q.tshift;
+            a.tiles_w = a.cv.LW >> a.tshift;
         }
         unsigned long long* r = dres + S.res_off[j];
-        Q.res = DevResult{r, (long long*)(r + np), r + 2 * (size_t)np};
+        a.res = DevResult{r, (long long*)(r + np), r + 2 * (size_t)np};
         qb[j] = s.rb;
         qe[j] = s.re;
         narrow_rows(t, s, &qb[j], &qe[j]);
@@ -1168,10 +1174,12 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
         }
         union_rows += cut[k + 1] - cut[k];
     }
+    // chunks: opt_batch_chunk_rows rows (whole row groups), fewer when the
+    // union is too small to give every workgroup one
     const int rows = opt_batch_rows;
     const int64_t group = 256LL * rows;
-    int64_t chunk = union_rows / std::max<int64_t>(1, (int64_t)opt_batch_wg * opt_batch_chunks);
-    chunk = std::max<int64_t>(group * 4, (chunk + group - 1) / group * group);
+    int64_t chunk = std::min<int64_t>(opt_batch_chunk_rows, union_rows / std::max(1, opt_batch_wg));
+    chunk = std::max<int64_t>(group, (chunk + group - 1) / group * group);
     int64_t nchunks = 0;
     for (auto& g : segs) {
         g.chunk0 = nchunks;
@@ -1182,21 +1190,27 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
     B.nchunks = nchunks;
     B.chunk_rows = chunk;
 
-    const float* xy = static_cast<const float*>(t->xy);
+    // LDS sums: direct for small collections, hashed for the others; sized
+    // for the largest need among the members
+    const int cap = lds_direct_cap(true);
+    B.lds_cap = cap;
+    size_t lds = 0;
+    for (int i : members) lds = std::max(lds, acc_lds_bytes(qs[i].P->ps.np, cap, true));
+    const void* xy = t->xy;
     const int nwg = opt_batch_wg;
     const BatchBlock* dblock = S.d_block.as<BatchBlock>();
     if (opt_graph) {
         // the pass as one captured HIP graph: block upload, result clear and
         // the fused kernel; everything that varies lives in the block
         if (!S.graph || S.graph_key[0] != xy || S.graph_key[1] != dblock || S.graph_rows != rows ||
-            S.graph_wg != nwg || S.graph_nrows != t->n) {
+            S.graph_wg != nwg || S.graph_nrows != t->n || S.graph_lds != lds) {
             if (S.graph) (void)hipGraphExecDestroy(S.graph);
             S.graph = nullptr;
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
             if (e != hipSuccess) return hip_fail(e, "graph capture");
             (void)hipMemcpyAsync(S.d_block.p, S.h_block, sizeof(BatchBlock), hipMemcpyHostToDevice, stream);
-            launch_batch(stream, dblock, counter, xy, t->n, rows, nwg);
+            launch_batch(stream, dblock, counter, rows, nwg, lds);
             e = hipStreamEndCapture(stream, &g);
             if (e != hipSuccess) return hip_fail(e, "graph capture");
             e = hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0);
@@ -1210,12 +1224,13 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
             S.graph_rows = rows;
             S.graph_wg = nwg;
             S.graph_nrows = t->n;
+            S.graph_lds = lds;
         }
         if ((rc = launch("batch", [&] { (void)hipGraphLaunch(S.graph, stream); }))) return rc;
     } else {
         hipError_t e = hipMemcpyAsync(S.d_block.p, S.h_block, sizeof(BatchBlock), hipMemcpyHostToDevice, stream);
         if (e != hipSuccess) return hip_fail(e, "batch block");
-        if ((rc = launch("batch", [&] { launch_batch(stream, dblock, counter, xy, t->n, rows, nwg); }))) return rc;
+        if ((rc = launch("batch", [&] { launch_batch(stream, dblock, counter, rows, nwg, lds); }))) return rc;
     }
     if (words) {
         hipError_t e = hipMemcpyAsync(S.h_res, S.d_res.p, (size_t)words * 8, hipMemcpyDeviceToHost, stream);
@@ -1248,32 +1263,12 @@ int rj_ctx::batch_finish(int slot, const QuerySpec* qs, QueryOut* outs)
     return RJ_OK;
 }
 
-// Group fused queries into passes of at most 64 queries reading at most 4
-// columns (first fit, in query order).
-static void group_passes(const QuerySpec* qs, const std::vector<int>& fused, int max_batch,
-                         std::vector<std::vector<int>>* passes)
+// Fused queries in passes of at most max_batch queries, in query order.
+static void group_passes(const std::vector<int>& fused, int max_batch, std::vector<std::vector<int>>* passes)
 {
     passes->clear();
-    std::vector<std::vector<int>> pcols;
-    for (int i : fused) {
-        std::vector<int> qc;
-        query_columns(qs[i], &qc);
-        bool placed = false;
-        for (size_t p = 0; p < passes->size() && !placed; p++) {
-            if ((int)(*passes)[p].size() >= max_batch) continue;
-            std::vector<int> u = pcols[p];
-            for (int c : qc)
-                if (std::find(u.begin(), u.end(), c) == u.end()) u.push_back(c);
-            if ((int)u.size() > kBatchSlots) continue;
-            (*passes)[p].push_back(i);
-            pcols[p] = u;
-            placed = true;
-        }
-        if (!placed) {
-            passes->push_back({i});
-            pcols.push_back(qc);
-        }
-    }
+    for (size_t k = 0; k < fused.size(); k += (size_t)max_batch)
+        passes->emplace_back(fused.begin() + k, fused.begin() + std::min(fused.size(), k + (size_t)max_batch));
 }
 
 int rj_ctx::execute_batch(rj_table* t, const QuerySpec* qs, int nq, QueryOut* outs, int* status)
@@ -1284,7 +1279,7 @@ int rj_ctx::execute_batch(rj_table* t, const QuerySpec* qs, int nq, QueryOut* ou
     int rc = batch_prepare(t, qs, nq, &fused, &alone, &st);
     if (rc) return rc;
     std::vector<std::vector<int>> passes;
-    group_passes(qs, fused, kBatchMax, &passes);
+    group_passes(fused, kBatchMax, &passes);
     for (size_t p = 0; p < passes.size() && !rc; p++) {
         const int slot = (int)(p & 1);
         rc = batch_finish(slot, qs, outs);
@@ -1326,7 +1321,7 @@ int rj_ctx::autotune(rj_table* t, const QuerySpec* qs_in, int nq, const std::vec
     const bool saved_timing = timing;
     timing = true;
     const int base_tshift = opt_tshift, base_rows = opt_batch_rows, base_wg = opt_batch_wg,
-              base_chunks = opt_batch_chunks, base_res = opt_auto_res;
+              base_chunks = opt_batch_chunk_rows, base_res = opt_auto_res;
     std::vector<QueryOut> ref(nq), cur(nq);
     std::vector<int> st(nq);
     bool have_ref = false;
@@ -1356,7 +1351,7 @@ int rj_ctx::autotune(rj_table* t, const QuerySpec* qs_in, int nq, const std::vec
     const int tsh[3] = {3, 4, 5};
     const int rws[2] = {4, 8};
     const int wgs[3] = {1024, 2048, 4096};
-    const int chs[3] = {2, 4, 8};
+    const int chs[3] = {4096, 8192, 16384};
     for (int a = 0; a < 3 && !rc; a++) {
         opt_tshift = tsh[a];
         for (int b = 0; b < 2 && !rc; b++)
@@ -1364,7 +1359,7 @@ int rj_ctx::autotune(rj_table* t, const QuerySpec* qs_in, int nq, const std::vec
                 for (int d = 0; d < 3 && !rc; d++) {
                     opt_batch_rows = rws[b];
                     opt_batch_wg = wgs[c];
-                    opt_batch_chunks = chs[d];
+                    opt_batch_chunk_rows = chs[d];
                     double ms = 0;
                     if ((rc = run(&ms))) break;
                     if (!have_ref) {
@@ -1383,7 +1378,7 @@ int rj_ctx::autotune(rj_table* t, const QuerySpec* qs_in, int nq, const std::vec
     opt_tshift = bestc[0];
     opt_batch_rows = bestc[1];
     opt_batch_wg = bestc[2];
-    opt_batch_chunks = bestc[3];
+    opt_batch_chunk_rows = bestc[3];
     if (!rc && !auto_res_q.empty()) {
         // grid resolution of queries that leave it to the engine: a
         // resolution is kept only where the results equal those at 256
@@ -1412,7 +1407,7 @@ int rj_ctx::autotune(rj_table* t, const QuerySpec* qs_in, int nq, const std::vec
         opt_tshift = base_tshift;
         opt_batch_rows = base_rows;
         opt_batch_wg = base_wg;
-        opt_batch_chunks = base_chunks;
+        opt_batch_chunk_rows = base_chunks;
         opt_auto_res = base_res;
         return rc;
     }

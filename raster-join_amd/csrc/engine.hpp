@@ -65,9 +65,9 @@ struct Canvas {
 
 struct Grid {
     GridIndex g;
-    DevBuf d_cell_off, d_cand;
+    DevBuf d_cell_off, d_cand, d_bands, d_edges;  // d_bands: band_lo | band_hi | pband | band_off
     uint64_t last_use = 0;
-    size_t device_bytes() const { return d_cell_off.bytes + d_cand.bytes; }
+    size_t device_bytes() const { return d_cell_off.bytes + d_cand.bytes + d_bands.bytes + d_edges.bytes; }
 };
 
 // One query as the engine runs it.
@@ -159,7 +159,7 @@ struct rj_ctx {
     int opt_tshift = rj::kTileShiftDefault;  // label tile side 2^tshift
     int opt_batch_rows = 8;                // rows per thread in fused passes
     int opt_batch_wg = 2048;               // persistent workgroups of a fused pass
-    int opt_batch_chunks = 4;              // chunks per workgroup a fused pass aims at
+    int opt_batch_chunk_rows = 8192;       // rows per chunk of a fused pass
     bool opt_graph = true;                 // fused passes replay a captured HIP graph
     int opt_auto_res = 256;                // grid resolution of queries that leave it open
 
@@ -186,6 +186,7 @@ struct rj_ctx {
         const void* graph_key[2] = {nullptr, nullptr};
         int graph_rows = 0, graph_wg = 0;
         int64_t graph_nrows = 0;
+        size_t graph_lds = 0;
         hipEvent_t done = nullptr;
         std::vector<int> members;  // query indices of the pass
         std::vector<int64_t> res_off;

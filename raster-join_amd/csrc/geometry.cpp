@@ -291,6 +291,84 @@ void parallel_polys(int np, F f)
 
 }  // namespace
 
+// Grid row of a y coordinate, as the point pass computes it (getYIndex,
+// GridIndexF.cpp:44-47, with the in-grid test of getRegion): -1 below the
+// grid, ys above it. Monotone in t.
+static int grid_row(float t, float miny, float height, int ys)
+{
+    const float fy = (t - miny) / height;
+    if (!(fy > -1.0f)) return -1;
+    if (!(fy < (float)ys)) return ys;
+    return (int)fy;
+}
+
+// Row bands of every polygon (see GridIndex). Edge (vi, vj) toggles the
+// crossing test of a point at ty iff min(vi.y, vj.y) <= ty < max(vi.y, vj.y)
+// ((vi.y > ty) != (vj.y > ty)); the rows of those ty run from
+// grid_row(min) to grid_row(the largest float below max), as grid_row is
+// monotone.
+static void build_bands(const PolySet& ps, GridIndex* g)
+{
+    const int ys = g->ys;
+    const float miny = g->miny, height = g->height;
+    const int np = ps.np;
+    g->band_lo.assign(np, 0);
+    g->band_hi.assign(np, -1);
+    g->pband.assign(np + 1, 0);
+    std::vector<std::vector<int32_t>> lo_hi(np);  // per polygon: (row0, row1) per edge
+    parallel_polys(np, [&](int t, int nt) {
+        for (int p = t; p < np; p += nt) {
+            const float* v = ps.xy.data() + 2 * (size_t)ps.off[p];
+            const int n = ps.off[p + 1] - ps.off[p];
+            std::vector<int32_t>& r = lo_hi[p];
+            r.resize(2 * (size_t)n);
+            int blo = ys, bhi = -1;
+            for (int i = 0; i < n; i++) {
+                const int j = i ? i - 1 : n - 1;
+                const float y0 = std::min(v[2 * i + 1], v[2 * j + 1]);
+                const float y1 = std::max(v[2 * i + 1], v[2 * j + 1]);
+                int r0 = 0, r1 = -1;
+                if (y0 < y1) {
+                    r0 = std::max(0, grid_row(y0, miny, height, ys));
+                    r1 = std::min(ys - 1, grid_row(std::nextafter(y1, -INFINITY), miny, height, ys));
+                }
+                r[2 * i] = r0;
+                r[2 * i + 1] = r1;
+                if (r0 <= r1) {
+                    blo = std::min(blo, r0);
+                    bhi = std::max(bhi, r1);
+                }
+            }
+            g->band_lo[p] = bhi >= blo ? blo : 0;
+            g->band_hi[p] = bhi >= blo ? bhi : -1;
+        }
+    });
+    for (int p = 0; p < np; p++) g->pband[p + 1] = g->pband[p] + (g->band_hi[p] - g->band_lo[p] + 1);
+    const size_t nbands = (size_t)g->pband[np];
+    g->band_off.assign(nbands + 1, 0);
+    for (int p = 0; p < np; p++) {
+        const int n = ps.off[p + 1] - ps.off[p];
+        for (int i = 0; i < n; i++)
+            for (int y = lo_hi[p][2 * i]; y <= lo_hi[p][2 * i + 1]; y++)
+                g->band_off[(size_t)g->pband[p] + (y - g->band_lo[p]) + 1]++;
+    }
+    for (size_t b = 0; b < nbands; b++) g->band_off[b + 1] += g->band_off[b];
+    g->edges.assign(4 * (size_t)g->band_off[nbands], 0.0f);
+    std::vector<int32_t> fill(g->band_off.begin(), g->band_off.end() - 1);
+    for (int p = 0; p < np; p++) {
+        const float* v = ps.xy.data() + 2 * (size_t)ps.off[p];
+        const int n = ps.off[p + 1] - ps.off[p];
+        for (int i = 0; i < n; i++) {
+            const int j = i ? i - 1 : n - 1;
+            for (int y = lo_hi[p][2 * i]; y <= lo_hi[p][2 * i + 1]; y++) {
+                float* e = &g->edges[4 * (size_t)fill[(size_t)g->pband[p] + (y - g->band_lo[p])]++];
+                e[0] = v[2 * i]; e[1] = v[2 * i + 1];   // vi
+                e[2] = v[2 * j]; e[3] = v[2 * j + 1];   // vj (the previous vertex)
+            }
+        }
+    }
+}
+
 void build_grid(const PolySet& ps, int res, GridIndex* g)
 {
     g->xs = g->ys = res;
@@ -349,6 +427,7 @@ void build_grid(const PolySet& ps, int res, GridIndex* g)
     for (auto& d : defi) g->deficient.insert(g->deficient.end(), d.begin(), d.end());
     std::sort(g->deficient.begin(), g->deficient.end());
     g->deficient.erase(std::unique(g->deficient.begin(), g->deficient.end()), g->deficient.end());
+    build_bands(ps, g);
 }
 
 void deficient_rects(const GridIndex& g, const CanvasPlan& plan, std::vector<int32_t>* rects)

@@ -389,163 +389,27 @@ __global__ void k_tile_labels(const void* __restrict__ label, int label16, int L
 // point pass: index join, label-map raster join, hybrid join, canvas splat
 // ---------------------------------------------------------------------------
 
-// One pass over rows [begin, end). Every thread takes kRows consecutive rows
-// per iteration (vector loads of x,y and of the query's columns), computes one
-// key per row (polygon id, or pixel for PT_SPLAT) and keeps a running
-// (key, count, sum) that it flushes only when the key changes: rows of a
-// spatially ordered table mostly repeat the key, so flushes are rare.
+// One pass over rows [begin, end): each workgroup takes a contiguous block of
+// rows (point_rows) and sums per polygon in LDS (acc_init / acc_flush).
 template <int MODE, bool AVG>
 __global__ __launch_bounds__(kBlock) void k_points(PointArgs a)
 {
     extern __shared__ unsigned long long sacc[];
-    const DevQuery& q = a.q;
     Acc A;
-    A.lds_np = (MODE == PT_SPLAT) ? 0 : a.lds_np;
-    A.cnt = sacc;
-    A.hi = sacc + A.lds_np;
-    A.lo = sacc + 2 * A.lds_np;
-    A.g = a.res;
-    const int nacc = AVG ? 3 * A.lds_np : A.lds_np;
-    for (int k = threadIdx.x; k < nacc; k += blockDim.x) sacc[k] = 0;
-    __syncthreads();
-
+    if (MODE == PT_SPLAT) {
+        A.cnt = A.hi = A.lo = nullptr;
+        A.tag = nullptr;
+        A.g = a.res;
+        A.lds_np = A.hash = 0;
+    } else {
+        acc_init(A, sacc, a.P.np, a.lds_np, AVG, a.res);
+    }
     const int64_t b0 = a.start + (int64_t)blockIdx.x * a.per_block;
     const int64_t b1 = min(a.end, b0 + a.per_block);
-    const int64_t lo_row = max(a.begin, b0);
-    int ck = -1;
-    uint32_t cc = 0;
-    unsigned long long ch = 0, cl = 0;
-    const int agg_t = AVG ? a.cols.type[q.slot_col[q.agg_slot < 0 ? 0 : q.agg_slot]] : RJ_ATTR_UINT;
-    // x,y of the next row group are loaded one iteration ahead: the tile
-    // gather needs only x,y and then overlaps the column loads of its group.
-    const int64_t stride = (int64_t)blockDim.x * kRows;
-    float nx[kRows], ny[kRows];
-    load_xy(a.xy, b0 + (int64_t)threadIdx.x * kRows, b1, a.end, nx, ny);
-    for (int64_t base = b0; base < b1; base += stride) {
-        const int64_t i0 = base + (int64_t)threadIdx.x * kRows;
-        float px[kRows], py[kRows];
-#pragma unroll
-        for (int r = 0; r < kRows; r++) {
-            px[r] = nx[r];
-            py[r] = ny[r];
-        }
-        load_xy(a.xy, i0 + stride, b1, a.end, nx, ny);
-        if (i0 >= b1) continue;
-        const bool full = (i0 + kRows <= a.end);
-        bool ok[kRows];
-#pragma unroll
-        for (int r = 0; r < kRows; r++) ok[r] = (i0 + r >= lo_row) && (i0 + r < b1);
-        // raw 32-bit aggregate values; decoded to fixed point only when added
-        uint32_t vraw[kRows];
-#pragma unroll
-        for (int r = 0; r < kRows; r++) vraw[r] = 0;
-#pragma unroll
-        for (int s = 0; s < kMaxSlots; s++) {
-            if (s >= q.nslots) break;
-            uint32_t v[kRows];
-            const int col = q.slot_col[s];
-            load_rows(a.cols, col, i0, full, a.end, v);
-            const int t = a.cols.type[col];
-            for (int k = 0; k < q.ncons; k++) {
-                if (q.con_slot[k] != s) continue;
-                const int op = q.con_op[k];
-                const uint32_t cv = q.con_val[k];
-#pragma unroll
-                for (int r = 0; r < kRows; r++) ok[r] = ok[r] && cmp_bits(t, op, v[r], cv);
-            }
-            if (AVG && s == q.agg_slot) {
-#pragma unroll
-                for (int r = 0; r < kRows; r++) vraw[r] = v[r];
-            }
-        }
-        int key[kRows];
-        if (MODE == PT_INDEX) {
-#pragma unroll
-            for (int r = 0; r < kRows; r++)
-                key[r] = ok[r] ? exact_first<AVG>(a.g, a.P, A, px[r], py[r], agg_t, vraw[r]) : -1;
-        } else {
-            int pix[kRows], ixs[kRows], iys[kRows];
-#pragma unroll
-            for (int r = 0; r < kRows; r++) {
-                pix[r] = pixel_xy(a.cv, px[r], py[r], ixs[r], iys[r]);
-                if (!ok[r]) pix[r] = -1;
-            }
-            if (MODE == PT_SPLAT) {
-#pragma unroll
-                for (int r = 0; r < kRows; r++) key[r] = pix[r];
-            } else {
-                uint32_t lab[kRows];
-                // the tile map answers for whole tiles; mixed tiles read the pixel
-#pragma unroll
-                for (int r = 0; r < kRows; r++) {
-                    if (pix[r] < 0) {
-                        lab[r] = kLabelNone;
-                    } else {
-                        lab[r] = a.tiles[(iys[r] >> a.tshift) * a.tiles_w + (ixs[r] >> a.tshift)];
-                    }
-                }
-#pragma unroll
-                for (int r = 0; r < kRows; r++)
-                    if (lab[r] == kLabelMixed) lab[r] = label_at(a.label, a.label16, (int64_t)iys[r] * a.cv.LW + ixs[r]);
-#pragma unroll
-                for (int r = 0; r < kRows; r++) {
-                    key[r] = lab[r] < kLabelMaxId ? (int)lab[r] : -1;
-                    if (MODE == PT_LABEL_HYBRID && lab[r] == kLabelBorder)
-                        key[r] = exact_first<AVG>(a.g, a.P, A, px[r], py[r], agg_t, vraw[r]);
-                }
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < kRows; r++) {
-            const int k = key[r];
-            if (k < 0) continue;
-            if (k != ck) {
-                if (ck >= 0) {
-                    if (MODE == PT_SPLAT) {
-                        atomicAdd(&a.cnt[ck], (uint32_t)cc);
-                        if (AVG) {
-                            atomicAdd(&a.chi[ck], ch);
-                            atomicAdd(&a.clo[ck], cl);
-                        }
-                    } else {
-                        acc_add(A, ck, cc, ch, cl, AVG);
-                    }
-                }
-                ck = k;
-                cc = ch = cl = 0;
-            }
-            cc++;
-            if (AVG) {
-                unsigned long long h, l;
-                agg_bits(agg_t, vraw[r], h, l);
-                ch += h;
-                cl += l;
-            }
-        }
-    }
-    if (ck >= 0) {
-        if (MODE == PT_SPLAT) {
-            atomicAdd(&a.cnt[ck], (uint32_t)cc);
-            if (AVG) {
-                atomicAdd(&a.chi[ck], ch);
-                atomicAdd(&a.clo[ck], cl);
-            }
-        } else {
-            acc_add(A, ck, cc, ch, cl, AVG);
-        }
-    }
-    if (MODE == PT_SPLAT) return;
-    __syncthreads();
-    for (int k = threadIdx.x; k < A.lds_np; k += blockDim.x) {
-        const unsigned long long c = sacc[k];
-        if (c) {
-            atomicAdd(&a.res.cnt[k], c);
-            if (AVG) {
-                atomicAdd((unsigned long long*)&a.res.hi[k], sacc[A.lds_np + k]);
-                atomicAdd(&a.res.lo[k], sacc[2 * A.lds_np + k]);
-            }
-        }
-    }
+    Run run{-1, 0, 0, 0};
+    point_rows<MODE, AVG, kRows>(a, b0, max(a.begin, b0), b1, A, run);
+    if (run.ck >= 0) run_flush<MODE, AVG>(a, A, run.ck, run.cc, run.ch, run.cl);
+    if (MODE != PT_SPLAT) acc_flush(A, AVG);
 }
 
 inline int blocks_for(int64_t n, int64_t per, int cap)
@@ -666,7 +530,7 @@ void launch_points(hipStream_t s, int mode, PointArgs a)
     a.per_block = per;
     const int blocks = (int)((n + per - 1) / per);
     const bool avg = a.q.agg_type != RJ_AGG_COUNT;
-    const size_t lds = (mode == PT_SPLAT) ? 0 : (size_t)(avg ? 3 : 1) * a.lds_np * sizeof(unsigned long long);
+    const size_t lds = (mode == PT_SPLAT) ? 0 : acc_lds_bytes(a.P.np, a.lds_np, avg);
 #define RJ_LAUNCH_PTS(M, A) \
     hipLaunchKernelGGL((k_points<M, A>), dim3(blocks), dim3(kBlock), lds, s, a)
     switch (mode) {

@@ -163,25 +163,31 @@ __device__ __forceinline__ int pixel_xy(const DevCanvas& c, float x, float y, in
 }
 
 // CPUJoin getXIndex/getYIndex (GridIndexF.cpp:39-47) with the in-grid test of
-// getRegion (:163); (int) truncation done only on in-range values.
-__device__ __forceinline__ int grid_cell(const DevGrid& g, float x, float y)
+// getRegion (:163); (int) truncation done only on in-range values. row
+// receives the cell's grid row.
+__device__ __forceinline__ int grid_cell(const DevGrid& g, float x, float y, int& row)
 {
     float fx = (x - g.minx) / g.width;
     float fy = (y - g.miny) / g.height;
+    row = 0;
     if (!(fx > -1.0f && fx < (float)g.xs && fy > -1.0f && fy < (float)g.ys)) return -1;
-    return (int)fy * g.xs + (int)fx;
+    row = (int)fy;
+    return row * g.xs + (int)fx;
 }
 
-// Crossing-number PIP, GridIndexF.cpp:383-402 (same operation order).
-__device__ __forceinline__ bool pip_f32(const float2* __restrict__ v, int n, float tx, float ty)
+// Crossing-number PIP (GridIndexF.cpp:383-402, the same operation order per
+// edge) over the edges of one row band of a polygon: the edges a point of that
+// row can cross, so the parity is that of the loop over all edges.
+__device__ __forceinline__ bool pip_band(const DevGrid& g, int p, int row, float tx, float ty)
 {
+    const int lo = g.band_lo[p];
+    if (row < lo || row > g.band_hi[p]) return false;
+    const int b = g.pband[p] + (row - lo);
+    const int k1 = g.band_off[b + 1];
     bool c = false;
-    float2 vj = v[n - 1];
-    for (int i = 0; i < n; i++) {
-        float2 vi = v[i];
-        if (((vi.y > ty) != (vj.y > ty)) && (tx < (vj.x - vi.x) * (ty - vi.y) / (vj.y - vi.y) + vi.x))
-            c = !c;
-        vj = vi;
+    for (int k = g.band_off[b]; k < k1; k++) {
+        const float4 e = g.edges[k];  // vi = (x, y), vj = (z, w)
+        if (((e.y > ty) != (e.w > ty)) && (tx < (e.z - e.x) * (ty - e.y) / (e.w - e.y) + e.x)) c = !c;
     }
     return c;
 }
@@ -227,58 +233,259 @@ __device__ __forceinline__ uint32_t label_at(const void* label, int label16, int
     return static_cast<const uint32_t*>(label)[pix];
 }
 
-// Per-polygon accumulators: the first lds_np polygons in LDS, the rest (or
-// all, lds_np = 0) straight into the result arrays.
+// Per-polygon accumulators of a workgroup in LDS, added to the result arrays
+// at the end (acc_flush). Direct: polygon k < lds_np at slot k. Hashed (for
+// collections too large to index directly): kHashSlots tagged slots; a
+// workgroup's rows are spatially close and meet few polygons, so nearly every
+// key finds its slot, and a key whose slot holds another polygon goes to the
+// result arrays directly.
+constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;
+
 struct Acc {
-    unsigned long long* cnt;  // LDS (lds_np entries)
+    unsigned long long* cnt;  // LDS
     unsigned long long* hi;
     unsigned long long* lo;
+    uint32_t* tag;            // hashed mode: the polygon of each slot
     DevResult g;
-    int lds_np;
+    int lds_np;               // direct mode: polygons in LDS (0: none)
+    int hash;                 // 1: hashed mode
 };
 
+__device__ __forceinline__ int hash_slot(int key)
+{
+    return (int)(((uint32_t)key * 2654435761u) >> (32 - 9));
+}
+static_assert(kHashSlots == 512, "hash_slot yields 9 bits");
+
+// Point A at the LDS buffer `sacc` (direct: np <= lds_cap; else hashed) and
+// clear it. Every thread of the workgroup calls it; a barrier follows inside.
+__device__ __forceinline__ void acc_init(Acc& A, unsigned long long* sacc, int np, int lds_cap, bool avg,
+                                         DevResult res)
+{
+    A.g = res;
+    A.hash = np > lds_cap ? 1 : 0;
+    A.lds_np = A.hash ? 0 : np;
+    const int slots = A.hash ? kHashSlots : np;
+    A.cnt = sacc;
+    A.hi = sacc + slots;
+    A.lo = sacc + 2 * slots;
+    A.tag = reinterpret_cast<uint32_t*>(sacc + (avg ? 3 : 1) * slots);
+    const int n = (avg ? 3 : 1) * slots;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) sacc[k] = 0;
+    if (A.hash)
+        for (int k = threadIdx.x; k < kHashSlots; k += blockDim.x) A.tag[k] = kHashEmpty;
+    __syncthreads();
+}
+
+template <bool LDS = true>
 __device__ __forceinline__ void acc_add(const Acc& A, int key, unsigned long long c, unsigned long long hi,
                                         unsigned long long lo, bool avg)
 {
-    if (key < A.lds_np) {
-        atomicAdd(&A.cnt[key], c);
-        if (avg) {
-            atomicAdd(&A.hi[key], hi);
-            atomicAdd(&A.lo[key], lo);
+    if (LDS) {
+        int slot = -1;
+        if (key < A.lds_np) {
+            slot = key;
+        } else if (A.hash) {
+            const int h = hash_slot(key);
+            uint32_t t = A.tag[h];
+            if (t == kHashEmpty) {
+                const uint32_t old = atomicCAS(&A.tag[h], kHashEmpty, (uint32_t)key);
+                t = old == kHashEmpty ? (uint32_t)key : old;
+            }
+            if (t == (uint32_t)key) slot = h;
         }
-    } else {
+        if (slot >= 0) {
+            atomicAdd(&A.cnt[slot], c);
+            if (avg) {
+                atomicAdd(&A.hi[slot], hi);
+                atomicAdd(&A.lo[slot], lo);
+            }
+            return;
+        }
+    }
+    atomicAdd(&A.g.cnt[key], c);
+    if (avg) {
+        atomicAdd((unsigned long long*)&A.g.hi[key], hi);
+        atomicAdd(&A.g.lo[key], lo);
+    }
+}
+
+// The workgroup's LDS sums to the result arrays (after a barrier inside).
+__device__ __forceinline__ void acc_flush(const Acc& A, bool avg)
+{
+    __syncthreads();
+    const int slots = A.hash ? kHashSlots : A.lds_np;
+    for (int k = threadIdx.x; k < slots; k += blockDim.x) {
+        const unsigned long long c = A.cnt[k];
+        if (!c) continue;
+        const int key = A.hash ? (int)A.tag[k] : k;
         atomicAdd(&A.g.cnt[key], c);
         if (avg) {
-            atomicAdd((unsigned long long*)&A.g.hi[key], hi);
-            atomicAdd(&A.g.lo[key], lo);
+            atomicAdd((unsigned long long*)&A.g.hi[key], A.hi[k]);
+            atomicAdd(&A.g.lo[key], A.lo[k]);
         }
     }
 }
 
 // CPUJoin getRegion over the cell's candidates. Returns the first polygon that
 // contains the point; further containing polygons are added directly.
-template <bool AVG>
+template <bool AVG, bool LDS = true>
 __device__ __forceinline__ int exact_first(const DevGrid& g, const DevPolys& P, const Acc& A, float x,
                                            float y, int agg_t, uint32_t vraw)
 {
-    const int cell = grid_cell(g, x, y);
+    int row;
+    const int cell = grid_cell(g, x, y, row);
     if (cell < 0) return -1;
     int first = -1;
     const int k1 = g.cell_off[cell + 1];
     for (int k = g.cell_off[cell]; k < k1; k++) {
         const int p = g.cand[k];
-        const int v0 = P.off[p];
-        if (pip_f32(reinterpret_cast<const float2*>(P.xy) + v0, P.off[p + 1] - v0, x, y)) {
+        if (pip_band(g, p, row, x, y)) {
             if (first < 0) {
                 first = p;
             } else {
                 unsigned long long h = 0, l = 0;
                 if (AVG) agg_bits(agg_t, vraw, h, l);
-                acc_add(A, p, 1ull, h, l, AVG);
+                acc_add<LDS>(A, p, 1ull, h, l, AVG);
             }
         }
     }
     return first;
+}
+
+// A lane's running (key, count, sum) of a point pass.
+struct Run {
+    int ck;
+    uint32_t cc;
+    unsigned long long ch, cl;
+};
+
+template <int MODE, bool AVG, bool LDS = true>
+__device__ __forceinline__ void run_flush(const PointArgs& a, const Acc& A, int ck, uint32_t cc,
+                                          unsigned long long ch, unsigned long long cl)
+{
+    if (MODE == PT_SPLAT) {
+        atomicAdd(&a.cnt[ck], cc);
+        if (AVG) {
+            atomicAdd(&a.chi[ck], ch);
+            atomicAdd(&a.clo[ck], cl);
+        }
+    } else {
+        acc_add<LDS>(A, ck, cc, ch, cl, AVG);
+    }
+}
+
+// Rows [lo, hi) of one point pass, read in groups of R consecutive rows per
+// thread from `start` (a multiple of R, <= lo). Every row gets one key
+// (polygon id, or pixel for PT_SPLAT) and the lane keeps a running
+// (key, count, sum) that it flushes only when the key changes: rows of a
+// spatially ordered table mostly repeat the key, so flushes are rare. x,y of
+// the next row group are loaded one iteration ahead: the tile gather needs
+// only x,y and then overlaps the column loads of its group. a.end is the last
+// row the loads may touch. LDS = false: every sum goes to the result arrays
+// (A.lds_np is ignored).
+template <int MODE, bool AVG, int R, bool LDS = true>
+__device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, int64_t lo, int64_t hi, const Acc& A,
+                                           Run& run)
+{
+    const DevQuery& q = a.q;
+    const int agg_t = AVG ? a.cols.type[q.slot_col[q.agg_slot < 0 ? 0 : q.agg_slot]] : RJ_ATTR_UINT;
+    const int64_t stride = (int64_t)blockDim.x * R;
+    float nx[R], ny[R];
+    load_xy<R>(a.xy, start + (int64_t)threadIdx.x * R, hi, a.end, nx, ny);
+    for (int64_t base = start; base < hi; base += stride) {
+        const int64_t i0 = base + (int64_t)threadIdx.x * R;
+        float px[R], py[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            px[r] = nx[r];
+            py[r] = ny[r];
+        }
+        load_xy<R>(a.xy, i0 + stride, hi, a.end, nx, ny);
+        if (i0 >= hi) continue;
+        const bool full = (i0 + R <= a.end);
+        bool ok[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) ok[r] = (i0 + r >= lo) && (i0 + r < hi);
+        // raw 32-bit aggregate values; decoded to fixed point only when added
+        uint32_t vraw[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) vraw[r] = 0;
+#pragma unroll
+        for (int s = 0; s < kMaxSlots; s++) {
+            if (s >= q.nslots) break;
+            uint32_t v[R];
+            const int col = q.slot_col[s];
+            load_rows<R>(a.cols, col, i0, full, a.end, v);
+            const int t = a.cols.type[col];
+            for (int k = 0; k < q.ncons; k++) {
+                if (q.con_slot[k] != s) continue;
+                const int op = q.con_op[k];
+                const uint32_t cv = q.con_val[k];
+#pragma unroll
+                for (int r = 0; r < R; r++) ok[r] = ok[r] && cmp_bits(t, op, v[r], cv);
+            }
+            if (AVG && s == q.agg_slot) {
+#pragma unroll
+                for (int r = 0; r < R; r++) vraw[r] = v[r];
+            }
+        }
+        int key[R];
+        if (MODE == PT_INDEX) {
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                key[r] = ok[r] ? exact_first<AVG, LDS>(a.g, a.P, A, px[r], py[r], agg_t, vraw[r]) : -1;
+        } else {
+            int pix[R], ixs[R], iys[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                pix[r] = pixel_xy(a.cv, px[r], py[r], ixs[r], iys[r]);
+                if (!ok[r]) pix[r] = -1;
+            }
+            if (MODE == PT_SPLAT) {
+#pragma unroll
+                for (int r = 0; r < R; r++) key[r] = pix[r];
+            } else {
+                uint32_t lab[R];
+                // the tile map answers for whole tiles; mixed tiles read the pixel
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    if (pix[r] < 0) {
+                        lab[r] = kLabelNone;
+                    } else {
+                        lab[r] = a.tiles[(iys[r] >> a.tshift) * a.tiles_w + (ixs[r] >> a.tshift)];
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    if (lab[r] == kLabelMixed) lab[r] = label_at(a.label, a.label16, (int64_t)iys[r] * a.cv.LW + ixs[r]);
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    key[r] = lab[r] < kLabelMaxId ? (int)lab[r] : -1;
+                    if (MODE == PT_LABEL_HYBRID && lab[r] == kLabelBorder)
+                        key[r] = exact_first<AVG, LDS>(a.g, a.P, A, px[r], py[r], agg_t, vraw[r]);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int k = key[r];
+            if (k < 0) continue;
+            if (k != run.ck) {
+                if (run.ck >= 0) run_flush<MODE, AVG, LDS>(a, A, run.ck, run.cc, run.ch, run.cl);
+                run.ck = k;
+                run.cc = 0;
+                run.ch = run.cl = 0;
+            }
+            run.cc++;
+            if (AVG) {
+                unsigned long long h, l;
+                agg_bits(agg_t, vraw[r], h, l);
+                run.ch += h;
+                run.cl += l;
+            }
+        }
+    }
 }
 
 }  // namespace

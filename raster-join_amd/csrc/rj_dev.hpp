@@ -49,12 +49,18 @@ struct DevCanvas {
     float sx, sy;
 };
 
-// CPUJoin grid index on the device (cell = y*xs + x).
+// CPUJoin grid index on the device (cell = y*xs + x), with the row bands of
+// the polygons' edges (the edges a point of a grid row can cross).
 struct DevGrid {
     int32_t xs, ys;
     float minx, miny, width, height;
     const int32_t* cell_off;
     const int32_t* cand;
+    const int32_t* band_lo;   // per polygon: first and last row with edges
+    const int32_t* band_hi;
+    const int32_t* pband;     // per polygon: its first band
+    const int32_t* band_off;  // per band: first edge
+    const float4* edges;      // (vi.x, vi.y, vj.x, vj.y)
 };
 
 // Polygon collection on the device.
@@ -103,38 +109,18 @@ struct PointArgs {
     int tiles_w;           // tiles per canvas row (LW >> tshift)
     int tshift;
     DevResult res;
-    int lds_np;           // polygons accumulated in LDS (0: global atomics)
+    int lds_np;           // LDS sums indexed by polygon when np <= lds_np, else hashed
     uint32_t* cnt;        // canvas (PT_SPLAT)
     unsigned long long* chi;
     unsigned long long* clo;
     int cap;              // workgroups of the pass (0: 4096)
+    int pad;
 };
 
 // ---- fused passes over many queries (batch.hip) ----
 
 constexpr int kBatchMax = 64;    // queries of one fused pass (one bit each in a segment mask)
-constexpr int kBatchSlots = 4;   // distinct columns one fused pass reads
 constexpr int kBatchSegs = 2 * kBatchMax + 1;
-
-// One query of a fused pass. Filters and the aggregate name batch slots.
-struct BatchQuery {
-    int32_t mode;      // PT_LABEL, PT_LABEL_HYBRID or PT_INDEX
-    int32_t avg;       // 1: sum the aggregate column
-    int32_t agg_slot;
-    int32_t agg_type;  // column type of the aggregate
-    int32_t ncons;
-    int32_t con_slot[RJ_MAX_CONSTRAINTS];
-    int32_t con_type[RJ_MAX_CONSTRAINTS];
-    int32_t con_op[RJ_MAX_CONSTRAINTS];
-    uint32_t con_val[RJ_MAX_CONSTRAINTS];
-    int32_t label16, tshift, tiles_w, pad;
-    const void* label;
-    const uint32_t* tiles;
-    DevCanvas cv;
-    DevGrid g;
-    DevPolys P;
-    DevResult res;
-};
 
 // Rows [r0, r1) read by the queries of `mask`; split into chunks of
 // chunk_rows rows numbered from chunk0.
@@ -144,20 +130,32 @@ struct BatchSeg {
     int64_t chunk0;
 };
 
-// Everything a fused pass reads, copied to the device in one piece.
+// Everything a fused pass reads, copied to the device in one piece: the
+// segments and, per query, its point-pass arguments (end = the table's rows)
+// and mode (PT_LABEL, PT_LABEL_HYBRID or PT_INDEX).
 struct BatchBlock {
     int32_t nseg;
-    int32_t nslots;
     int32_t nq;
+    int32_t lds_cap;      // queries of at most lds_cap polygons sum in LDS
+    int32_t pad;
     int64_t nchunks;
     int64_t chunk_rows;
     unsigned long long* res;  // result words of all queries, cleared before the pass
     int64_t res_words;
-    const void* col[kBatchSlots];
-    int32_t type[kBatchSlots];
     BatchSeg seg[kBatchSegs];
-    BatchQuery q[kBatchMax];
+    int32_t mode[kBatchMax];
+    PointArgs q[kBatchMax];
 };
+
+// Per-polygon sums of a point pass in LDS: indexed by polygon when np <=
+// lds_cap, else kHashSlots hashed slots with a 4-byte tag each.
+constexpr int kHashSlots = 512;
+inline size_t acc_lds_bytes(int np, int lds_cap, bool avg)
+{
+    const bool hash = np > lds_cap;
+    const size_t slots = hash ? kHashSlots : (size_t)np;
+    return (avg ? 3 : 1) * slots * 8 + (hash ? kHashSlots * 4 : 0);
+}
 
 // LDS bytes a band of `nrows` x `wpr` words needs in each mode.
 inline size_t band_lds_bytes(int mode, int nrows, int wpr)
@@ -193,7 +191,7 @@ void launch_points(hipStream_t s, int mode, PointArgs a);
 // Fused pass: clears the results named by the block and the work counter,
 // then `nwg` persistent workgroups take chunks from the counter until none is
 // left; rows = rows per thread (4 or 8). The block is read-only on the device.
-void launch_batch(hipStream_t s, const BatchBlock* dev_block, uint32_t* counter, const float* xy, int64_t nrows,
-                  int rows, int nwg);
+// lds: dynamic LDS bytes (3 x 8 x the block's lds_cap).
+void launch_batch(hipStream_t s, const BatchBlock* dev_block, uint32_t* counter, int rows, int nwg, size_t lds);
 
 }  // namespace rj

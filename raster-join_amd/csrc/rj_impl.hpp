@@ -56,12 +56,21 @@ int64_t snap_y(const CanvasPlan& plan, float y);
 // `deficient` lists cells whose candidate list misses a polygon that meets
 // the cell (possible through the reference's cell test); the hybrid join
 // treats their pixels as border pixels so its result stays CPUJoin's.
+//
+// Row bands: for polygon p and grid row y in [band_lo[p], band_hi[p]], the
+// edges (vi, vj) whose y-span [min, max) holds the y of some point that the
+// grid maps to row y, as float4 (vi.x, vi.y, vj.x, vj.y) in
+// edges[band_off[pband[p] + y - band_lo[p]] ..). The crossing test of a point
+// in row y toggles only on those edges (every other edge fails its straddle
+// test), so testing them gives the full loop's result with far fewer edges.
 struct GridIndex {
     int32_t xs = 0, ys = 0;
     float minx = 0, miny = 0, width = 0, height = 0;
     std::vector<int32_t> cell_off;
     std::vector<int32_t> cand;
     std::vector<int32_t> deficient;
+    std::vector<int32_t> band_lo, band_hi, pband, band_off;
+    std::vector<float> edges;  // 4 floats per entry
 };
 void build_grid(const PolySet& ps, int res, GridIndex* g);
 // Pixel rectangles (i0, i1, j0, j1) covering the deficient cells, one pixel margin.

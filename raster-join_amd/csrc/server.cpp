@@ -186,34 +186,10 @@ void rj_server::run_job(std::shared_ptr<Job> J)
     }
     for (int i = 0; i < n; i++)
         if (J->st[i] != RJ_OK) complete(*J, i, J->st[i]);
-    // passes of at most max_batch queries and 4 columns
+    // passes of at most max_batch queries
     std::vector<std::vector<int>> passes;
-    {
-        std::vector<std::vector<int>> pcols;
-        for (int i : fused) {
-            std::vector<int> qc;
-            for (auto& c : J->specs[i].cons)
-                if (std::find(qc.begin(), qc.end(), c.attrib_id) == qc.end()) qc.push_back(c.attrib_id);
-            if (J->specs[i].agg_type != RJ_AGG_COUNT &&
-                std::find(qc.begin(), qc.end(), J->specs[i].agg_col) == qc.end())
-                qc.push_back(J->specs[i].agg_col);
-            bool placed = false;
-            for (size_t p = 0; p < passes.size() && !placed; p++) {
-                if ((int)passes[p].size() >= max_batch) continue;
-                std::vector<int> u = pcols[p];
-                for (int c : qc)
-                    if (std::find(u.begin(), u.end(), c) == u.end()) u.push_back(c);
-                if ((int)u.size() > kBatchSlots) continue;
-                passes[p].push_back(i);
-                pcols[p] = u;
-                placed = true;
-            }
-            if (!placed) {
-                passes.push_back({i});
-                pcols.push_back(qc);
-            }
-        }
-    }
+    for (size_t k = 0; k < fused.size(); k += (size_t)max_batch)
+        passes.emplace_back(fused.begin() + k, fused.begin() + std::min(fused.size(), k + (size_t)max_batch));
     J->passes_left = (int)passes.size() + 1;  // + 1 until the queries that run alone are done
     for (auto& p : passes) {
         const int s = next_slot;

@@ -27,7 +27,7 @@ EQ, LT, LTE, GT, GTE = 0, 1, 2, 3, 4
 COUNT, AVG, SUM = 0, 1, 2
 # options
 OPT_RASTER_PATH, OPT_TIMING, OPT_HYBRID_DILATION, OPT_CACHE, OPT_ORGANIZE = 1, 2, 3, 4, 5
-OPT_TILE_SHIFT, OPT_PASS_WORKGROUPS, OPT_BATCH_ROWS, OPT_BATCH_WORKGROUPS, OPT_BATCH_CHUNKS, OPT_GRAPH = \
+OPT_TILE_SHIFT, OPT_PASS_WORKGROUPS, OPT_BATCH_ROWS, OPT_BATCH_WORKGROUPS, OPT_BATCH_CHUNK_ROWS, OPT_GRAPH = \
     6, 7, 8, 9, 10, 11
 # how a query ran
 PATH_SINGLE, PATH_FUSED = 0, 1
@@ -510,4 +510,4 @@ class Handler:
         qa = (Query * max(1, n))(*queries)
         chosen = np.zeros(5, np.int32)
         self._check(lib().rj_autotune(self.ctx, table.h, n, C.cast(qa, C.c_void_p), _p(chosen)))
-        return dict(zip(("tile_shift", "rows", "workgroups", "chunks", "grid_res"), chosen.tolist()))
+        return dict(zip(("tile_shift", "rows", "workgroups", "chunk_rows", "grid_res"), chosen.tolist()))

@@ -175,8 +175,8 @@ def test_batch_graph_replays_match_eager(handler, data):
 
 
 def test_batch_many_passes_and_columns(handler, data):
-    """100 queries over 6 columns: a pass reads at most 4 columns and holds at
-    most 64 queries, so they run as three passes."""
+    """100 queries over 6 columns: a pass holds at most 64 queries, so they run
+    as two passes."""
     h = handler
     pts, (t, fare, pas) = data
     cols6 = (t, fare, pas, t.copy(), fare.copy(), pas.copy())
@@ -236,7 +236,7 @@ def test_autotune_keeps_results(handler, data):
     before = h.execute_batch(tb, qs, nps)
     chosen = h.autotune(tb, qs)
     assert chosen["tile_shift"] in (3, 4, 5) and chosen["rows"] in (4, 8)
-    assert chosen["workgroups"] in (1024, 2048, 4096) and chosen["chunks"] in (2, 4, 8)
+    assert chosen["workgroups"] in (1024, 2048, 4096) and chosen["chunk_rows"] in (4096, 8192, 16384)
     assert chosen["grid_res"] in (128, 256, 512, 1024)
     after = h.execute_batch(tb, qs, nps)
     for a, b in zip(before[:-1], after[:-1]):
