@@ -375,6 +375,61 @@ __device__ __forceinline__ void run_flush(const PointArgs& a, const Acc& A, int 
     }
 }
 
+// Exact tests of a wave's border rows (hybrid join), compacted: the rows of
+// all lanes that need the test are queued in LDS and the lanes take them one
+// each, so a wave runs about one round of the test instead of one per row of
+// its lanes. Only the wave itself touches its queue (no workgroup barrier).
+constexpr int kExactQueue = 128;  // entries per wave
+
+template <bool AVG, bool LDS, int R>
+__device__ __forceinline__ void exact_compact(const PointArgs& a, const Acc& A, int agg_t, const float (&px)[R],
+                                              const float (&py)[R], const uint32_t (&vraw)[R], bool (&need)[R],
+                                              int (&key)[R])
+{
+    __shared__ float4 s_q[kBlock / kWave][kExactQueue];
+    __shared__ int s_res[kBlock / kWave][kExactQueue];
+    const int w = threadIdx.x / kWave;
+    const int lane = (int)__lane_id();
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // lanes past the end of the rows may have left the loop: the active ones
+    // share the queue by rank
+    const unsigned long long act = __ballot(1);
+    const int rank = __popcll(act & below);
+    const int nact = __popcll(act);
+    for (;;) {
+        int cnt = 0;
+        int pos[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const unsigned long long m = __ballot(need[r]);
+            const int p = cnt + __popcll(m & below);
+            pos[r] = -1;
+            if (need[r] && p < kExactQueue) {
+                s_q[w][p] = make_float4(px[r], py[r], __uint_as_float(vraw[r]), 0.0f);
+                pos[r] = p;
+                need[r] = false;
+            }
+            cnt += __popcll(m);
+        }
+        cnt = min(cnt, kExactQueue);
+        if (cnt == 0) break;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int i = rank; i < cnt; i += nact) {
+            const float4 e = s_q[w][i];
+            s_res[w][i] = exact_first<AVG, LDS>(a.g, a.P, A, e.x, e.y, agg_t, __float_as_uint(e.z));
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            if (pos[r] >= 0) key[r] = s_res[w][pos[r]];
+        __builtin_amdgcn_wave_barrier();  // the queue is rewritten by the next round
+    }
+}
+
 // Rows [lo, hi) of one point pass, read in groups of R consecutive rows per
 // thread from `start` (a multiple of R, <= lo). Every row gets one key
 // (polygon id, or pixel for PT_SPLAT) and the lane keeps a running
@@ -460,10 +515,12 @@ __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, in
                 for (int r = 0; r < R; r++)
                     if (lab[r] == kLabelMixed) lab[r] = label_at(a.label, a.label16, (int64_t)iys[r] * a.cv.LW + ixs[r]);
 #pragma unroll
-                for (int r = 0; r < R; r++) {
-                    key[r] = lab[r] < kLabelMaxId ? (int)lab[r] : -1;
-                    if (MODE == PT_LABEL_HYBRID && lab[r] == kLabelBorder)
-                        key[r] = exact_first<AVG, LDS>(a.g, a.P, A, px[r], py[r], agg_t, vraw[r]);
+                for (int r = 0; r < R; r++) key[r] = lab[r] < kLabelMaxId ? (int)lab[r] : -1;
+                if (MODE == PT_LABEL_HYBRID) {
+                    bool need[R];
+#pragma unroll
+                    for (int r = 0; r < R; r++) need[r] = lab[r] == kLabelBorder;
+                    exact_compact<AVG, LDS, R>(a, A, agg_t, px, py, vraw, need, key);
                 }
             }
         }
