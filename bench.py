@@ -256,6 +256,14 @@ def batch_section(h, table, lib, args):
     for _ in range(reps):
         single = [h.execute_query(table, q, k) for q, k in zip(qs, nps)]
     seq_s = (time.perf_counter() - t0) / reps
+    # plain: every query alone with its polygon structures rebuilt, as the
+    # reference executes each query
+    h.set_option(rj.OPT_CACHE, 0)
+    t0 = time.perf_counter()
+    plain = [h.execute_query(table, q, k) for q, k in zip(qs, nps)]
+    plain_s = time.perf_counter() - t0
+    h.set_option(rj.OPT_CACHE, 1)
+    single = [h.execute_query(table, q, k) for q, k in zip(qs, nps)]  # warm again
     h.set_option(rj.OPT_TIMING, 1)
     h.execute_batch(table, qs, nps)
     kt = h.kernel_times()
@@ -264,7 +272,7 @@ def batch_section(h, table, lib, args):
     for _ in range(reps):
         out = h.execute_batch(table, qs, nps)
     bat_s = (time.perf_counter() - t0) / reps
-    ok = all(o.status == 0 and same(a, o) for a, o in zip(single, out))
+    ok = all(o.status == 0 and same(a, o) and same(p, o) for a, p, o in zip(single, plain, out))
     fused = sum(o.path == rj.PATH_FUSED for o in out)
     # rows each query reads and the union the fused pass reads
     rngs = []
@@ -280,7 +288,9 @@ def batch_section(h, table, lib, args):
            "queries": len(qs), "fused": fused, "identical_results": ok,
            "batch_ms": round(bat_s * 1e3, 3), "one_at_a_time_ms": round(seq_s * 1e3, 3),
            "batch_qps": round(len(qs) / bat_s, 1), "one_at_a_time_qps": round(len(qs) / seq_s, 1),
-           "speedup": round(seq_s / bat_s, 2), "rows_read_one_at_a_time": int(total),
+           "one_at_a_time_plain_ms": round(plain_s * 1e3, 3),
+           "speedup": round(seq_s / bat_s, 2), "speedup_vs_plain": round(plain_s / bat_s, 2),
+           "rows_read_one_at_a_time": int(total),
            "rows_read_fused": int(union), "autotuned": chosen, "autotune_s": round(tune_s, 2)}
     if bms > 0:
         ach = union * ROW_BYTES / (bms * 1e-3) / 1e9
@@ -335,10 +345,20 @@ def serving_section(h, table, lib, args):
         log(f"serving at {rate} qps: {rates[-1]}")
     outs, st, wall = serve(h, table, qs, nps, 0, seed=1)
     ok = all(o is not None and same(a, o) for a, o in zip(alone, outs))
+    # plain one-at-a-time on a sample: polygon structures rebuilt per query
+    npl = min(nq, args.plain_queries)
+    h.set_option(rj.OPT_CACHE, 0)
+    t0 = time.perf_counter()
+    plain = [h.execute_query(table, q, k) for q, k in zip(qs[:npl], nps[:npl])]
+    plain_s = time.perf_counter() - t0
+    h.set_option(rj.OPT_CACHE, 1)
+    ok = ok and all(same(a, p) for a, p in zip(alone, plain))
+    plain_qps = npl / plain_s
     sat = {"queries": nq, "served_qps": round(nq / wall, 1), "p50_ms": round(st["p50_ms"], 3),
            "p99_ms": round(st["p99_ms"], 3), "mean_batch": round(st["mean_batch"], 2),
            "one_at_a_time_qps": round(nq / one_s, 1), "speedup": round(one_s / wall, 2),
-           "identical_results": ok}
+           "one_at_a_time_plain_qps": round(plain_qps, 1), "plain_sample": npl,
+           "speedup_vs_plain": round(nq / wall / plain_qps, 2), "identical_results": ok}
     log(f"saturated serving: {sat}")
     return {"workload": "Poisson arrivals, 60/30/10 raster/hybrid/index, 10-5000 polygons, windows "
                         "1 h - 1 yr, 30% with a passengers filter, resident 1B-row table",
@@ -356,6 +376,8 @@ def main():
     ap.add_argument("--path", type=int, default=0, help="0 auto, 1 canvas, 2 label map")
     ap.add_argument("--serve-queries", type=int, default=400)
     ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--plain-queries", type=int, default=100,
+                    help="queries of the plain one-at-a-time serving baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--headline-only", action="store_true")
     args = ap.parse_args()
@@ -412,10 +434,14 @@ def main():
         log("batch:", out["batch"])
         srv = serving_section(h, table, lib, args)
         out["serving"] = srv
-        top = max(srv["rates"], key=lambda r: r["offered_qps"])
+        # the highest offered rate the loop keeps up with (served >= 90% of offered)
+        kept = [r for r in srv["rates"] if r["served_qps"] >= 0.9 * r["offered_qps"]]
+        top = max(kept or srv["rates"][:1], key=lambda r: r["offered_qps"])
         out["served_qps"] = top["served_qps"]
+        out["offered_qps"] = top["offered_qps"]
         out["p50_ms"] = top["p50_ms"]
         out["p99_ms"] = top["p99_ms"]
+        out["saturated_qps"] = srv["saturated"]["served_qps"]
         for ps in lib.values():
             ps.close()
         table.close()

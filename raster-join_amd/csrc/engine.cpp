@@ -39,7 +39,7 @@ int lds_direct_cap(bool avg) { return avg ? 16384 / 24 : 16384 / 8; }
 // exact test.
 double canvas_eps(const QuerySpec& s)
 {
-    if (s.fn != RJ_HYBRID) return s.eps;
+    if (s.exec != RJ_HYBRID) return s.eps;
     const double* b = s.P->ps.bbox;
     const double ext = std::max(b[2] - b[0], b[3] - b[1]);
     return ext / 8191.999 * std::sqrt(2.0) * s.ground_res;
@@ -510,14 +510,38 @@ int rj_ctx::hybrid_radius(const CanvasPlan& plan) const
     return std::max(1, (int)std::ceil(2.0 * err / pw));
 }
 
-// Hybrid label map: polygon labels, with every pixel within the dilation
-// radius of an edge (and every pixel of a grid cell with an incomplete
-// candidate list) marked for the exact test (HybridJoin::drawOutline + the
-// 3x3 dilation of hybrid.glsl:174-185).
+// Width in pixels of the zone around a touched pixel where a point may be
+// misclassified by its pixel center: CPUJoin's float PIP error
+// (16 |coord| 2^-24), the snapping of vertices to 1/256 pixel and the pixel
+// formula's float error (each below 0.003 pixel), doubled.
+double rj_ctx::hybrid_delta(const CanvasPlan& plan) const
+{
+    const double m = std::max(std::max(std::fabs(plan.minx), std::fabs(plan.maxx)),
+                              std::max(std::fabs(plan.miny), std::fabs(plan.maxy)));
+    const double err = 16.0 * m * std::ldexp(1.0, -24);
+    const double pw = std::min((plan.maxx - plan.minx) / plan.W, (plan.maxy - plan.miny) / plan.H);
+    return 2.0 * (err / pw + 0.006);
+}
+
+void rj_ctx::hybrid_args(const Canvas* C, PointArgs* pa) const
+{
+    pa->halo = C->hyb_halo ? C->d_mask.as<uint32_t>() : nullptr;
+    pa->halo_wpr = (C->plan.W + 31) / 32;
+    pa->halo_d = C->hyb_delta;
+}
+
+// Hybrid label map: polygon labels, with the pixels a snapped edge touches
+// (and every pixel of a grid cell with an incomplete candidate list) marked
+// for the exact test (HybridJoin::drawOutline). Points of untouched pixels
+// near a touched one are sent to the exact test by the point pass (halo_near)
+// when the float margin is a small part of a pixel; otherwise the border is
+// dilated by whole pixels, as the 3x3 dilation of hybrid.glsl:174-185.
 int rj_ctx::hybrid_labels(rj_polyset* P, Canvas* C, Grid* G)
 {
     if (C->hyb_res == G->g.xs && cache && C->hyb_tshift == opt_tshift) return RJ_OK;
-    const int r = hybrid_radius(C->plan);
+    const double delta = hybrid_delta(C->plan);
+    const bool halo = hybrid_dilation == 0 && delta <= 0.25;
+    const int r = halo ? 0 : hybrid_radius(C->plan);
     if (r > 31) return fail(RJ_ERR_UNSUPPORTED, "accuracy too fine for the float margin of the exact test");
     const DevCanvas cv = dev_canvas(C->plan);
     const size_t npx = (size_t)cv.LW * cv.H;
@@ -565,6 +589,8 @@ int rj_ctx::hybrid_labels(rj_polyset* P, Canvas* C, Grid* G)
     C->hyb16 = l16;
     C->hyb_res = G->g.xs;
     C->hyb_tshift = opt_tshift;
+    C->hyb_halo = halo;
+    C->hyb_delta = (float)delta;
     return RJ_OK;
 }
 
@@ -574,13 +600,13 @@ int64_t rj_ctx::estimate_bytes(const QuerySpec& s) const
 {
     const int np = s.P->ps.np;
     int64_t b = (int64_t)np * 7 * 8 + 64;
-    if (s.fn == RJ_INDEX || s.fn == RJ_HYBRID) {
+    if (s.exec == RJ_INDEX || s.exec == RJ_HYBRID) {
         if (!s.P->grids.count(s.index_res)) {
             const int64_t cells = (int64_t)s.index_res * s.index_res;
             b += cells * 4 + (int64_t)np * 64 * 4;
         }
     }
-    if (s.fn != RJ_INDEX) {
+    if (s.exec != RJ_INDEX) {
         CanvasPlan plan;
         if (make_plan(s.P->ps.bbox, canvas_eps(s), s.ground_res, &plan)) return b;
         auto it = s.P->canvases.find(plan.cell);
@@ -589,9 +615,9 @@ int64_t rj_ctx::estimate_bytes(const QuerySpec& s) const
         const int64_t npx = LW * plan.H;
         const int64_t lw = np < 0xFFF0 ? 2 : 4;
         if (!C) b += (int64_t)s.P->ps.off[np] * 8 + (int64_t)np * 1024;
-        if (s.fn == RJ_RASTER && (!C || !C->lab_built)) b += npx * lw + npx / 64;
-        if (s.fn == RJ_HYBRID && (!C || C->hyb_res != s.index_res)) b += npx * lw + npx / 64 + npx / 8;
-        if (s.fn == RJ_ERRORBOUNDS) b += (int64_t)plan.W * plan.H * 4;
+        if (s.exec == RJ_RASTER && (!C || !C->lab_built)) b += npx * lw + npx / 64;
+        if (s.exec == RJ_HYBRID && (!C || C->hyb_res != s.index_res)) b += npx * lw + npx / 64 + npx / 8;
+        if (s.exec == RJ_ERRORBOUNDS) b += (int64_t)plan.W * plan.H * 4;
     }
     return b;
 }
@@ -736,7 +762,7 @@ int rj_ctx::ensure_canvas_buffers(const CanvasPlan& plan, bool sums)
 // function type (RasterJoin, IndexJoin, HybridJoin, RasterJoinBounds).
 int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
 {
-    const int fn = s.fn;
+    const int fn = s.exec;
     if (fn < RJ_RASTER || fn > RJ_ERRORBOUNDS) return fail(RJ_ERR_UNSUPPORTED, "unknown function type");
     if (!s.P) return fail(RJ_ERR_STATE, "no polygons set");
     if (!t) return fail(RJ_ERR_STATE, "no points set");
@@ -818,6 +844,7 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
         pa.tshift = C->hyb_tshift;
         pa.tiles = C->d_htiles.as<uint32_t>();
         pa.tiles_w = pa.cv.LW >> pa.tshift;
+        hybrid_args(C, &pa);
         if ((rc = launch("points_hybrid", [&] { launch_points(stream, PT_LABEL_HYBRID, pa); }))) return rc;
     } else if (fn == RJ_RASTER) {
         if ((rc = canvas_for(P, canvas_eps(s), s.ground_res, &C))) return rc;
@@ -900,6 +927,7 @@ int rj_ctx::execute(int fn, int64_t* out_counts, double* out_sums, int32_t* lega
     QuerySpec s;
     s.P = polys;
     s.fn = fn;
+    set_exec(s);
     s.agg_type = agg_type;
     s.agg_col = agg_col;
     s.cons = cons;
@@ -968,6 +996,7 @@ int spec_check(const rj_ctx* ctx, const rj_table* t, const rj_query* q, QuerySpe
     if (q->ground_res < 0 || q->index_res < 0 || q->index_res > 16384) return bad(RJ_ERR_ARG, "bad query field");
     s->P = q->polys;
     s->fn = q->fn;
+    ctx->set_exec(*s);
     s->agg_type = q->agg_type;
     s->agg_col = q->agg_col;
     s->cons.assign(q->cons, q->cons + q->ncons);
@@ -1006,15 +1035,15 @@ int rj_ctx::batch_prepare(rj_table* t, const QuerySpec* qs, int nq, std::vector<
             (*status)[i] = rc;
             continue;
         }
-        if (s.fn == RJ_ERRORBOUNDS || (s.fn == RJ_RASTER && raster_path == 1)) {
+        if (s.exec == RJ_ERRORBOUNDS || (s.exec == RJ_RASTER && raster_path == 1)) {
             alone->push_back(i);
             continue;
         }
         Canvas* C = nullptr;
         Grid* G = nullptr;
-        if (s.fn == RJ_INDEX) {
+        if (s.exec == RJ_INDEX) {
             rc = grid_for(s.P, s.index_res, &G);
-        } else if (s.fn == RJ_HYBRID) {
+        } else if (s.exec == RJ_HYBRID) {
             rc = canvas_for(s.P, canvas_eps(s), s.ground_res, &C);
             if (!rc) rc = grid_for(s.P, s.index_res, &G);
             if (!rc) rc = hybrid_labels(s.P, C, G);
@@ -1063,7 +1092,7 @@ int rj_ctx::batch_prepare(rj_table* t, const QuerySpec* qs, int nq, std::vector<
         *fused = keep;
         if (raster_path == 2)
             for (int i : *alone)
-                if (qs[i].fn == RJ_RASTER) (*status)[i] = RJ_ERR_UNSUPPORTED;
+                if (qs[i].exec == RJ_RASTER) (*status)[i] = RJ_ERR_UNSUPPORTED;
     }
     return RJ_OK;
 }
@@ -1114,13 +1143,13 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
         a.cols = dev_cols(t);
         a.P = dev_polys(P, nullptr);
         Grid* G = nullptr;
-        if (s.fn != RJ_RASTER) {
+        if (s.exec != RJ_RASTER) {
             auto gi = P->grids.find(s.index_res);
             if (gi == P->grids.end()) return fail(RJ_ERR_STATE, "grid of a fused query is not resident");
             G = gi->second.get();
             a.g = dev_grid(G);
         }
-        if (s.fn == RJ_INDEX) {
+        if (s.exec == RJ_INDEX) {
             B.mode[j] = PT_INDEX;
         } else {
             CanvasPlan plan;
@@ -1130,12 +1159,13 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
             Canvas* C = ci->second.get();
             a.cv = dev_canvas(C->plan);
             a.P = dev_polys(P, C);
-            if (s.fn == RJ_HYBRID) {
+            if (s.exec == RJ_HYBRID) {
                 B.mode[j] = PT_LABEL_HYBRID;
                 a.label = C->d_hlab.p;
                 a.label16 = C->hyb16;
                 a.tshift = C->hyb_tshift;
                 a.tiles = C->d_htiles.as<uint32_t>();
+                hybrid_args(C, &a);
             } else {
                 B.mode[j] = PT_LABEL;
                 a.label = C->d_lab.p;

@@ -58,6 +58,8 @@ struct Canvas {
     int hyb_res = -1;
     int hyb16 = 0;
     int hyb_tshift = kTileShiftDefault;
+    bool hyb_halo = false;  // border = touched pixels + the halo test (else dilated)
+    float hyb_delta = 0;    // halo width in pixels
     DevBuf d_hlab, d_htiles, d_mask, d_rects;
     uint64_t last_use = 0;
     size_t device_bytes() const;
@@ -74,6 +76,8 @@ struct Grid {
 struct QuerySpec {
     rj_polyset* P = nullptr;
     int fn = RJ_RASTER;
+    int exec = RJ_RASTER;  // the pass that runs it: fn, or RJ_HYBRID for an index join
+                           // answered by the label-accelerated exact classification
     int agg_type = RJ_AGG_COUNT, agg_col = -1;
     std::vector<rj_constraint> cons;
     double eps = 100.0, ground_res = 1.0;
@@ -162,6 +166,8 @@ struct rj_ctx {
     int opt_batch_chunk_rows = 8192;       // rows per chunk of a fused pass
     bool opt_graph = true;                 // fused passes replay a captured HIP graph
     int opt_auto_res = 256;                // grid resolution of queries that leave it open
+    int opt_index_path = 0;                // 0: index joins by the hybrid pass, 1: per-point grid PIP
+    void set_exec(rj::QuerySpec& s) const { s.exec = (s.fn == RJ_INDEX && opt_index_path == 0) ? RJ_HYBRID : s.fn; }
 
     // canvas (canvas path, errorbounds)
     rj::DevBuf d_cnt, d_chi, d_clo;
@@ -256,6 +262,8 @@ struct rj_ctx {
     int build_labels(rj_polyset* P, rj::Canvas* C, void* lab, int label16, rj::DevBuf& tiles, int tshift,
                      bool count);
     int hybrid_radius(const rj::CanvasPlan& plan) const;
+    double hybrid_delta(const rj::CanvasPlan& plan) const;
+    void hybrid_args(const rj::Canvas* C, rj::PointArgs* pa) const;
     int make_room(int64_t bytes);
     void forget_polyset(rj_polyset* P);
 

@@ -375,6 +375,29 @@ __device__ __forceinline__ void run_flush(const PointArgs& a, const Acc& A, int 
     }
 }
 
+// Hybrid join: does the point (in untouched pixel ix, iy) lie within halo_d
+// pixels of a side or corner it shares with a touched pixel? Edges lie only
+// in touched pixels, so a point farther than that from every touched
+// neighbour is farther than the float and snapping error from every edge and
+// takes its pixel center's label. The fraction uses the pixel formula's own
+// float operations.
+__device__ __forceinline__ bool halo_near(const PointArgs& a, float x, float y, int ix, int iy)
+{
+    const float fx = (x - a.cv.minx) * a.cv.sx - (float)ix;
+    const float fy = (y - a.cv.miny) * a.cv.sy - (float)iy;
+    const float d = a.halo_d;
+    const int dx = fx < d ? -1 : (fx > 1.0f - d ? 1 : 0);
+    const int dy = fy < d ? -1 : (fy > 1.0f - d ? 1 : 0);
+    if (!dx && !dy) return false;
+    auto touched = [&](int cx, int cy) -> bool {
+        if (cx < 0 || cy < 0 || cx >= a.cv.W || cy >= a.cv.H) return false;
+        return (a.halo[(int64_t)cy * a.halo_wpr + (cx >> 5)] >> (cx & 31)) & 1u;
+    };
+    if (dx && touched(ix + dx, iy)) return true;
+    if (dy && touched(ix, iy + dy)) return true;
+    return dx && dy && touched(ix + dx, iy + dy);
+}
+
 // Exact tests of a wave's border rows (hybrid join), compacted: the rows of
 // all lanes that need the test are queued in LDS and the lanes take them one
 // each, so a wave runs about one round of the test instead of one per row of
@@ -514,6 +537,12 @@ __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, in
 #pragma unroll
                 for (int r = 0; r < R; r++)
                     if (lab[r] == kLabelMixed) lab[r] = label_at(a.label, a.label16, (int64_t)iys[r] * a.cv.LW + ixs[r]);
+                if (MODE == PT_LABEL_HYBRID && a.halo) {
+#pragma unroll
+                    for (int r = 0; r < R; r++)
+                        if (pix[r] >= 0 && lab[r] != kLabelBorder && halo_near(a, px[r], py[r], ixs[r], iys[r]))
+                            lab[r] = kLabelBorder;
+                }
 #pragma unroll
                 for (int r = 0; r < R; r++) key[r] = lab[r] < kLabelMaxId ? (int)lab[r] : -1;
                 if (MODE == PT_LABEL_HYBRID) {

@@ -108,6 +108,13 @@ struct PointArgs {
     const uint32_t* tiles; // per tile of 2^tshift pixels square: its single label, or kLabelMixed
     int tiles_w;           // tiles per canvas row (LW >> tshift)
     int tshift;
+    // hybrid: bitmap of the pixels a snapped edge touches (W bits per row in
+    // halo_wpr words); null when the border is dilated instead. A point of an
+    // untouched pixel within halo_d pixels of a side it shares with a touched
+    // pixel takes the exact test.
+    const uint32_t* halo;
+    int halo_wpr;
+    float halo_d;
     DevResult res;
     int lds_np;           // LDS sums indexed by polygon when np <= lds_np, else hashed
     uint32_t* cnt;        // canvas (PT_SPLAT)

@@ -21,6 +21,14 @@ def oracle_exact(xy, off, pts, cols=(), cons=(), agg=rj.COUNT, agg_col=0, res=25
     return O.join_exact(g, O.PointSet(pts, cols, types, cons, agg_type=agg, agg_col=agg_col))
 
 
+@pytest.fixture(params=[0, 1], ids=["index-via-labels", "index-grid-pip"])
+def ih(request, handler):
+    """The handler with each index-join path: the label-accelerated exact
+    classification (default) and the per-point grid PIP of IndexJoin."""
+    handler.set_option(rj.OPT_INDEX_PATH, request.param)
+    return handler
+
+
 def run(h, fn, xy, off, pts, cols=(), cons=(), agg=rj.COUNT, agg_col=0, eps=None, res=None):
     h.set_polygons(xy, off)
     h.set_points(pts, cols, COLS_TYPES[:len(cols)])
@@ -39,57 +47,57 @@ def run(h, fn, xy, off, pts, cols=(), cons=(), agg=rj.COUNT, agg_col=0, eps=None
 # index join (K4 grid + K5 PIP), CPUJoin semantics
 # ---------------------------------------------------------------------------
 
-def test_index_toy(handler):
+def test_index_toy(ih):
     d = np.load(os.path.join(GOLD, "toy_squares.npz"))
-    c, _, _, _ = run(handler, rj.INDEX, d["xy"], d["off"], d["pts"], res=4)
+    c, _, _, _ = run(ih, rj.INDEX, d["xy"], d["off"], d["pts"], res=4)
     assert c.tolist() == [2, 3]
 
 
 @pytest.mark.parametrize("name", ["squares_edges", "voronoi20"])
-def test_index_golden_reference_fixture(handler, name):
+def test_index_golden_reference_fixture(ih, name):
     d = np.load(os.path.join(GOLD, name + ".npz"))
-    c, _, _, _ = run(handler, rj.INDEX, d["xy"], d["off"], d["pts"], res=int(d["res"]))
+    c, _, _, _ = run(ih, rj.INDEX, d["xy"], d["off"], d["pts"], res=int(d["res"]))
     assert np.array_equal(c, d["counts"])
 
 
-def test_index_config1_matches_reference_counts(handler):
+def test_index_config1_matches_reference_counts(ih):
     """1M points, 260 polygons, grid 256 (the CPUJoin-comparable workload)."""
     d = np.load(os.path.join(GOLD, "config1.npz"))
     xy, off = polys("voronoi260")
     pts = points(1_000_000, 7)[0]
-    c, _, _, _ = run(handler, rj.INDEX, xy, off, pts, res=256)
+    c, _, _, _ = run(ih, rj.INDEX, xy, off, pts, res=256)
     assert np.array_equal(c, d["counts"])
 
 
-def test_index_filters_and_avg_exact(handler):
+def test_index_filters_and_avg_exact(ih):
     xy, off = polys("voronoi260")
     pts, t, fare, pas = points(600_000, 8)
     cons = WINDOW_30 + PASS_LE2
-    c, s, hi, lo = run(handler, rj.INDEX, xy, off, pts, (t, fare, pas), cons, rj.AVG, 1, res=256)
+    c, s, hi, lo = run(ih, rj.INDEX, xy, off, pts, (t, fare, pas), cons, rj.AVG, 1, res=256)
     oc, ohi, olo, of64 = oracle_exact(xy, off, pts, (t, fare, pas), cons, rj.AVG, 1, res=256)
     assert np.array_equal(c, oc)
     assert np.array_equal(hi, ohi) and np.array_equal(lo, olo)
     assert np.allclose(s, of64, rtol=1e-9, atol=0)
 
 
-def test_index_overlapping_polygons(handler):
+def test_index_overlapping_polygons(ih):
     xy, off = polys("stars")
     pts = points(400_000, 9)[0]
-    c, _, _, _ = run(handler, rj.INDEX, xy, off, pts, res=64)
+    c, _, _, _ = run(ih, rj.INDEX, xy, off, pts, res=64)
     oc, _, _, _ = oracle_exact(xy, off, pts, res=64)
     assert np.array_equal(c, oc)
     assert c.sum() > 400_000 // 4
 
 
-def test_index_edge_cases(handler):
+def test_index_edge_cases(ih):
     xy, off = polys("voronoi20")
     # empty input
-    c, _, _, _ = run(handler, rj.INDEX, xy, off, np.zeros(0, np.float32), res=32)
+    c, _, _, _ = run(ih, rj.INDEX, xy, off, np.zeros(0, np.float32), res=32)
     assert c.sum() == 0
     # outside the bbox, NaN, infinities, max edges
     pts = np.array([-1, 5, 5, -1, 50000, 10, 10, 50000, np.nan, 3, 3, np.nan, np.inf, 4,
                     -np.inf, 4, 0, 0, 49999.996, 49999.996], np.float32)
-    c, _, _, _ = run(handler, rj.INDEX, xy, off, pts, res=32)
+    c, _, _, _ = run(ih, rj.INDEX, xy, off, pts, res=32)
     oc, _, _, _ = oracle_exact(xy, off, pts, res=32)
     assert np.array_equal(c, oc)
 
@@ -172,6 +180,34 @@ def test_hybrid_equals_cpujoin(handler, name, eps, res):
     assert np.array_equal(c, oc)
     assert np.array_equal(hi, ohi) and np.array_equal(lo, olo)
     assert np.allclose(s, of64, rtol=1e-9, atol=0)
+
+
+@pytest.mark.parametrize("name", ["voronoi260", "tracts400", "stars"])
+def test_hybrid_and_index_points_near_edges(ih, name):
+    """Points on edges, on vertices and at 1e-4 m .. 7 m from edges (the
+    float-error zone the hybrid border must send to the exact test). Every
+    point carries its own id as the aggregate, so the per-polygon id sums
+    catch any single misassigned point."""
+    xy, off = polys(name)
+    rng = np.random.default_rng(21)
+    v = xy.reshape(-1, 2).astype(np.float64)
+    prev = np.arange(len(v)) - 1
+    prev[off[:-1]] = off[1:] - 1
+    e = rng.integers(0, len(v), 200_000)
+    a, b = v[prev[e]], v[e]
+    mid = a + rng.uniform(0, 1, len(e))[:, None] * (b - a)
+    nrm = np.stack([a[:, 1] - b[:, 1], b[:, 0] - a[:, 0]], 1)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True) + 1e-30
+    d = np.array([0, 1e-4, 1e-3, 1e-2, 0.05, 0.2, 1, 4, 7])
+    offs = rng.choice(np.concatenate([d, -d]), len(e))
+    p = np.concatenate([mid + nrm * offs[:, None], v[rng.integers(0, len(v), 20_000)]])
+    pts = p.astype(np.float32).ravel()
+    ids = np.arange(len(p), dtype=np.uint32)
+    oc, ohi, _, _ = oracle_exact(xy, off, pts, (ids,), (), rj.AVG, 0, res=256)
+    for fn in (rj.HYBRID, rj.INDEX):
+        c, _, hi, _ = run(ih, fn, xy, off, pts, (ids,), (), rj.AVG, 0, eps=10.0, res=256)
+        assert np.array_equal(c, oc), fn
+        assert np.array_equal(hi, ohi), fn
 
 
 def test_hybrid_count_config1_reference(handler):
