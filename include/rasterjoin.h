@@ -245,11 +245,13 @@ int rj_canvas_info(rj_ctx* ctx, int32_t* W, int32_t* H);
  *   RJ_OPT_INDEX_PATH:      0 (default) index joins run the hybrid pass (label map +
  *                           exact test of the points near an edge: the same CPUJoin
  *                           result); 1 the exact test of every point over its grid
- *                           cell's candidates, as IndexJoin (index.glsl:186-249) */
+ *                           cell's candidates, as IndexJoin (index.glsl:186-249)
+ *   RJ_OPT_EXACT_QUEUE:     1 (default) single-query hybrid passes queue their border
+ *                           points for a separate exact pass; 0 tests them inline */
 enum { RJ_OPT_RASTER_PATH = 1, RJ_OPT_TIMING = 2, RJ_OPT_HYBRID_DILATION = 3, RJ_OPT_CACHE = 4,
        RJ_OPT_ORGANIZE = 5, RJ_OPT_TILE_SHIFT = 6, RJ_OPT_PASS_WORKGROUPS = 7, RJ_OPT_BATCH_ROWS = 8,
        RJ_OPT_BATCH_WORKGROUPS = 9, RJ_OPT_BATCH_CHUNK_ROWS = 10, RJ_OPT_GRAPH = 11,
-       RJ_OPT_INDEX_PATH = 12 };
+       RJ_OPT_INDEX_PATH = 12, RJ_OPT_EXACT_QUEUE = 13 };
 int rj_set_option(rj_ctx* ctx, int32_t key, int64_t value);
 
 /* Per-kernel device time of the last execute (timing on): name i, total ms and

@@ -444,6 +444,7 @@ int rj_set_option(rj_ctx* ctx, int32_t key, int64_t value)
         ctx->opt_batch_chunk_rows = (int)value;
         return RJ_OK;
     case RJ_OPT_GRAPH: ctx->opt_graph = value != 0; return RJ_OK;
+    case RJ_OPT_EXACT_QUEUE: ctx->opt_queue = value != 0; return RJ_OK;
     case RJ_OPT_INDEX_PATH:
         if (value < 0 || value > 1) return ctx->fail(RJ_ERR_ARG, "index path is 0 or 1");
         ctx->opt_index_path = (int)value;

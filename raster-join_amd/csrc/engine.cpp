@@ -803,6 +803,7 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
     Canvas* C = nullptr;
     Grid* G = nullptr;
     const unsigned long long* check_aux = nullptr;  // overlap counters of a fresh label map
+    const uint32_t* check_queue = nullptr;          // border-queue count of a hybrid pass
     out->path = RJ_PATH_SINGLE;
 
     auto canvas_pass = [&](bool sums) -> int {
@@ -845,7 +846,22 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
         pa.tiles = C->d_htiles.as<uint32_t>();
         pa.tiles_w = pa.cv.LW >> pa.tshift;
         hybrid_args(C, &pa);
-        if ((rc = launch("points_hybrid", [&] { launch_points(stream, PT_LABEL_HYBRID, pa); }))) return rc;
+        // border points queued for a separate exact pass; a queue that runs
+        // out of room is seen at the readback and the pass is rerun inline
+        const int64_t rows = std::max<int64_t>(0, re0 - rb0);
+        const uint32_t qcap = (uint32_t)std::min<int64_t>(1LL << 27, std::max<int64_t>(65536, rows / 8));
+        if (opt_queue && alloc(d_queue, (size_t)qcap * 16 + 256, "border queue") == RJ_OK) {
+            pa.queue = d_queue.as<float4>();
+            pa.qcount = reinterpret_cast<uint32_t*>(d_queue.as<char>() + (size_t)qcap * 16);
+            pa.qcap = qcap;
+            uint32_t* qc = pa.qcount;
+            if ((rc = launch("fill_queue", [&] { launch_fill_u32(stream, qc, 1, 0); }))) return rc;
+            if ((rc = launch("points_hybrid", [&] { launch_points(stream, PT_HYBRID_QUEUE, pa); }))) return rc;
+            if ((rc = launch("exact_queue", [&] { launch_exact(stream, pa); }))) return rc;
+            check_queue = pa.qcount;
+        } else {
+            if ((rc = launch("points_hybrid", [&] { launch_points(stream, PT_LABEL_HYBRID, pa); }))) return rc;
+        }
     } else if (fn == RJ_RASTER) {
         if ((rc = canvas_for(P, canvas_eps(s), s.ground_res, &C))) return rc;
         const bool known_overlap = C->lab_built && cache && C->lab_state == 2;
@@ -882,14 +898,28 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
             return rc;
     }
 
-    // one pinned copy of [results | bounds] (+ the overlap counters)
-    const size_t res_bytes = (res_words + 2) * 8;
+    // one pinned copy of [results | bounds] (+ the overlap counters, the
+    // border-queue count)
+    const size_t res_bytes = (res_words + 3) * 8;
     if ((rc = pinned((void**)&h_res, &h_res_bytes, res_bytes, "pinned results"))) return rc;
     hipError_t e = hipMemcpyAsync(h_res, res_base, res_words * 8, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess && check_aux)
         e = hipMemcpyAsync(h_res + res_words, check_aux, 16, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess && check_queue)
+        e = hipMemcpyAsync(h_res + res_words + 2, check_queue, 4, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "execute");
+    if (check_queue && (uint32_t)h_res[res_words + 2] > pa.qcap) {
+        // the queue overflowed: the whole pass again with the exact test inline
+        PointArgs ia = pa;
+        ia.queue = nullptr;
+        ia.qcount = nullptr;
+        if ((rc = clear_results())) return rc;
+        if ((rc = launch("points_hybrid", [&] { launch_points(stream, PT_LABEL_HYBRID, ia); }))) return rc;
+        e = hipMemcpyAsync(h_res, res_base, res_words * 8, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return hip_fail(e, "execute");
+    }
     if (check_aux) {
         C->lab_state = (h_res[res_words] != h_res[res_words + 1]) ? 2 : 1;
         if (C->lab_state == 2) {

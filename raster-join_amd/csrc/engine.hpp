@@ -173,6 +173,8 @@ struct rj_ctx {
     rj::DevBuf d_cnt, d_chi, d_clo;
     // single-query results
     rj::DevBuf d_res;
+    rj::DevBuf d_queue;  // hybrid border queue (float4 entries, then its counter)
+    bool opt_queue = true;
     unsigned long long* h_res = nullptr;  // pinned
     size_t h_res_bytes = 0;
     void* h_stage = nullptr;  // pinned staging of polygon uploads

@@ -412,6 +412,30 @@ __global__ __launch_bounds__(kBlock) void k_points(PointArgs a)
     if (MODE != PT_SPLAT) acc_flush(A, AVG);
 }
 
+// The queued border points of a PT_HYBRID_QUEUE pass: the exact test of
+// each (one per lane, no idle lanes), summed per workgroup in LDS.
+template <bool AVG>
+__global__ __launch_bounds__(kBlock) void k_exact(PointArgs a)
+{
+    extern __shared__ unsigned long long sacc[];
+    Acc A;
+    acc_init(A, sacc, a.P.np, a.lds_np, AVG, a.res);
+    const uint32_t n = min(*a.qcount, a.qcap);
+    const DevQuery& q = a.q;
+    const int agg_t = AVG ? a.cols.type[q.slot_col[q.agg_slot < 0 ? 0 : q.agg_slot]] : RJ_ATTR_UINT;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4 e = a.queue[i];
+        const uint32_t v = __float_as_uint(e.z);
+        const int key = exact_first<AVG>(a.g, a.P, A, e.x, e.y, agg_t, v);
+        if (key >= 0) {
+            unsigned long long h = 0, l = 0;
+            if (AVG) agg_bits(agg_t, v, h, l);
+            acc_add(A, key, 1ull, h, l, AVG);
+        }
+    }
+    acc_flush(A, AVG);
+}
+
 inline int blocks_for(int64_t n, int64_t per, int cap)
 {
     int64_t b = (n + per - 1) / per;
@@ -539,9 +563,21 @@ void launch_points(hipStream_t s, int mode, PointArgs a)
     case PT_LABEL_HYBRID:
         if (avg) RJ_LAUNCH_PTS(PT_LABEL_HYBRID, true); else RJ_LAUNCH_PTS(PT_LABEL_HYBRID, false);
         break;
+    case PT_HYBRID_QUEUE:
+        if (avg) RJ_LAUNCH_PTS(PT_HYBRID_QUEUE, true); else RJ_LAUNCH_PTS(PT_HYBRID_QUEUE, false);
+        break;
     default: if (avg) RJ_LAUNCH_PTS(PT_SPLAT, true); else RJ_LAUNCH_PTS(PT_SPLAT, false); break;
     }
 #undef RJ_LAUNCH_PTS
+}
+
+void launch_exact(hipStream_t s, const PointArgs& a)
+{
+    const bool avg = a.q.agg_type != RJ_AGG_COUNT;
+    const size_t lds = acc_lds_bytes(a.P.np, a.lds_np, avg);
+    const int blocks = 2048;  // grid-stride over the queue; the count is read on the device
+    if (avg) hipLaunchKernelGGL(k_exact<true>, dim3(blocks), dim3(kBlock), lds, s, a);
+    else hipLaunchKernelGGL(k_exact<false>, dim3(blocks), dim3(kBlock), lds, s, a);
 }
 
 }  // namespace rj

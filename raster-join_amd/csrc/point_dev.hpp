@@ -453,6 +453,37 @@ __device__ __forceinline__ void exact_compact(const PointArgs& a, const Acc& A, 
     }
 }
 
+// Append a wave's border rows to the pass's queue: one atomic per wave
+// reserves the space, the lanes write their rows in ballot order. Rows past
+// the capacity are dropped; the count still grows, so the host sees the
+// overflow.
+template <int R>
+__device__ __forceinline__ void queue_border(const PointArgs& a, const float (&px)[R], const float (&py)[R],
+                                             const uint32_t (&vraw)[R], const bool (&need)[R])
+{
+    const int lane = (int)__lane_id();
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int cnt = 0;
+    int pos[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const unsigned long long m = __ballot(need[r]);
+        pos[r] = cnt + __popcll(m & below);
+        cnt += __popcll(m);
+    }
+    if (cnt == 0) return;
+    const unsigned long long act = __ballot(1);
+    const int lead = __ffsll((long long)act) - 1;
+    uint32_t base = 0;
+    if (lane == lead) base = atomicAdd(a.qcount, (uint32_t)cnt);
+    base = __shfl(base, lead, kWave);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t i = base + (uint32_t)pos[r];
+        if (need[r] && i < a.qcap) a.queue[i] = make_float4(px[r], py[r], __uint_as_float(vraw[r]), 0.0f);
+    }
+}
+
 // Rows [lo, hi) of one point pass, read in groups of R consecutive rows per
 // thread from `start` (a multiple of R, <= lo). Every row gets one key
 // (polygon id, or pixel for PT_SPLAT) and the lane keeps a running
@@ -537,7 +568,7 @@ __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, in
 #pragma unroll
                 for (int r = 0; r < R; r++)
                     if (lab[r] == kLabelMixed) lab[r] = label_at(a.label, a.label16, (int64_t)iys[r] * a.cv.LW + ixs[r]);
-                if (MODE == PT_LABEL_HYBRID && a.halo) {
+                if ((MODE == PT_LABEL_HYBRID || MODE == PT_HYBRID_QUEUE) && a.halo) {
 #pragma unroll
                     for (int r = 0; r < R; r++)
                         if (pix[r] >= 0 && lab[r] != kLabelBorder && halo_near(a, px[r], py[r], ixs[r], iys[r]))
@@ -545,11 +576,12 @@ __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, in
                 }
 #pragma unroll
                 for (int r = 0; r < R; r++) key[r] = lab[r] < kLabelMaxId ? (int)lab[r] : -1;
-                if (MODE == PT_LABEL_HYBRID) {
+                if (MODE == PT_LABEL_HYBRID || MODE == PT_HYBRID_QUEUE) {
                     bool need[R];
 #pragma unroll
                     for (int r = 0; r < R; r++) need[r] = lab[r] == kLabelBorder;
-                    exact_compact<AVG, LDS, R>(a, A, agg_t, px, py, vraw, need, key);
+                    if (MODE == PT_LABEL_HYBRID) exact_compact<AVG, LDS, R>(a, A, agg_t, px, py, vraw, need, key);
+                    else queue_border<R>(a, px, py, vraw, need);
                 }
             }
         }

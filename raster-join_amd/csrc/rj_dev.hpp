@@ -90,7 +90,9 @@ struct Band {
 };
 
 enum BandMode { BAND_SUM = 0, BAND_LABEL = 1, BAND_BOUNDS = 2, BAND_LABEL_ATOMIC = 3 };
-enum PointMode { PT_INDEX = 0, PT_LABEL = 1, PT_LABEL_HYBRID = 2, PT_SPLAT = 3 };
+// PT_HYBRID_QUEUE: the hybrid pass that appends its border points to a queue
+// for k_exact instead of testing them itself.
+enum PointMode { PT_INDEX = 0, PT_LABEL = 1, PT_LABEL_HYBRID = 2, PT_SPLAT = 3, PT_HYBRID_QUEUE = 4 };
 
 // Arguments of one point pass.
 struct PointArgs {
@@ -115,6 +117,11 @@ struct PointArgs {
     const uint32_t* halo;
     int halo_wpr;
     float halo_d;
+    // PT_HYBRID_QUEUE: border points (x, y, aggregate bits) go to queue[0 ..
+    // qcap); *qcount counts them (a count above qcap means some were dropped)
+    float4* queue;
+    uint32_t* qcount;
+    uint32_t qcap;
     DevResult res;
     int lds_np;           // LDS sums indexed by polygon when np <= lds_np, else hashed
     uint32_t* cnt;        // canvas (PT_SPLAT)
@@ -195,6 +202,9 @@ void launch_dilate_to_label(hipStream_t s, const uint32_t* mask_bits, const DevC
                             void* label, int label16);
 // mode PT_*; a.start and a.per_block are filled in by the launcher.
 void launch_points(hipStream_t s, int mode, PointArgs a);
+// Exact test of the queued border points of a PT_HYBRID_QUEUE pass (the same
+// arguments), summed per workgroup in LDS.
+void launch_exact(hipStream_t s, const PointArgs& a);
 // Fused pass: clears the results named by the block and the work counter,
 // then `nwg` persistent workgroups take chunks from the counter until none is
 // left; rows = rows per thread (4 or 8). The block is read-only on the device.
