@@ -19,6 +19,11 @@
 namespace rj {
 namespace {
 
+// The block through the constant address space: it does not change during a
+// pass, so the compiler may keep a query's arguments in scalar registers
+// across the loop's global atomics.
+typedef const __attribute__((address_space(4))) BatchBlock ConstBlock;
+
 // One query over one chunk, with the single-query loop and its LDS sums
 // (acc_init / acc_flush per chunk).
 template <int MODE, bool AVG, int R>
@@ -33,8 +38,6 @@ __device__ __forceinline__ void query_rows(const PointArgs& a, int64_t r0, int64
     if (run.ck >= 0) run_flush<MODE, AVG>(a, A, run.ck, run.cc, run.ch, run.cl);
     acc_flush(A, AVG);
 }
-
-typedef const __attribute__((address_space(4))) BatchBlock ConstBlock;
 
 template <int R>
 __global__ __launch_bounds__(kBlock) void k_batch(const BatchBlock* __restrict__ B, uint32_t* __restrict__ next)
@@ -72,9 +75,6 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchBlock* __restrict__
         while (m) {
             const int q = __ffsll((long long)m) - 1;
             m &= m - 1;
-            // the block is read through the constant address space: it does
-            // not change during the pass, so the compiler may keep the query's
-            // arguments in scalar registers across the loop's global atomics
             const PointArgs& a = *(const PointArgs*)(&CB->q[q]);
             const bool avg = a.q.agg_type != RJ_AGG_COUNT;
             switch (CB->mode[q] * 2 + (avg ? 1 : 0)) {
@@ -82,6 +82,8 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchBlock* __restrict__
             case PT_LABEL * 2 + 1: query_rows<PT_LABEL, true, R>(a, r0, r1, sacc, lds_cap); break;
             case PT_LABEL_HYBRID * 2: query_rows<PT_LABEL_HYBRID, false, R>(a, r0, r1, sacc, lds_cap); break;
             case PT_LABEL_HYBRID * 2 + 1: query_rows<PT_LABEL_HYBRID, true, R>(a, r0, r1, sacc, lds_cap); break;
+            case PT_HYBRID_QUEUE * 2: query_rows<PT_HYBRID_QUEUE, false, R>(a, r0, r1, sacc, lds_cap); break;
+            case PT_HYBRID_QUEUE * 2 + 1: query_rows<PT_HYBRID_QUEUE, true, R>(a, r0, r1, sacc, lds_cap); break;
             case PT_INDEX * 2: query_rows<PT_INDEX, false, R>(a, r0, r1, sacc, lds_cap); break;
             default: query_rows<PT_INDEX, true, R>(a, r0, r1, sacc, lds_cap); break;
             }
@@ -89,25 +91,92 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchBlock* __restrict__
     }
 }
 
-// Zero the result words of the pass and its work counter.
-__global__ void k_batch_clear(const BatchBlock* __restrict__ B, uint32_t* __restrict__ next)
+// The queued border points of one work item (one query's queue, entries
+// [i0, i1)): exact test and LDS sums, as k_exact.
+template <bool AVG>
+__device__ __forceinline__ void exact_tile(const PointArgs& a, uint32_t i0, uint32_t i1, unsigned long long* sacc,
+                                           int lds_cap)
+{
+    Acc A;
+    __syncthreads();  // the previous item's sums have been flushed
+    acc_init(A, sacc, a.P.np, lds_cap, AVG, a.res);
+    const DevQuery& q = a.q;
+    const int agg_t = AVG ? a.cols.type[q.slot_col[q.agg_slot < 0 ? 0 : q.agg_slot]] : RJ_ATTR_UINT;
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        const float4 e = a.queue[i];
+        const uint32_t v = __float_as_uint(e.z);
+        const int key = exact_first<AVG>(a.g, a.P, A, e.x, e.y, agg_t, v);
+        if (key >= 0) {
+            unsigned long long h = 0, l = 0;
+            if (AVG) agg_bits(agg_t, v, h, l);
+            acc_add(A, key, 1ull, h, l, AVG);
+        }
+    }
+    acc_flush(A, AVG);
+}
+
+// Second kernel of a fused pass: the border queues of its PT_HYBRID_QUEUE
+// queries in items of kExactTile points, taken from a counter.
+__global__ __launch_bounds__(kBlock) void k_exact_batch(const BatchBlock* __restrict__ B,
+                                                       uint32_t* __restrict__ counters)
+{
+    __shared__ int64_t s_item0[kBatchMax + 1];
+    __shared__ uint32_t s_take;
+    extern __shared__ unsigned long long sacc[];
+    ConstBlock* CB = (ConstBlock*)B;
+    const int nq = B->nq;
+    const int lds_cap = B->lds_cap;
+    if (threadIdx.x == 0) {
+        int64_t t = 0;
+        for (int j = 0; j < nq; j++) {
+            s_item0[j] = t;
+            if (B->mode[j] == PT_HYBRID_QUEUE) {
+                const uint32_t n = min(counters[2 + j], B->q[j].qcap);
+                t += (n + kExactTile - 1) / kExactTile;
+            }
+        }
+        s_item0[nq] = t;
+    }
+    __syncthreads();
+    const int64_t nitems = s_item0[nq];
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_take = atomicAdd(&counters[1], 1u);
+        __syncthreads();
+        const uint32_t c = __builtin_amdgcn_readfirstlane(s_take);
+        if ((int64_t)c >= nitems) break;  // every workgroup ends here once the items are taken
+        int j = 0;
+        while (j + 1 < nq && s_item0[j + 1] <= (int64_t)c) j++;
+        j = __builtin_amdgcn_readfirstlane(j);
+        const PointArgs& a = *(const PointArgs*)(&CB->q[j]);
+        const uint32_t n = min(counters[2 + j], a.qcap);
+        const uint32_t i0 = (uint32_t)((int64_t)c - s_item0[j]) * kExactTile;
+        const uint32_t i1 = min(n, i0 + kExactTile);
+        if (a.q.agg_type != RJ_AGG_COUNT) exact_tile<true>(a, i0, i1, sacc, lds_cap);
+        else exact_tile<false>(a, i0, i1, sacc, lds_cap);
+    }
+}
+
+// Zero the result words of the pass and its counters.
+__global__ void k_batch_clear(const BatchBlock* __restrict__ B, uint32_t* __restrict__ counters)
 {
     unsigned long long* res = B->res;
     const int64_t n = B->res_words;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *next = 0;
+    if (blockIdx.x == 0 && threadIdx.x < kBatchCounters) counters[threadIdx.x] = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         res[i] = 0;
 }
 
 }  // namespace
 
-void launch_batch(hipStream_t s, const BatchBlock* dev_block, uint32_t* counter, int rows, int nwg, size_t lds)
+void launch_batch(hipStream_t s, const BatchBlock* dev_block, uint32_t* counters, int rows, int nwg, size_t lds)
 {
-    hipLaunchKernelGGL(k_batch_clear, dim3(256), dim3(kBlock), 0, s, dev_block, counter);
+    hipLaunchKernelGGL(k_batch_clear, dim3(256), dim3(kBlock), 0, s, dev_block, counters);
     if (rows == 4)
-        hipLaunchKernelGGL(k_batch<4>, dim3(nwg), dim3(kBlock), lds, s, dev_block, counter);
+        hipLaunchKernelGGL(k_batch<4>, dim3(nwg), dim3(kBlock), lds, s, dev_block, counters);
     else
-        hipLaunchKernelGGL(k_batch<8>, dim3(nwg), dim3(kBlock), lds, s, dev_block, counter);
+        hipLaunchKernelGGL(k_batch<8>, dim3(nwg), dim3(kBlock), lds, s, dev_block, counters);
+    hipLaunchKernelGGL(k_exact_batch, dim3(nwg), dim3(kBlock), lds, s, dev_block, counters);
 }
 
 }  // namespace rj

@@ -523,6 +523,14 @@ double rj_ctx::hybrid_delta(const CanvasPlan& plan) const
     return 2.0 * (err / pw + 0.006);
 }
 
+// Border-queue capacity for a pass over `rows` rows: the canvas's last
+// measured border share with half again as margin, at least 65536 entries.
+uint32_t rj_ctx::queue_cap(const Canvas* C, int64_t rows)
+{
+    const double share = std::min(1.0, C->hyb_border * 1.5);
+    return (uint32_t)std::min<int64_t>(1LL << 27, std::max<int64_t>(65536, (int64_t)(share * (double)rows) + 4096));
+}
+
 void rj_ctx::hybrid_args(const Canvas* C, PointArgs* pa) const
 {
     pa->halo = C->hyb_halo ? C->d_mask.as<uint32_t>() : nullptr;
@@ -849,7 +857,7 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
         // border points queued for a separate exact pass; a queue that runs
         // out of room is seen at the readback and the pass is rerun inline
         const int64_t rows = std::max<int64_t>(0, re0 - rb0);
-        const uint32_t qcap = (uint32_t)std::min<int64_t>(1LL << 27, std::max<int64_t>(65536, rows / 8));
+        const uint32_t qcap = queue_cap(C, rows);
         if (opt_queue && alloc(d_queue, (size_t)qcap * 16 + 256, "border queue") == RJ_OK) {
             pa.queue = d_queue.as<float4>();
             pa.qcount = reinterpret_cast<uint32_t*>(d_queue.as<char>() + (size_t)qcap * 16);
@@ -909,6 +917,8 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
         e = hipMemcpyAsync(h_res + res_words + 2, check_queue, 4, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "execute");
+    if (check_queue && re0 - rb0 >= 65536)  // remember the border share for the next queue
+        C->hyb_border = std::max(0.01, (double)(uint32_t)h_res[res_words + 2] / (double)(re0 - rb0));
     if (check_queue && (uint32_t)h_res[res_words + 2] > pa.qcap) {
         // the queue overflowed: the whole pass again with the exact test inline
         PointArgs ia = pa;
@@ -1137,9 +1147,9 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
         size_t cur = 0;
         if ((rc = pinned((void**)&S.h_block, &cur, sizeof(BatchBlock), "pinned batch block"))) return rc;
     }
-    // the block, then the pass's work counter
+    // the block, then the pass's counters
     const size_t ctr_off = al256(sizeof(BatchBlock));
-    if ((rc = alloc(S.d_block, ctr_off + 256, "batch block"))) return rc;
+    if ((rc = alloc(S.d_block, ctr_off + al256(kBatchCounters * 4), "batch block"))) return rc;
     uint32_t* counter = reinterpret_cast<uint32_t*>(S.d_block.as<char>() + ctr_off);
     if (!S.done) (void)hipEventCreateWithFlags(&S.done, hipEventDisableTiming);
     BatchBlock& B = *S.h_block;
@@ -1154,7 +1164,6 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
     }
     S.res_words = words;
     if ((rc = alloc(S.d_res, (size_t)words * 8, "batch results"))) return rc;
-    if ((rc = pinned((void**)&S.h_res, &S.h_res_cap, (size_t)words * 8, "pinned batch results"))) return rc;
     unsigned long long* dres = S.d_res.as<unsigned long long>();
     B.res = dres;
     B.res_words = words;
@@ -1210,6 +1219,41 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
         qb[j] = s.rb;
         qe[j] = s.re;
         narrow_rows(t, s, &qb[j], &qe[j]);
+    }
+    // border queues of the hybrid members: one region each, 1/8 of its rows
+    // (at least 65536); members beyond 2^28 queued points in all test inline
+    S.qcap.assign(members.size(), 0);
+    S.qrows.assign(members.size(), 0);
+    S.qcanvas.assign(members.size(), nullptr);
+    if (opt_queue) {
+        int64_t total = 0;
+        for (size_t j = 0; j < members.size(); j++) {
+            if (B.mode[j] != PT_LABEL_HYBRID) continue;
+            const QuerySpec& s = qs[members[j]];
+            CanvasPlan plan;
+            make_plan(s.P->ps.bbox, canvas_eps(s), s.ground_res, &plan);
+            Canvas* C = s.P->canvases.at(plan.cell).get();
+            S.qcanvas[j] = C;
+            S.qrows[j] = qe[j] - qb[j];
+            const int64_t cap = queue_cap(C, qe[j] - qb[j]);
+            if (total + cap > (1LL << 28)) continue;
+            S.qcap[j] = (uint32_t)cap;
+            total += cap;
+        }
+        if (total > 0 && alloc(S.d_queue, (size_t)total * 16, "batch border queues") != RJ_OK) {
+            std::fill(S.qcap.begin(), S.qcap.end(), 0u);
+            total = 0;
+        }
+        int64_t off = 0;
+        for (size_t j = 0; j < members.size(); j++) {
+            if (!S.qcap[j]) continue;
+            PointArgs& a = B.q[j];
+            B.mode[j] = PT_HYBRID_QUEUE;
+            a.queue = S.d_queue.as<float4>() + off;
+            a.qcount = counter + 2 + j;
+            a.qcap = S.qcap[j];
+            off += S.qcap[j];
+        }
     }
     // segments: elementary intervals of the row ranges with a constant query set
     std::vector<int64_t> cut;
@@ -1292,9 +1336,15 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
         if (e != hipSuccess) return hip_fail(e, "batch block");
         if ((rc = launch("batch", [&] { launch_batch(stream, dblock, counter, rows, nwg, lds); }))) return rc;
     }
+    if ((rc = pinned((void**)&S.h_res, &S.h_res_cap, ((size_t)words + kBatchCounters) * 8, "pinned batch results")))
+        return rc;
     if (words) {
         hipError_t e = hipMemcpyAsync(S.h_res, S.d_res.p, (size_t)words * 8, hipMemcpyDeviceToHost, stream);
         if (e != hipSuccess) return hip_fail(e, "batch results");
+    }
+    {
+        hipError_t e = hipMemcpyAsync(S.h_res + words, counter, kBatchCounters * 4, hipMemcpyDeviceToHost, stream);
+        if (e != hipSuccess) return hip_fail(e, "batch counters");
     }
     (void)hipEventRecord(S.done, stream);
     S.busy = true;
@@ -1302,16 +1352,25 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
 }
 
 // Wait for a pass slot and decode its results.
-int rj_ctx::batch_finish(int slot, const QuerySpec* qs, QueryOut* outs)
+int rj_ctx::batch_finish(int slot, rj_table* t, const QuerySpec* qs, QueryOut* outs)
 {
     PassSlot& S = slots[slot];
     if (!S.busy) return RJ_OK;
     hipError_t e = hipEventSynchronize(S.done);
     S.busy = false;
     if (e != hipSuccess) return hip_fail(e, "fused pass");
+    const uint32_t* qcount = reinterpret_cast<const uint32_t*>(S.h_res + S.res_words) + 2;
     for (size_t j = 0; j < S.members.size(); j++) {
         const QuerySpec& s = qs[S.members[j]];
         QueryOut& o = outs[S.members[j]];
+        if (S.qcap[j] && S.qrows[j] >= 65536)
+            S.qcanvas[j]->hyb_border = std::max(0.01, (double)qcount[j] / (double)S.qrows[j]);
+        if (S.qcap[j] && qcount[j] > S.qcap[j]) {
+            // its border queue overflowed: the query again on its own
+            const int rc = run_single(t, s, &o);
+            if (rc) return rc;
+            continue;
+        }
         const int np = s.P->ps.np;
         const unsigned long long* r = S.h_res + S.res_off[j];
         o.counts.assign((const int64_t*)r, (const int64_t*)r + np);
@@ -1342,11 +1401,11 @@ int rj_ctx::execute_batch(rj_table* t, const QuerySpec* qs, int nq, QueryOut* ou
     group_passes(fused, kBatchMax, &passes);
     for (size_t p = 0; p < passes.size() && !rc; p++) {
         const int slot = (int)(p & 1);
-        rc = batch_finish(slot, qs, outs);
+        rc = batch_finish(slot, t, qs, outs);
         if (!rc) rc = batch_launch(slot, t, qs, passes[p]);
     }
     for (int slot = 0; slot < 2; slot++) {
-        const int r2 = batch_finish(slot, qs, outs);
+        const int r2 = batch_finish(slot, t, qs, outs);
         if (!rc) rc = r2;
     }
     for (size_t k = 0; k < alone.size() && !rc; k++) {

@@ -59,6 +59,7 @@ struct Canvas {
     int hyb16 = 0;
     int hyb_tshift = kTileShiftDefault;
     bool hyb_halo = false;  // border = touched pixels + the halo test (else dilated)
+    double hyb_border = 0.125;  // share of a pass's rows last queued for the exact test
     float hyb_delta = 0;    // halo width in pixels
     DevBuf d_hlab, d_htiles, d_mask, d_rects;
     uint64_t last_use = 0;
@@ -198,6 +199,10 @@ struct rj_ctx {
         hipEvent_t done = nullptr;
         std::vector<int> members;  // query indices of the pass
         std::vector<int64_t> res_off;
+        std::vector<uint32_t> qcap;  // border-queue capacity per member (0: inline test)
+        std::vector<int64_t> qrows;  // rows each member reads
+        std::vector<rj::Canvas*> qcanvas;
+        rj::DevBuf d_queue;
         int64_t res_words = 0;
         bool busy = false;
     } slots[2];
@@ -264,6 +269,7 @@ struct rj_ctx {
     int build_labels(rj_polyset* P, rj::Canvas* C, void* lab, int label16, rj::DevBuf& tiles, int tshift,
                      bool count);
     int hybrid_radius(const rj::CanvasPlan& plan) const;
+    static uint32_t queue_cap(const rj::Canvas* C, int64_t rows);
     double hybrid_delta(const rj::CanvasPlan& plan) const;
     void hybrid_args(const rj::Canvas* C, rj::PointArgs* pa) const;
     int make_room(int64_t bytes);
@@ -289,7 +295,7 @@ struct rj_ctx {
     int batch_prepare(rj_table* t, const rj::QuerySpec* qs, int nq, std::vector<int>* fused,
                       std::vector<int>* alone, std::vector<int>* status);
     int batch_launch(int slot, rj_table* t, const rj::QuerySpec* qs, const std::vector<int>& members);
-    int batch_finish(int slot, const rj::QuerySpec* qs, rj::QueryOut* outs);
+    int batch_finish(int slot, rj_table* t, const rj::QuerySpec* qs, rj::QueryOut* outs);
     // every query of the batch: fused passes, then the queries that run alone;
     // status[i] receives each query's own code
     int execute_batch(rj_table* t, const rj::QuerySpec* qs, int nq, rj::QueryOut* outs, int* status);

@@ -135,6 +135,10 @@ struct PointArgs {
 
 constexpr int kBatchMax = 64;    // queries of one fused pass (one bit each in a segment mask)
 constexpr int kBatchSegs = 2 * kBatchMax + 1;
+// Device counters of a pass, after its block: the chunk counter, the exact-tile
+// counter, then the border-queue count of every query.
+constexpr int kBatchCounters = 2 + kBatchMax;
+constexpr int kExactTile = 4096;  // queued border points per exact-pass work item
 
 // Rows [r0, r1) read by the queries of `mask`; split into chunks of
 // chunk_rows rows numbered from chunk0.
@@ -208,7 +212,9 @@ void launch_exact(hipStream_t s, const PointArgs& a);
 // Fused pass: clears the results named by the block and the work counter,
 // then `nwg` persistent workgroups take chunks from the counter until none is
 // left; rows = rows per thread (4 or 8). The block is read-only on the device.
-// lds: dynamic LDS bytes (3 x 8 x the block's lds_cap).
-void launch_batch(hipStream_t s, const BatchBlock* dev_block, uint32_t* counter, int rows, int nwg, size_t lds);
+// lds: dynamic LDS bytes of the per-polygon sums. Queries in PT_HYBRID_QUEUE
+// mode then get their queued border points tested by a second persistent
+// kernel. counters: kBatchCounters words.
+void launch_batch(hipStream_t s, const BatchBlock* dev_block, uint32_t* counters, int rows, int nwg, size_t lds);
 
 }  // namespace rj

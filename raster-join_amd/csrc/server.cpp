@@ -167,7 +167,7 @@ void rj_server::finish_slot(int s)
     if (!J) return;
     slot_job[s].reset();
     const std::vector<int> members = ctx->slots[s].members;
-    const int rc = ctx->batch_finish(s, J->specs.data(), J->outs.data());
+    const int rc = ctx->batch_finish(s, t, J->specs.data(), J->outs.data());
     for (int i : members) complete(*J, i, rc ? rc : J->st[i]);
     if (--J->passes_left == 0) release(*J);
 }
