@@ -380,6 +380,10 @@ def main():
                     help="queries of the plain one-at-a-time serving baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--headline-only", action="store_true")
+    ap.add_argument("--sections", default="headline,count,batch,serving",
+                    help="comma list of headline, count, batch, serving")
+    ap.add_argument("--calib", action="store_true",
+                    help="first run a plain 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -387,7 +391,18 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    big = 0 if args.headline_only else max(args.big, 0)
+    if args.calib:
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 1 GiB read + 1 GiB written")
+        del src, dst
+    sections = set(args.sections.split(","))
+    if args.headline_only:
+        sections = {"headline"}
+    big = max(args.big, 0) if sections & {"count", "batch", "serving"} else 0
     nrows = max(big, args.points)
     cp = rj.CONFIG_POLYS["voronoi260"]
     t0 = time.perf_counter()
@@ -397,7 +412,8 @@ def main():
     n = args.points
 
     h = rj.Handler()
-    head = headline(h, pts[:2 * n], (tm[:n], fare[:n], pas[:n]), xy, off, args)
+    head = headline(h, pts[:2 * n], (tm[:n], fare[:n], pas[:n]), xy, off, args) if "headline" in sections else {
+        "value": 0.0, "ms_per_step": 0.0, "config": {}, "roofline": None, "kernels_ms_per_step": {}}
     out = {
         "metric": METRIC,
         "value": round(head["value"], 4),
@@ -415,23 +431,26 @@ def main():
         "roofline": head["roofline"],
         "kernels_ms_per_step": head["kernels_ms_per_step"],
     }
-    if big:
+    if big and "count" in sections:
         t0 = time.perf_counter()
         mt = rj.Table(h, pts[:2 * big], (), (), organize=True)
         h.sync()
         log(f"resident {big}-row location table (Morton order) in {time.perf_counter() - t0:.1f} s")
         out["raster_count_1B"] = count_1b(h, mt, xy, off, args)
         log("raster count 1B:", out["raster_count_1B"])
+        mt.close()  # the context's bound table is replaced before any further join
+    if big and sections & {"batch", "serving"}:
         t0 = time.perf_counter()
         table = rj.Table(h, pts[:2 * big], (tm[:big], fare[:big], pas[:big]), TYPES, time_col=0,
                          time0=T0, bucket_seconds=86400)
         h.sync()
-        h.bind_table(table)  # the context no longer points at the location table
-        mt.close()
+        h.bind_table(table)
         log(f"resident {big}-row table (day buckets, Morton order) in {time.perf_counter() - t0:.1f} s")
         lib = poly_library(h, list(POLY_LIB))
-        out["batch"] = batch_section(h, table, lib, args)
-        log("batch:", out["batch"])
+        if "batch" in sections:
+            out["batch"] = batch_section(h, table, lib, args)
+            log("batch:", out["batch"])
+    if big and "serving" in sections:
         srv = serving_section(h, table, lib, args)
         out["serving"] = srv
         # the highest offered rate the loop keeps up with (served >= 90% of offered)
@@ -442,10 +461,11 @@ def main():
         out["p50_ms"] = top["p50_ms"]
         out["p99_ms"] = top["p99_ms"]
         out["saturated_qps"] = srv["saturated"]["served_qps"]
+    if big and sections & {"batch", "serving"}:
         for ps in lib.values():
             ps.close()
         table.close()
-    if not args.no_cpu:
+    if not args.no_cpu and "headline" in sections:
         out["cpu_baseline"] = cpu_baseline(xy, off, pts[:2 * n], fare[:n])
     h.close()
     print(json.dumps(out), flush=True)
