@@ -95,12 +95,21 @@ int rj_bind_table(rj_ctx* ctx, rj_table* t, int64_t row_begin, int64_t row_end);
  * column) rows are ordered by (bucket, Morton key), bucket = (t - time0) /
  * bucket_seconds clamped to the table's buckets, so a query whose constraints
  * bound that column reads only the rows of its buckets; its constraints are
- * still applied exactly. */
+ * still applied exactly.
+ * host = 1 keeps the rows in host memory (pinned when the driver grants it):
+ * each query streams its rows through the device in chunks of
+ * RJ_OPT_STREAM_ROWS rows, as the reference's point passes do
+ * (GLFunction.cpp:329-337, RasterJoin.cpp:214-236), with the copy of one
+ * chunk overlapped with the pass over the previous one. Host tables are
+ * ordered by time bucket only (no Morton order); their queries in a batch
+ * run one at a time. rj_set_points makes such a table when the points do
+ * not fit the memory budget. */
 typedef struct rj_table_options {
     int32_t organize;        /* 1: Morton order (within each bucket) */
     int32_t time_col;        /* -1: no buckets */
     uint32_t time0;          /* start of bucket 0 */
     uint32_t bucket_seconds; /* bucket width, > 0 when time_col >= 0 */
+    int32_t host;            /* 1: rows stay in host memory and stream through the device */
 } rj_table_options;
 int rj_table_create_ex(rj_ctx* ctx, int64_t n, const float* xy, int32_t ncols, const void* const* cols,
                        const int32_t* types, const rj_table_options* opt, rj_table** out);
@@ -247,11 +256,13 @@ int rj_canvas_info(rj_ctx* ctx, int32_t* W, int32_t* H);
  *                           result); 1 the exact test of every point over its grid
  *                           cell's candidates, as IndexJoin (index.glsl:186-249)
  *   RJ_OPT_EXACT_QUEUE:     1 (default) single-query hybrid passes queue their border
- *                           points for a separate exact pass; 0 tests them inline */
+ *                           points for a separate exact pass; 0 tests them inline
+ *   RJ_OPT_STREAM_ROWS:     rows per streamed chunk of host tables (0, the default:
+ *                           32M rows or what two chunks of the budget left hold) */
 enum { RJ_OPT_RASTER_PATH = 1, RJ_OPT_TIMING = 2, RJ_OPT_HYBRID_DILATION = 3, RJ_OPT_CACHE = 4,
        RJ_OPT_ORGANIZE = 5, RJ_OPT_TILE_SHIFT = 6, RJ_OPT_PASS_WORKGROUPS = 7, RJ_OPT_BATCH_ROWS = 8,
        RJ_OPT_BATCH_WORKGROUPS = 9, RJ_OPT_BATCH_CHUNK_ROWS = 10, RJ_OPT_GRAPH = 11,
-       RJ_OPT_INDEX_PATH = 12, RJ_OPT_EXACT_QUEUE = 13 };
+       RJ_OPT_INDEX_PATH = 12, RJ_OPT_EXACT_QUEUE = 13, RJ_OPT_STREAM_ROWS = 14 };
 int rj_set_option(rj_ctx* ctx, int32_t key, int64_t value);
 
 /* Per-kernel device time of the last execute (timing on): name i, total ms and

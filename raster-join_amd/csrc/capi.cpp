@@ -9,6 +9,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <thread>
+#include <vector>
 
 #include "engine.hpp"
 
@@ -33,6 +35,8 @@ size_t type_bytes(int t) { return t == RJ_ATTR_UINT8 ? 1 : 4; }
 
 constexpr uint32_t kMaxBuckets = 1u << 20;
 
+void table_bbox(rj_table* t, int64_t n, const float* xy);
+
 int table_fill(rj_ctx* ctx, rj_table* t, int64_t n, const float* xy, int32_t ncols, const void* const* cols,
                const int32_t* types)
 {
@@ -48,6 +52,13 @@ int table_fill(rj_ctx* ctx, rj_table* t, int64_t n, const float* xy, int32_t nco
         if (e == hipSuccess && n) e = hipMemcpyAsync(t->cols[c], cols[c], b, hipMemcpyHostToDevice, ctx->stream);
     }
     if (e != hipSuccess) return ctx->hip_fail(e, "point table");
+    table_bbox(t, n, xy);
+    return RJ_OK;
+}
+
+// Bounds of the finite points (0..1 when there are none).
+void table_bbox(rj_table* t, int64_t n, const float* xy)
+{
     float mn[2] = {0, 0}, mx[2] = {1, 1};
     bool first = true;
     for (int64_t i = 0; i < n; i++) {
@@ -61,6 +72,68 @@ int table_fill(rj_ctx* ctx, rj_table* t, int64_t n, const float* xy, int32_t nco
         mn[1] = std::min(mn[1], y); mx[1] = std::max(mx[1], y);
     }
     t->bbox[0] = mn[0]; t->bbox[1] = mn[1]; t->bbox[2] = mx[0]; t->bbox[3] = mx[1];
+}
+
+// A host-resident table: the rows are copied into one host allocation
+// (pinned when the driver grants it), xy then each column, and stream
+// through the device per query. With a time column the rows are put in
+// bucket order (a stable counting sort) so that a time-constrained query
+// streams only its buckets' rows.
+int host_fill(rj_ctx* ctx, rj_table* t, int64_t n, const float* xy, int32_t ncols, const void* const* cols,
+              const int32_t* types)
+{
+    t->n = n;
+    t->ncols = ncols;
+    t->host = true;
+    const int nparts = ncols + 1;
+    size_t w[RJ_MAX_COLUMNS + 1];
+    size_t total = 0;
+    for (int part = 0; part < nparts; part++) {
+        if (part) t->types[part - 1] = types[part - 1];
+        w[part] = part == 0 ? 8 : type_bytes(types[part - 1]);
+        t->hoff[part] = total;
+        total += ((size_t)n * w[part] + 255) / 256 * 256;
+    }
+    total = std::max<size_t>(total, 256);
+    void* p = nullptr;
+    if (hipHostMalloc(&p, total, hipHostMallocDefault) == hipSuccess) {
+        t->hpinned = true;
+    } else {
+        (void)hipGetLastError();
+        p = std::malloc(total);
+        if (!p) return ctx->fail(RJ_ERR_NOMEM, "host point table (" + std::to_string(total) + " bytes)");
+    }
+    t->hdata = static_cast<char*>(p);
+    auto src = [&](int part) { return part == 0 ? static_cast<const char*>((const void*)xy) : static_cast<const char*>(cols[part - 1]); };
+    std::vector<int64_t> perm;  // bucket order of the rows
+    if (t->time_col >= 0 && n > 0) {
+        const uint32_t* tm = static_cast<const uint32_t*>(cols[t->time_col]);
+        uint32_t mx = 0;
+        for (int64_t i = 0; i < n; i++) mx = std::max(mx, tm[i]);
+        const uint64_t last = mx < t->t0 ? 0 : (uint64_t)(mx - t->t0) / t->span;
+        if (last + 1 > kMaxBuckets) return ctx->fail(RJ_ERR_LIMIT, "more than 2^20 time buckets");
+        const size_t nb = (size_t)last + 1;
+        auto bucket = [&](uint32_t v) -> size_t { return v < t->t0 ? 0 : (size_t)((v - t->t0) / t->span); };
+        t->bucket_off.assign(nb + 1, 0);
+        for (int64_t i = 0; i < n; i++) t->bucket_off[bucket(tm[i]) + 1]++;
+        for (size_t b = 0; b < nb; b++) t->bucket_off[b + 1] += t->bucket_off[b];
+        std::vector<int64_t> next(t->bucket_off.begin(), t->bucket_off.end() - 1);
+        perm.resize((size_t)n);
+        for (int64_t i = 0; i < n; i++) perm[(size_t)next[bucket(tm[i])]++] = i;
+    }
+    // one host thread per part
+    std::vector<std::thread> th;
+    for (int part = 0; part < nparts; part++)
+        th.emplace_back([&, part] {
+            char* dst = t->hdata + t->hoff[part];
+            const char* sp = src(part);
+            if (perm.empty()) {
+                if (n) std::memcpy(dst, sp, (size_t)n * w[part]);
+                return;
+            }
+            for (int64_t i = 0; i < n; i++) std::memcpy(dst + (size_t)i * w[part], sp + (size_t)perm[(size_t)i] * w[part], w[part]);
+        });
+    for (auto& x : th) x.join();
     return RJ_OK;
 }
 
@@ -138,6 +211,13 @@ int make_table(rj_ctx* ctx, int64_t n, const float* xy, int32_t ncols, const voi
     t->time_col = opt->time_col;
     t->t0 = opt->time0;
     t->span = opt->bucket_seconds;
+    if (opt->host) {
+        rc = host_fill(ctx, t, n, xy, ncols, cols, types);
+        if (!rc) table_bbox(t, n, xy);
+        if (rc) delete t;
+        else *out = t;
+        return rc;
+    }
     rc = table_fill(ctx, t, n, xy, ncols, cols, types);
     if (!rc && (opt->organize || opt->time_col >= 0))
         rc = table_organize(ctx, t, opt->time_col >= 0 ? static_cast<const uint32_t*>(cols[opt->time_col]) : nullptr);
@@ -285,13 +365,14 @@ int rj_set_points(rj_ctx* ctx, int64_t n, const float* xy, int32_t ncols, const 
     if (rc) return rc;
     size_t bytes = (size_t)n * 8;
     for (int c = 0; c < ncols; c++) bytes += (size_t)n * type_bytes(types[c]);
-    if (ctx->budget->limit > 0 && ctx->budget->used + (int64_t)bytes > ctx->budget->limit)
-        return ctx->fail(RJ_ERR_NOMEM, "points do not fit the memory budget (" + std::to_string(bytes) + " bytes)");
     double t0 = now_ms();
     if (ctx->own_table) delete ctx->table;
     ctx->table = nullptr;
     ctx->own_table = false;
-    rj_table_options opt{ctx->organize ? 1 : 0, -1, 0, 0};
+    // points that do not fit the memory budget stay in host memory and
+    // stream through the device in passes (GLFunction.cpp:329-337)
+    const bool host = ctx->budget->limit > 0 && ctx->budget->used + (int64_t)bytes > ctx->budget->limit;
+    rj_table_options opt{ctx->organize ? 1 : 0, -1, 0, 0, host ? 1 : 0};
     rj_table* t = nullptr;
     if ((rc = make_table(ctx, n, xy, ncols, cols, types, &opt, &t))) return rc;
     ctx->table = t;
@@ -307,7 +388,7 @@ int rj_table_create(rj_ctx* ctx, int64_t n, const float* xy, int32_t ncols, cons
 {
     RJ_GUARD(ctx);
     if (!out) return RJ_ERR_ARG;
-    rj_table_options opt{organize, -1, 0, 0};
+    rj_table_options opt{organize, -1, 0, 0, 0};
     return make_table(ctx, n, xy, ncols, cols, types, &opt, out);
 }
 
@@ -336,6 +417,15 @@ int rj_table_rows(const rj_table* t, int64_t* n)
 int rj_table_read(const rj_table* t, int64_t row0, int64_t n, float* xy, void* const* cols)
 {
     if (!t || row0 < 0 || n < 0 || row0 + n > t->n) return RJ_ERR_ARG;
+    if (t->host) {
+        if (xy && n) std::memcpy(xy, t->hdata + t->hoff[0] + row0 * 8, (size_t)n * 8);
+        for (int c = 0; c < t->ncols && cols; c++) {
+            if (!cols[c] || !n) continue;
+            const size_t w = type_bytes(t->types[c]);
+            std::memcpy(cols[c], t->hdata + t->hoff[c + 1] + row0 * w, (size_t)n * w);
+        }
+        return RJ_OK;
+    }
     (void)hipSetDevice(t->device);
     hipError_t e = hipSuccess;
     if (xy && n) e = hipMemcpy(xy, (const char*)t->xy + row0 * 8, (size_t)n * 8, hipMemcpyDeviceToHost);
@@ -445,6 +535,10 @@ int rj_set_option(rj_ctx* ctx, int32_t key, int64_t value)
         return RJ_OK;
     case RJ_OPT_GRAPH: ctx->opt_graph = value != 0; return RJ_OK;
     case RJ_OPT_EXACT_QUEUE: ctx->opt_queue = value != 0; return RJ_OK;
+    case RJ_OPT_STREAM_ROWS:
+        if (value < 0 || value > ((int64_t)1 << 32)) return ctx->fail(RJ_ERR_ARG, "stream rows out of range");
+        ctx->opt_stream_rows = value;
+        return RJ_OK;
     case RJ_OPT_INDEX_PATH:
         if (value < 0 || value > 1) return ctx->fail(RJ_ERR_ARG, "index path is 0 or 1");
         ctx->opt_index_path = (int)value;

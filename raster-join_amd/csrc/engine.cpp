@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <thread>
 
 namespace rj {
 size_t organize_scratch_bytes(int64_t n);
@@ -92,9 +93,18 @@ static size_t type_bytes(int t) { return t == RJ_ATTR_UINT8 ? 1 : 4; }
 
 rj_table::~rj_table()
 {
+    if (hdata && hpinned) (void)hipHostFree(hdata);
+    if (hdata && !hpinned) std::free(hdata);
     if (xy) (void)hipFree(xy);
     for (int c = 0; c < ncols; c++)
         if (cols[c]) (void)hipFree(cols[c]);
+}
+
+size_t rj_table::row_bytes() const
+{
+    size_t b = 8;
+    for (int c = 0; c < ncols; c++) b += type_bytes(types[c]);
+    return b;
 }
 
 size_t rj_table::bytes() const
@@ -142,6 +152,13 @@ rj_ctx::~rj_ctx()
     if (h_res) (void)hipHostFree(h_res);
     if (h_stage) (void)hipHostFree(h_stage);
     if (stage_ev) (void)hipEventDestroy(stage_ev);
+    for (int k = 0; k < 2; k++) {
+        if (h_chunk[k]) (void)hipHostFree(h_chunk[k]);
+        if (ev_h2d[k]) (void)hipEventDestroy(ev_h2d[k]);
+        if (ev_pass[k]) (void)hipEventDestroy(ev_pass[k]);
+    }
+    if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
     for (auto e : pool) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -766,11 +783,122 @@ int rj_ctx::ensure_canvas_buffers(const CanvasPlan& plan, bool sums)
     });
 }
 
+// A point pass. For a host-resident table the rows [rb, re) stream through
+// the device in chunks, as the reference's point passes do
+// (GLFunction.cpp:329-337 sizes them, RasterJoin.cpp:214-236 loops over them):
+// the copy of chunk k + 1 to the device runs on its own stream while the pass
+// over chunk k runs, two device chunk buffers alternate, and the per-polygon
+// sums (or the canvas) accumulate across chunks. Pinned table memory is
+// copied from directly; pageable memory goes through two pinned staging
+// buffers filled by host threads.
+int rj_ctx::point_pass(const char* name, int mode, PointArgs pa, rj_table* t, int64_t rb, int64_t re)
+{
+    if (!t->host) {
+        pa.begin = rb;
+        pa.end = re;
+        return launch(name, [&] { launch_points(stream, mode, pa); });
+    }
+    if (re <= rb) return RJ_OK;
+    const size_t rowb = t->row_bytes();
+    int64_t chunk = opt_stream_rows;
+    if (chunk <= 0) {
+        chunk = 32LL << 20;
+        if (budget->limit > 0) {
+            const int64_t avail = budget->limit - budget->used + (int64_t)(d_chunk[0].bytes + d_chunk[1].bytes);
+            chunk = std::min<int64_t>(chunk, avail / 2 / (int64_t)rowb - 256);
+        }
+    }
+    chunk = std::min<int64_t>(std::max<int64_t>(chunk, 1024), re - rb);
+    chunk = (chunk + 255) / 256 * 256;
+    // chunk layout: xy, then the columns the query reads, each part
+    // 256-byte aligned; the other columns are not copied
+    bool need[RJ_MAX_COLUMNS + 1] = {true};
+    for (int k = 0; k < pa.q.nslots; k++) need[pa.q.slot_col[k] + 1] = true;
+    int nparts = 0;
+    int part_of[RJ_MAX_COLUMNS + 1];  // table part (0: xy, c + 1: column c) of each chunk part
+    for (int part = 0; part <= t->ncols; part++)
+        if (need[part]) part_of[nparts++] = part;
+    size_t off[RJ_MAX_COLUMNS + 1], width[RJ_MAX_COLUMNS + 1];
+    size_t total = 0;
+    for (int i = 0; i < nparts; i++) {
+        const int part = part_of[i];
+        width[i] = part == 0 ? 8 : type_bytes(t->types[part - 1]);
+        off[i] = total;
+        total += al256((size_t)chunk * width[i]);
+    }
+    int rc;
+    hipError_t e = hipSuccess;
+    if (!copy_stream) {
+        e = hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking);
+        if (e != hipSuccess) return hip_fail(e, "copy stream");
+    }
+    if ((d_chunk[0].p && d_chunk[0].bytes < total) || (h_chunk[0] && !t->hpinned && h_chunk_bytes[0] < total)) {
+        // buffers about to be replaced may still be read by earlier copies
+        e = hipStreamSynchronize(copy_stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return hip_fail(e, "streamed chunk");
+    }
+    for (int k = 0; k < 2; k++) {
+        if ((rc = alloc(d_chunk[k], total, "streamed chunk"))) return rc;
+        if (!t->hpinned && (rc = pinned(&h_chunk[k], &h_chunk_bytes[k], total, "pinned chunk"))) return rc;
+        if (!ev_h2d[k]) {
+            e = hipEventCreateWithFlags(&ev_h2d[k], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_pass[k], hipEventDisableTiming);
+            // recorded once so that the first waits on them return at once
+            if (e == hipSuccess) e = hipEventRecord(ev_h2d[k], copy_stream);
+            if (e == hipSuccess) e = hipEventRecord(ev_pass[k], stream);
+            if (e != hipSuccess) return hip_fail(e, "stream events");
+        }
+    }
+    int k = 0;
+    for (int64_t r = rb; r < re; r += chunk, k ^= 1) {
+        const int64_t n = std::min<int64_t>(chunk, re - r);
+        char* db = d_chunk[k].as<char>();
+        // device chunk k is free once the pass over its previous rows is done
+        e = hipStreamWaitEvent(copy_stream, ev_pass[k], 0);
+        if (t->hpinned) {
+            for (int i = 0; i < nparts && e == hipSuccess; i++)
+                e = hipMemcpyAsync(db + off[i], t->hdata + t->hoff[part_of[i]] + (size_t)r * width[i],
+                                   (size_t)n * width[i], hipMemcpyHostToDevice, copy_stream);
+        } else {
+            // staging buffer k is free once its previous copy is done
+            if (e == hipSuccess) e = hipEventSynchronize(ev_h2d[k]);
+            if (e != hipSuccess) return hip_fail(e, "streamed chunk");
+            char* hb = static_cast<char*>(h_chunk[k]);
+            const int nth = 8;
+            auto copy_rows = [&](int i) {
+                const int64_t a = n * i / nth, b = n * (i + 1) / nth;
+                for (int j = 0; j < nparts; j++)
+                    std::memcpy(hb + off[j] + (size_t)a * width[j],
+                                t->hdata + t->hoff[part_of[j]] + (size_t)(r + a) * width[j], (size_t)(b - a) * width[j]);
+            };
+            std::vector<std::thread> th;
+            for (int i = 0; i < nth; i++) th.emplace_back(copy_rows, i);
+            for (auto& x : th) x.join();
+            e = hipMemcpyAsync(db, hb, total, hipMemcpyHostToDevice, copy_stream);
+        }
+        if (e == hipSuccess) e = hipEventRecord(ev_h2d[k], copy_stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(stream, ev_h2d[k], 0);
+        if (e != hipSuccess) return hip_fail(e, "streamed chunk");
+        PointArgs c = pa;
+        c.xy = reinterpret_cast<const float*>(db);
+        for (int col = 0; col < t->ncols; col++) c.cols.col[col] = nullptr;
+        for (int i = 1; i < nparts; i++) c.cols.col[part_of[i] - 1] = db + off[i];
+        c.begin = 0;
+        c.end = n;
+        if ((rc = launch(name, [&] { launch_points(stream, mode, c); }))) return rc;
+        e = hipEventRecord(ev_pass[k], stream);
+        if (e != hipSuccess) return hip_fail(e, "streamed chunk");
+        stream_passes++;
+    }
+    return RJ_OK;
+}
+
 // One query with the single-query kernels: GLFunction::execute of the
 // function type (RasterJoin, IndexJoin, HybridJoin, RasterJoinBounds).
 int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
 {
-    const int fn = s.exec;
+    int fn = s.exec;
     if (fn < RJ_RASTER || fn > RJ_ERRORBOUNDS) return fail(RJ_ERR_UNSUPPORTED, "unknown function type");
     if (!s.P) return fail(RJ_ERR_STATE, "no polygons set");
     if (!t) return fail(RJ_ERR_STATE, "no points set");
@@ -822,7 +950,7 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
         sp.cnt = d_cnt.as<uint32_t>();
         sp.chi = sums ? d_chi.as<unsigned long long>() : nullptr;
         sp.clo = sums ? d_clo.as<unsigned long long>() : nullptr;
-        return launch("splat", [&] { launch_points(stream, PT_SPLAT, sp); });
+        return point_pass("splat", PT_SPLAT, sp, t, rb0, re0);
     };
     auto canvas_join = [&]() -> int {
         int r2 = canvas_pass(avg);
@@ -836,15 +964,30 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
         });
     };
 
-    if (fn == RJ_INDEX) {
-        if ((rc = grid_for(P, s.index_res, &G))) return rc;
+    auto index_join = [&]() -> int {
+        int r2 = grid_for(P, s.index_res, &G);
+        if (r2) return r2;
         pa.g = dev_grid(G);
         pa.P = dev_polys(P, nullptr);
-        if ((rc = launch("points_index", [&] { launch_points(stream, PT_INDEX, pa); }))) return rc;
+        return point_pass("points_index", PT_INDEX, pa, t, rb0, re0);
+    };
+
+    if (fn == RJ_HYBRID) {
+        rc = canvas_for(P, canvas_eps(s), s.ground_res, &C);
+        if (!rc) rc = grid_for(P, s.index_res, &G);
+        if (!rc) rc = hybrid_labels(P, C, G);
+        // an index join whose label map does not fit the budget runs the
+        // per-point grid test instead (the same results)
+        if (rc == RJ_ERR_NOMEM && s.fn == RJ_INDEX) {
+            C = nullptr;
+            fn = RJ_INDEX;
+        } else if (rc) {
+            return rc;
+        }
+    }
+    if (fn == RJ_INDEX) {
+        if ((rc = index_join())) return rc;
     } else if (fn == RJ_HYBRID) {
-        if ((rc = canvas_for(P, canvas_eps(s), s.ground_res, &C))) return rc;
-        if ((rc = grid_for(P, s.index_res, &G))) return rc;
-        if ((rc = hybrid_labels(P, C, G))) return rc;
         pa.cv = dev_canvas(C->plan);
         pa.g = dev_grid(G);
         pa.P = dev_polys(P, C);
@@ -858,7 +1001,7 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
         // out of room is seen at the readback and the pass is rerun inline
         const int64_t rows = std::max<int64_t>(0, re0 - rb0);
         const uint32_t qcap = queue_cap(C, rows);
-        if (opt_queue && alloc(d_queue, (size_t)qcap * 16 + 256, "border queue") == RJ_OK) {
+        if (opt_queue && !t->host && alloc(d_queue, (size_t)qcap * 16 + 256, "border queue") == RJ_OK) {
             pa.queue = d_queue.as<float4>();
             pa.qcount = reinterpret_cast<uint32_t*>(d_queue.as<char>() + (size_t)qcap * 16);
             pa.qcap = qcap;
@@ -868,7 +1011,7 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
             if ((rc = launch("exact_queue", [&] { launch_exact(stream, pa); }))) return rc;
             check_queue = pa.qcount;
         } else {
-            if ((rc = launch("points_hybrid", [&] { launch_points(stream, PT_LABEL_HYBRID, pa); }))) return rc;
+            if ((rc = point_pass("points_hybrid", PT_LABEL_HYBRID, pa, t, rb0, re0))) return rc;
         }
     } else if (fn == RJ_RASTER) {
         if ((rc = canvas_for(P, canvas_eps(s), s.ground_res, &C))) return rc;
@@ -887,7 +1030,7 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
             pa.tshift = C->lab_tshift;
             pa.tiles = C->d_tiles.as<uint32_t>();
             pa.tiles_w = cv.LW >> pa.tshift;
-            if ((rc = launch("points_label", [&] { launch_points(stream, PT_LABEL, pa); }))) return rc;
+            if ((rc = point_pass("points_label", PT_LABEL, pa, t, rb0, re0))) return rc;
             if (C->lab_state == 0) check_aux = tiles_aux(C->d_tiles, cv, C->lab_tshift);
         } else {
             if ((rc = canvas_join())) return rc;
@@ -976,6 +1119,7 @@ int rj_ctx::execute(int fn, int64_t* out_counts, double* out_sums, int32_t* lega
     s.index_res = index_res;
     s.rb = rb;
     s.re = re;
+    stream_passes = 0;
     int rc = run_single(table, s, &last);
     if (rc) return rc;
     last_fn = fn;
@@ -1006,7 +1150,7 @@ int rj_ctx::execute(int fn, int64_t* out_counts, double* out_sums, int32_t* lega
     T.backend.push_back(0.0);
     T.pts = re - rb;
     T.np = np;
-    T.passes = (fn == RJ_INDEX || fn == RJ_HYBRID) ? index_res : 1;
+    T.passes = (fn == RJ_INDEX || fn == RJ_HYBRID) ? index_res : std::max(1, stream_passes);
     T.ncons = (int)cons.size();
     return RJ_OK;
 }
@@ -1075,7 +1219,7 @@ int rj_ctx::batch_prepare(rj_table* t, const QuerySpec* qs, int nq, std::vector<
             (*status)[i] = rc;
             continue;
         }
-        if (s.exec == RJ_ERRORBOUNDS || (s.exec == RJ_RASTER && raster_path == 1)) {
+        if (s.exec == RJ_ERRORBOUNDS || (s.exec == RJ_RASTER && raster_path == 1) || t->host) {
             alone->push_back(i);
             continue;
         }
@@ -1087,6 +1231,10 @@ int rj_ctx::batch_prepare(rj_table* t, const QuerySpec* qs, int nq, std::vector<
             rc = canvas_for(s.P, canvas_eps(s), s.ground_res, &C);
             if (!rc) rc = grid_for(s.P, s.index_res, &G);
             if (!rc) rc = hybrid_labels(s.P, C, G);
+            if (rc == RJ_ERR_NOMEM && s.fn == RJ_INDEX) {  // alone, on the grid index
+                alone->push_back(i);
+                continue;
+            }
         } else {
             rc = canvas_for(s.P, canvas_eps(s), s.ground_res, &C);
             if (!rc && C->lab_built && C->lab_state == 2) {

@@ -124,6 +124,14 @@ struct rj_table {
     int32_t time_col = -1;
     uint32_t t0 = 0, span = 0;
     std::vector<int64_t> bucket_off;
+    // host-resident (out-of-core) table: the rows stay in host memory
+    // (xy, then each column, at hoff[]) and stream through the device in
+    // chunks per pass; hdata is pinned unless hipHostMalloc refused it
+    bool host = false;
+    char* hdata = nullptr;
+    bool hpinned = false;
+    size_t hoff[RJ_MAX_COLUMNS + 1] = {0};
+    size_t row_bytes() const;
     ~rj_table();
     size_t bytes() const;
     // Rows that may hold times in [lo, hi] (all rows without buckets).
@@ -176,6 +184,14 @@ struct rj_ctx {
     rj::DevBuf d_res;
     rj::DevBuf d_queue;  // hybrid border queue (float4 entries, then its counter)
     bool opt_queue = true;
+    // out-of-core streaming: two pinned staging buffers, two device chunks
+    int64_t opt_stream_rows = 0;  // rows per chunk (0: from the budget)
+    void* h_chunk[2] = {nullptr, nullptr};
+    size_t h_chunk_bytes[2] = {0, 0};
+    rj::DevBuf d_chunk[2];
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_h2d[2] = {nullptr, nullptr}, ev_pass[2] = {nullptr, nullptr};
+    int stream_passes = 0;  // chunks of the last execute
     unsigned long long* h_res = nullptr;  // pinned
     size_t h_res_bytes = 0;
     void* h_stage = nullptr;  // pinned staging of polygon uploads
@@ -282,6 +298,8 @@ struct rj_ctx {
     rj::DevPolys dev_polys(const rj_polyset* P, const rj::Canvas* C) const;
     rj::DevGrid dev_grid(const rj::Grid* G) const;
     int ensure_canvas_buffers(const rj::CanvasPlan& plan, bool sums);
+    // One point pass over rows [rb, re) of t (streamed in chunks for a host table).
+    int point_pass(const char* name, int mode, rj::PointArgs pa, rj_table* t, int64_t rb, int64_t re);
     int64_t estimate_bytes(const rj::QuerySpec& s) const;
 
     // the single-query path (plain kernels); out receives the results

@@ -29,7 +29,7 @@ COUNT, AVG, SUM = 0, 1, 2
 OPT_RASTER_PATH, OPT_TIMING, OPT_HYBRID_DILATION, OPT_CACHE, OPT_ORGANIZE = 1, 2, 3, 4, 5
 OPT_TILE_SHIFT, OPT_PASS_WORKGROUPS, OPT_BATCH_ROWS, OPT_BATCH_WORKGROUPS, OPT_BATCH_CHUNK_ROWS, OPT_GRAPH = \
     6, 7, 8, 9, 10, 11
-OPT_INDEX_PATH, OPT_EXACT_QUEUE = 12, 13
+OPT_INDEX_PATH, OPT_EXACT_QUEUE, OPT_STREAM_ROWS = 12, 13, 14
 # how a query ran
 PATH_SINGLE, PATH_FUSED = 0, 1
 
@@ -65,7 +65,7 @@ class VoronoiParams(C.Structure):
 
 class TableOptions(C.Structure):
     _fields_ = [("organize", C.c_int32), ("time_col", C.c_int32), ("time0", C.c_uint32),
-                ("bucket_seconds", C.c_uint32)]
+                ("bucket_seconds", C.c_uint32), ("host", C.c_int32)]
 
 
 class Query(C.Structure):
@@ -208,10 +208,12 @@ def make_constraints(cons):
 # ---------------------------------------------------------------------------
 
 class Table:
-    """A resident device point table (rj_table)."""
+    """A point table (rj_table): resident in device memory, or with
+    host=True kept in host memory and streamed through the device in
+    passes per query."""
 
     def __init__(self, handler, xy, cols=(), types=(), organize=True, time_col=-1, time0=0,
-                 bucket_seconds=0):
+                 bucket_seconds=0, host=False):
         """time_col >= 0 (a UINT column) orders the rows by (time bucket,
         Morton key) with buckets of bucket_seconds starting at time0."""
         self.h = C.c_void_p()
@@ -219,8 +221,8 @@ class Table:
         cols = [np.ascontiguousarray(c) for c in cols]
         colptr = (C.c_void_p * max(1, len(cols)))(*[_p(c) for c in cols])
         typ = np.array(list(types) or [0], np.int32)
-        if time_col >= 0:
-            opt = TableOptions(int(organize), time_col, time0, bucket_seconds)
+        if time_col >= 0 or host:
+            opt = TableOptions(int(organize), time_col, time0, bucket_seconds, int(host))
             handler._check(lib().rj_table_create_ex(handler.ctx, len(xy) // 2, _p(xy), len(cols),
                                                     C.cast(colptr, C.c_void_p), _p(typ),
                                                     C.byref(opt), C.byref(self.h)))
@@ -230,6 +232,7 @@ class Table:
                                                  C.byref(self.h)))
         self.n = len(xy) // 2
         self.types = list(types)
+        self.host = bool(host)
 
     def time_rows(self, t_lo, t_hi):
         b, e = C.c_int64(), C.c_int64()

@@ -279,13 +279,13 @@ def test_errors_are_status_codes(handler):
     assert e.value.code == -6
 
 
-def test_memory_budget_rejects_oversized_inputs():
-    h = rj.Handler(gpu_mem_mb=110)   # 10 MB after the reserve
-    xy, off = polys("voronoi20")
-    h.set_polygons(xy, off)
-    pts = points(2_000_000, 2)[0]
+def test_memory_budget_rejects_oversized_polygons():
+    h = rj.Handler(gpu_mem_mb=101)   # 1 MB after the reserve
+    a = np.linspace(0, 2 * np.pi, 200_000, endpoint=False)
+    xy = np.stack([1000 * np.cos(a), 1000 * np.sin(a)], 1).astype(np.float32).ravel()
+    off = np.array([0, 200_000], np.int32)   # 1.6 MB of vertices
     with pytest.raises(rj.RJError) as e:
-        h.set_points(pts)
+        h.set_polygons(xy, off)
     assert e.value.code == -3
     h.close()
 
