@@ -510,7 +510,8 @@ __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, in
             px[r] = nx[r];
             py[r] = ny[r];
         }
-        load_xy<R>(a.xy, i0 + stride, hi, a.end, nx, ny);
+        constexpr bool late = MODE == PT_LABEL || MODE == PT_LABEL_HYBRID || MODE == PT_HYBRID_QUEUE;
+        if (!late) load_xy<R>(a.xy, i0 + stride, hi, a.end, nx, ny);
         if (i0 >= hi) continue;
         const bool full = (i0 + R <= a.end);
         bool ok[R];
@@ -556,18 +557,34 @@ __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, in
                 for (int r = 0; r < R; r++) key[r] = pix[r];
             } else {
                 uint32_t lab[R];
-                // the tile map answers for whole tiles; mixed tiles read the pixel
+                // the tile map answers for whole tiles; mixed tiles read the
+                // pixel. A row in the same tile (pixel) as the row before it
+                // takes that row's label without a load.
+                int tix[R];
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    tix[r] = pix[r] < 0 ? -1 : (iys[r] >> a.tshift) * a.tiles_w + (ixs[r] >> a.tshift);
+                uint32_t tl[R];
 #pragma unroll
                 for (int r = 0; r < R; r++) {
-                    if (pix[r] < 0) {
-                        lab[r] = kLabelNone;
-                    } else {
-                        lab[r] = a.tiles[(iys[r] >> a.tshift) * a.tiles_w + (ixs[r] >> a.tshift)];
-                    }
+                    tl[r] = kLabelNone;
+                    if (tix[r] >= 0 && (r == 0 || tix[r] != tix[r - 1])) tl[r] = a.tiles[tix[r]];
+                }
+                // the next group's x,y after the tile loads: waiting for the
+                // tiles then does not wait for them
+                if (late) load_xy<R>(a.xy, i0 + stride, hi, a.end, nx, ny);
+#pragma unroll
+                for (int r = 0; r < R; r++) lab[r] = (r == 0 || tix[r] != tix[r - 1]) ? tl[r] : lab[r > 0 ? r - 1 : 0];
+                uint32_t pl[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    pl[r] = 0;
+                    if (lab[r] == kLabelMixed && (r == 0 || pix[r] != pix[r - 1]))
+                        pl[r] = label_at(a.label, a.label16, (int64_t)iys[r] * a.cv.LW + ixs[r]);
                 }
 #pragma unroll
                 for (int r = 0; r < R; r++)
-                    if (lab[r] == kLabelMixed) lab[r] = label_at(a.label, a.label16, (int64_t)iys[r] * a.cv.LW + ixs[r]);
+                    if (lab[r] == kLabelMixed) lab[r] = (r == 0 || pix[r] != pix[r - 1]) ? pl[r] : lab[r > 0 ? r - 1 : 0];
                 if ((MODE == PT_LABEL_HYBRID || MODE == PT_HYBRID_QUEUE) && a.halo) {
 #pragma unroll
                     for (int r = 0; r < R; r++)
