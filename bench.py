@@ -172,6 +172,73 @@ def count_1b(h, table, xy, off, args):
                          "kernel_ms": round(k_ms, 4), "alg_bytes": int(alg)}}
 
 
+def configs_section(h, pts, cols, args):
+    """The index join of 1M points (grid 256, both index paths) and the
+    hybrid join of 500M points with two filters over 2200 tracts."""
+    out = {}
+    cp = rj.CONFIG_POLYS["voronoi260"]
+    xy, off = rj.gen_voronoi(cp["nsites"], seed=cp["seed"], spacing=cp["spacing"], jitter=cp["jitter"])
+    n1 = 1_000_000
+    t1 = rj.Table(h, pts[:2 * n1], (), (), organize=False)
+    h.set_polygons(xy, off)
+    h.bind_table(t1)
+    h.set_query_constraints(())
+    h.set_aggregation(rj.COUNT, -1)
+    h.set_poly_index_resolution(256)
+    cfg1 = {"workload": "index join count(), 1M points, 260 polygons, grid 256 (CPUJoin membership)"}
+    for path, name in ((0, "via_labels"), (1, "grid_pip")):
+        h.set_option(rj.OPT_INDEX_PATH, path)
+        h.execute_function(rj.INDEX)
+        ts = []
+        for _ in range(max(6, args.steps // 2)):
+            t0 = time.perf_counter()
+            h.execute_function(rj.INDEX)
+            ts.append(time.perf_counter() - t0)
+        cfg1[name] = {"ms_min": round(min(ts) * 1e3, 3), "ms_median": round(float(np.median(ts)) * 1e3, 3),
+                      "G_points_per_s": round(n1 / min(ts) / 1e9, 3)}
+    h.set_option(rj.OPT_INDEX_PATH, 0)
+    out["config1_index_1M"] = cfg1
+    t1.close()
+
+    n3 = min(500_000_000, len(pts) // 2)
+    cp = rj.CONFIG_POLYS["tracts2200"]
+    txy, toff = rj.gen_voronoi(cp["nsites"], seed=cp["seed"], spacing=cp["spacing"], jitter=cp["jitter"])
+    t3 = rj.Table(h, pts[:2 * n3], tuple(c[:n3] for c in cols), TYPES, organize=True)
+    h.set_polygons(txy, toff)
+    h.bind_table(t3)
+    lo = T0 + 5_000_000
+    h.set_query_constraints([(0, rj.GTE, lo), (0, rj.LTE, lo + int(0.3 * 31536000)), (2, rj.LTE, 2)])
+    h.set_aggregation(rj.AVG, 1)
+    h.set_accuracy_distance(20.0)
+    h.set_poly_index_resolution(1024)
+    h.set_option(rj.OPT_TIMING, 1)
+    t0 = time.perf_counter()
+    counts, _ = h.execute_function(rj.HYBRID)
+    cold_ms = (time.perf_counter() - t0) * 1e3
+    cold_k = {k: round(v[0], 3) for k, v in h.kernel_times().items()}
+    steps = max(6, args.steps // 2)
+    dt, ktot = timed_steps(h, rj.HYBRID, steps, 1)
+    kms = {k: round(v[0] / v[1], 4) for k, v in ktot.items()}
+    # the pass reads x,y, time, passengers and fare of every row (17 B) and at
+    # most once each 2-byte label of the hybrid map (8192 pixels a side)
+    kname = "points_hybrid"
+    k_ms = ktot[kname][0] / ktot[kname][1]
+    alg = n3 * 17 + 8192 * 8192 * 2
+    ach = alg / (k_ms * 1e-3) / 1e9
+    h.set_option(rj.OPT_TIMING, 0)
+    out["config3_hybrid_500M"] = {
+        "workload": f"hybrid eps=20 m, count()+avg(fare), {n3} points (Morton order), time window 30% "
+                    "and passengers<=2, 2200 tract polygons, grid 1024, polygon structures cached",
+        "ms_per_join": round(dt / steps * 1e3, 3), "G_points_per_s": round(n3 * steps / dt / 1e9, 2),
+        "selected_points": int(counts.sum()), "cold_ms": round(cold_ms, 1), "cold_kernels_ms": cold_k,
+        "kernels_ms": kms,
+        "roofline": {"kernel": kname, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "kernel_ms": round(k_ms, 4),
+                     "alg_bytes": int(alg)}}
+    t3.close()
+    return out
+
+
 POLY_LIB = {  # name: (sites, seed, spacing, jitter)
     "v10": (10, 101, (400.0, 400.0), 30.0),
     "v40": (40, 102, (150.0, 150.0), 30.0),
@@ -380,8 +447,8 @@ def main():
                     help="queries of the plain one-at-a-time serving baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--headline-only", action="store_true")
-    ap.add_argument("--sections", default="headline,count,batch,serving",
-                    help="comma list of headline, count, batch, serving")
+    ap.add_argument("--sections", default="headline,count,configs,batch,serving",
+                    help="comma list of headline, count, configs, batch, serving")
     ap.add_argument("--calib", action="store_true",
                     help="first run a plain 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -402,7 +469,7 @@ def main():
     sections = set(args.sections.split(","))
     if args.headline_only:
         sections = {"headline"}
-    big = max(args.big, 0) if sections & {"count", "batch", "serving"} else 0
+    big = max(args.big, 0) if sections & {"count", "configs", "batch", "serving"} else 0
     nrows = max(big, args.points)
     cp = rj.CONFIG_POLYS["voronoi260"]
     t0 = time.perf_counter()
@@ -439,6 +506,9 @@ def main():
         out["raster_count_1B"] = count_1b(h, mt, xy, off, args)
         log("raster count 1B:", out["raster_count_1B"])
         mt.close()  # the context's bound table is replaced before any further join
+    if big and "configs" in sections:
+        out.update(configs_section(h, pts, (tm, fare, pas), args))
+        log("configs:", {k: out[k] for k in ("config1_index_1M", "config3_hybrid_500M")})
     if big and sections & {"batch", "serving"}:
         t0 = time.perf_counter()
         table = rj.Table(h, pts[:2 * big], (tm[:big], fare[:big], pas[:big]), TYPES, time_col=0,

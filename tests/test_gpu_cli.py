@@ -1,0 +1,148 @@
+"""The --joinType driver (RasterJoin/main.cpp) end to end: polygons read
+from the reference's text format through a polygon-list file, the synthetic
+point table, the per-polygon CSVs (main.cpp:178-225) and the timing line
+(GLHandler.cpp:214-248) against the same joins through the library API."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import rasterjoin as rj
+from common import T0
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "raster-join_amd", "bin", "raster_join")
+N = 300_000
+
+# polygons as the text format holds them: a square given with its closing
+# vertex and a repeated vertex, a triangle, and a two-part polygon
+POLYS = [
+    [[(10000, 10000), (30000, 10000), (30000, 10000), (30000, 30000), (10000, 30000), (10000, 10000)]],
+    [[(31000, 5000), (45000, 12000), (33000, 28000)]],
+    [[(5000, 32000), (20000, 32000), (20000, 40000)], [(20000, 47000), (5000, 47000)]],
+]
+
+
+def write_polys(d):
+    lines = [str(len(POLYS))]
+    for p in POLYS:
+        lines.append(str(len(p)))
+        for part in p:
+            lines.append(str(len(part)))
+            lines += [f"{x} {y}" for x, y in part]
+    with open(os.path.join(d, "polys.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    with open(os.path.join(d, "list.csv"), "w") as f:
+        f.write("other,missing.txt\ntest,polys.txt\n")
+
+
+def api_polys():
+    """The same polygons after the reader's clean-up: consecutive duplicates
+    and the closing vertex dropped, parts concatenated."""
+    xy, off = [], [0]
+    for p in POLYS:
+        v = []
+        for part in p:
+            for k, pt in enumerate(part):
+                if k > 0 and v and v[-1] == pt:
+                    continue
+                v.append(pt)
+        if v[0] == v[-1]:
+            v.pop()
+        xy += [c for pt in v for c in pt]
+        off.append(len(xy) // 2)
+    return np.array(xy, np.float32), np.array(off, np.int32)
+
+
+def run_cli(tmp, *flags):
+    out = os.path.join(tmp, "out")
+    os.makedirs(out, exist_ok=True)
+    cmd = [CLI, "--polygonList", os.path.join(tmp, "list.csv"), "--polygonDataset", "test",
+           "--inputSize", str(N), "--opAggregation", out, "--outputTime", os.path.join(tmp, "time.txt"),
+           "--nIter", "2", *flags]
+    print("cli:", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return out, r.stdout
+
+
+def read_csv(path):
+    rows = [l.split("\t") for l in open(path).read().split("\n") if l]
+    return rows
+
+
+def api_handler(cons=()):
+    pts, t, fare, pas = rj.gen_points(N, params=rj.point_params(seed=7))
+    h = rj.Handler()
+    xy, off = api_polys()
+    h.set_polygons(xy, off)
+    h.set_points(pts, (t, fare, pas), (rj.UINT, rj.FLOAT, rj.UINT8))
+    h.set_query_constraints(cons)
+    return h
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="driver not built")
+def test_cli_index_count_and_timing_line(tmp_path):
+    d = str(tmp_path)
+    write_polys(d)
+    out, stdout = run_cli(d, "--joinType", "index", "--indexRes", "128")
+    rows = read_csv(os.path.join(out, "raster_0_0.csv"))
+    h = api_handler()
+    h.set_aggregation(rj.COUNT, -1)
+    h.set_poly_index_resolution(128)
+    c, _ = h.execute_function(rj.INDEX)
+    assert [int(r[0]) for r in rows] == list(range(len(POLYS)))
+    assert np.array_equal(np.array([int(r[1]) for r in rows]), c)
+    assert c.min() > 0
+    line = open(os.path.join(d, "time.txt")).read().strip().split("\n")[-1].split()
+    ref = h.print_time_stats(rj.INDEX).split()
+    assert len(line) == len(ref) and line[:5] == ref[:5]
+    assert line[0] == str(N) and line[1] == str(len(POLYS))
+    h.close()
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="driver not built")
+def test_cli_raster_avg_with_time_window(tmp_path):
+    d = str(tmp_path)
+    write_polys(d)
+    t_lo, t_hi = T0 + 1_000_000, T0 + 20_000_000
+    out, _ = run_cli(d, "--joinType", "raster", "--accuracy", "20", "--avg", "8",
+                     "--startTime", str(t_lo), "--endTime", str(t_hi))
+    rows = read_csv(os.path.join(out, f"raster_{t_hi}_20.csv"))
+    h = api_handler([(0, rj.GTE, t_lo), (0, rj.LTE, t_hi)])
+    h.set_aggregation(rj.AVG, 1)
+    h.set_accuracy_distance(20.0)
+    c, s = h.execute_function(rj.RASTER)
+    avg = np.where(c > 0, s / np.maximum(c, 1), 0.0)
+    got = np.array([float(r[1]) for r in rows])
+    assert np.allclose(got, avg, rtol=1e-9, atol=0)
+    h.close()
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="driver not built")
+def test_cli_errorbounds_csv(tmp_path):
+    d = str(tmp_path)
+    write_polys(d)
+    out, _ = run_cli(d, "--joinType", "errorbounds", "--accuracy", "50")
+    rows = read_csv(os.path.join(out, "err_bound_0_50.csv"))
+    h = api_handler()
+    h.set_aggregation(rj.COUNT, -1)
+    h.set_accuracy_distance(50.0)
+    c, _ = h.execute_function(rj.ERRORBOUNDS)
+    b = h.get_error_bounds()
+    np_ = len(POLYS)
+    fn_, fp, fn1, fp1 = b[:np_], b[np_:2 * np_], b[2 * np_:3 * np_], b[3 * np_:]
+    exp = np.stack([np.arange(np_), c - fp1, c + fn1, c - fp, c + fn_], 1)
+    assert np.array_equal(np.array([[int(x) for x in r] for r in rows]), exp)
+    # the guaranteed range holds the exact count
+    exact, _ = h.execute_function(rj.INDEX)
+    assert np.all(exp[:, 1] <= exact) and np.all(exact <= exp[:, 2])
+    h.close()
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="driver not built")
+def test_cli_rejects_unknown_join_type(tmp_path):
+    r = subprocess.run([CLI, "--joinType", "bogus"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "Wrong join type" in r.stderr
