@@ -611,6 +611,7 @@ int rj_ctx::hybrid_labels(rj_polyset* P, Canvas* C, Grid* G)
     uint32_t* t = C->d_htiles.as<uint32_t>();
     if ((rc = launch("tile_labels", [&] { launch_tile_labels(stream, lab, l16, cv, opt_tshift, t, nullptr); })))
         return rc;
+    if (halo && (rc = launch("tile_halo", [&] { launch_tile_halo(stream, cv, opt_tshift, t, mask); }))) return rc;
     C->hyb16 = l16;
     C->hyb_res = G->g.xs;
     C->hyb_tshift = opt_tshift;

@@ -541,6 +541,38 @@ void launch_tile_labels(hipStream_t s, const void* label, int label16, const Dev
         hipLaunchKernelGGL(k_tile_labels<4>, grid, dim3(kBlock), 0, s, label, label16, c.LW, c.H, TW, tiles, labeled);
 }
 
+// Hybrid tile maps: a tile of one label whose one-pixel ring holds a touched
+// pixel becomes MIXED, so that the point pass applies the halo test (which
+// reads the touched bitmap) only to the points of MIXED tiles: a point of a
+// tile left uniform has no touched pixel among its 8 neighbours.
+__global__ void k_tile_halo(uint32_t* __restrict__ tiles, int TW, int TH, int tshift,
+                            const uint32_t* __restrict__ mask, int64_t wpr, int W, int H)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= TW * TH) return;
+    if (tiles[t] == kLabelMixed) return;
+    const int T = 1 << tshift;
+    const int x0 = (t % TW) * T - 1, x1 = (t % TW) * T + T;
+    const int y0 = (t / TW) * T - 1, y1 = (t / TW) * T + T;
+    auto touched = [&](int x, int y) -> bool {
+        if (x < 0 || y < 0 || x >= W || y >= H) return false;
+        return (mask[(int64_t)y * wpr + (x >> 5)] >> (x & 31)) & 1u;
+    };
+    bool hit = false;
+    for (int x = x0; x <= x1 && !hit; x++) hit = touched(x, y0) || touched(x, y1);
+    for (int y = y0 + 1; y < y1 && !hit; y++) hit = touched(x0, y) || touched(x1, y);
+    if (hit) tiles[t] = kLabelMixed;
+}
+
+void launch_tile_halo(hipStream_t s, const DevCanvas& c, int tshift, uint32_t* tiles, const uint32_t* mask)
+{
+    const int TW = c.LW >> tshift, TH = (c.H + (1 << tshift) - 1) >> tshift;
+    const int64_t wpr = (c.W + 31) / 32;
+    const int n = TW * TH;
+    hipLaunchKernelGGL(k_tile_halo, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, tiles, TW, TH,
+                       tshift, mask, wpr, c.W, c.H);
+}
+
 void launch_points(hipStream_t s, int mode, PointArgs a)
 {
     if (a.end <= a.begin) return;

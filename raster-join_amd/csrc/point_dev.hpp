@@ -575,6 +575,11 @@ __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, in
                 if (late) load_xy<R>(a.xy, i0 + stride, hi, a.end, nx, ny);
 #pragma unroll
                 for (int r = 0; r < R; r++) lab[r] = (r == 0 || tix[r] != tix[r - 1]) ? tl[r] : lab[r > 0 ? r - 1 : 0];
+                // hybrid tile maps leave a tile uniform only when no touched
+                // pixel borders it: only points of MIXED tiles need the halo test
+                bool hchk[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) hchk[r] = lab[r] == kLabelMixed;
                 uint32_t pl[R];
 #pragma unroll
                 for (int r = 0; r < R; r++) {
@@ -588,7 +593,7 @@ __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, in
                 if ((MODE == PT_LABEL_HYBRID || MODE == PT_HYBRID_QUEUE) && a.halo) {
 #pragma unroll
                     for (int r = 0; r < R; r++)
-                        if (pix[r] >= 0 && lab[r] != kLabelBorder && halo_near(a, px[r], py[r], ixs[r], iys[r]))
+                        if (hchk[r] && lab[r] != kLabelBorder && halo_near(a, px[r], py[r], ixs[r], iys[r]))
                             lab[r] = kLabelBorder;
                 }
 #pragma unroll

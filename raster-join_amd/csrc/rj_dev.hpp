@@ -198,6 +198,7 @@ void launch_band(hipStream_t s, int mode, const Band* bands, int nbands, const D
 // null) receives the count of labeled pixels.
 void launch_tile_labels(hipStream_t s, const void* label, int label16, const DevCanvas& c, int tshift,
                         uint32_t* tiles, unsigned long long* labeled);
+void launch_tile_halo(hipStream_t s, const DevCanvas& c, int tshift, uint32_t* tiles, const uint32_t* mask);
 void launch_touch(hipStream_t s, const int32_t* X, const int32_t* Y, const int32_t* prev,
                   int64_t nverts, const DevCanvas& c, uint32_t* mask_bits);
 void launch_mark_rects(hipStream_t s, const int32_t* rects, int nrects, const DevCanvas& c,
