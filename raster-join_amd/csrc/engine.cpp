@@ -578,7 +578,9 @@ int rj_ctx::hybrid_labels(rj_polyset* P, Canvas* C, Grid* G)
     if (!rects.empty() && (rc = upload(C->d_rects, rects.data(), rects.size() * 4, "deficient cells"))) return rc;
     const DevPolys dp = dev_polys(P, C);
     uint32_t* mask = C->d_mask.as<uint32_t>();
-    int l16 = P->ps.np < (int)0xFFF0 ? 1 : 0;
+    // 16-bit maps keep one bit for the halo flag
+    int l16 = P->ps.np < (int)(kHaloFlag16 - 16) ? 1 : 0;
+    if (P->ps.np >= (int)kHaloFlag32) return fail(RJ_ERR_LIMIT, "too many polygons for a hybrid label map");
     if ((rc = alloc(C->d_hlab, npx * (l16 ? 2 : 4) + 16, "hybrid label map"))) return rc;
     if ((rc = build_labels(P, C, C->d_hlab.p, l16, C->d_htiles, opt_tshift, true))) return rc;
     unsigned long long a[2] = {0, 0};
@@ -608,10 +610,10 @@ int rj_ctx::hybrid_labels(rj_polyset* P, Canvas* C, Grid* G)
             return rc;
     }
     if ((rc = launch("dilate", [&] { launch_dilate_to_label(stream, mask, cv, r, lab, l16); }))) return rc;
+    if (halo && (rc = launch("halo_flags", [&] { launch_halo_flags(stream, mask, cv, lab, l16); }))) return rc;
     uint32_t* t = C->d_htiles.as<uint32_t>();
     if ((rc = launch("tile_labels", [&] { launch_tile_labels(stream, lab, l16, cv, opt_tshift, t, nullptr); })))
         return rc;
-    if (halo && (rc = launch("tile_halo", [&] { launch_tile_halo(stream, cv, opt_tshift, t, mask); }))) return rc;
     C->hyb16 = l16;
     C->hyb_res = G->g.xs;
     C->hyb_tshift = opt_tshift;

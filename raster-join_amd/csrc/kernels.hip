@@ -541,36 +541,53 @@ void launch_tile_labels(hipStream_t s, const void* label, int label16, const Dev
         hipLaunchKernelGGL(k_tile_labels<4>, grid, dim3(kBlock), 0, s, label, label16, c.LW, c.H, TW, tiles, labeled);
 }
 
-// Hybrid tile maps: a tile of one label whose one-pixel ring holds a touched
-// pixel becomes MIXED, so that the point pass applies the halo test (which
-// reads the touched bitmap) only to the points of MIXED tiles: a point of a
-// tile left uniform has no touched pixel among its 8 neighbours.
-__global__ void k_tile_halo(uint32_t* __restrict__ tiles, int TW, int TH, int tshift,
-                            const uint32_t* __restrict__ mask, int64_t wpr, int W, int H)
+// Hybrid label maps in the halo mode: every pixel that is not itself border
+// and has a touched pixel among its 8 neighbours gets its label flagged (or
+// kLabelNoneHalo), so that the point pass runs the halo test (which reads the
+// touched bitmap) only for points of flagged pixels. One thread per 32-pixel
+// word of a row.
+__global__ void k_halo_flags(const uint32_t* __restrict__ mask, int W, int H, int LW, int wpr, void* label,
+                             int label16)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= TW * TH) return;
-    if (tiles[t] == kLabelMixed) return;
-    const int T = 1 << tshift;
-    const int x0 = (t % TW) * T - 1, x1 = (t % TW) * T + T;
-    const int y0 = (t / TW) * T - 1, y1 = (t / TW) * T + T;
-    auto touched = [&](int x, int y) -> bool {
-        if (x < 0 || y < 0 || x >= W || y >= H) return false;
-        return (mask[(int64_t)y * wpr + (x >> 5)] >> (x & 31)) & 1u;
-    };
-    bool hit = false;
-    for (int x = x0; x <= x1 && !hit; x++) hit = touched(x, y0) || touched(x, y1);
-    for (int y = y0 + 1; y < y1 && !hit; y++) hit = touched(x0, y) || touched(x1, y);
-    if (hit) tiles[t] = kLabelMixed;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)wpr * H) return;
+    const int row = (int)(t / wpr), w = (int)(t % wpr);
+    uint32_t m = 0, ml = 0, mr = 0;
+    for (int dr = -1; dr <= 1; dr++) {
+        const int rr = row + dr;
+        if (rr < 0 || rr >= H) continue;
+        const uint32_t* mrow = mask + (int64_t)rr * wpr;
+        m |= mrow[w];
+        if (w > 0) ml |= mrow[w - 1];
+        if (w + 1 < wpr) mr |= mrow[w + 1];
+    }
+    const uint32_t d = m | (m << 1) | (ml >> 31) | (m >> 1) | (mr << 31);
+    if (!d) return;
+    const int64_t rowpix = (int64_t)row * LW;
+    for (int b = 0; b < 32; b++) {
+        const int col = w * 32 + b;
+        if (col >= W) break;
+        if (!((d >> b) & 1u)) continue;
+        if (label16) {
+            uint16_t* l = static_cast<uint16_t*>(label) + rowpix + col;
+            const uint16_t v = *l;
+            if (v == (uint16_t)kLabelBorder) continue;
+            *l = v == (uint16_t)kLabelNone ? (uint16_t)kLabelNoneHalo : (uint16_t)(v | kHaloFlag16);
+        } else {
+            uint32_t* l = static_cast<uint32_t*>(label) + rowpix + col;
+            const uint32_t v = *l;
+            if (v == kLabelBorder) continue;
+            *l = v == kLabelNone ? kLabelNoneHalo : (v | kHaloFlag32);
+        }
+    }
 }
 
-void launch_tile_halo(hipStream_t s, const DevCanvas& c, int tshift, uint32_t* tiles, const uint32_t* mask)
+void launch_halo_flags(hipStream_t s, const uint32_t* mask, const DevCanvas& c, void* label, int label16)
 {
-    const int TW = c.LW >> tshift, TH = (c.H + (1 << tshift) - 1) >> tshift;
-    const int64_t wpr = (c.W + 31) / 32;
-    const int n = TW * TH;
-    hipLaunchKernelGGL(k_tile_halo, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, tiles, TW, TH,
-                       tshift, mask, wpr, c.W, c.H);
+    const int wpr = (c.W + 31) / 32;
+    const int64_t n = (int64_t)wpr * c.H;
+    hipLaunchKernelGGL(k_halo_flags, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, mask, c.W, c.H,
+                       c.LW, wpr, label, label16);
 }
 
 void launch_points(hipStream_t s, int mode, PointArgs a)

@@ -575,11 +575,6 @@ __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, in
                 if (late) load_xy<R>(a.xy, i0 + stride, hi, a.end, nx, ny);
 #pragma unroll
                 for (int r = 0; r < R; r++) lab[r] = (r == 0 || tix[r] != tix[r - 1]) ? tl[r] : lab[r > 0 ? r - 1 : 0];
-                // hybrid tile maps leave a tile uniform only when no touched
-                // pixel borders it: only points of MIXED tiles need the halo test
-                bool hchk[R];
-#pragma unroll
-                for (int r = 0; r < R; r++) hchk[r] = lab[r] == kLabelMixed;
                 uint32_t pl[R];
 #pragma unroll
                 for (int r = 0; r < R; r++) {
@@ -591,10 +586,14 @@ __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, in
                 for (int r = 0; r < R; r++)
                     if (lab[r] == kLabelMixed) lab[r] = (r == 0 || pix[r] != pix[r - 1]) ? pl[r] : lab[r > 0 ? r - 1 : 0];
                 if ((MODE == PT_LABEL_HYBRID || MODE == PT_HYBRID_QUEUE) && a.halo) {
+                    // only pixels flagged next to a touched pixel take the halo test
+                    const uint32_t fb = a.label16 ? kHaloFlag16 : kHaloFlag32;
 #pragma unroll
-                    for (int r = 0; r < R; r++)
-                        if (hchk[r] && lab[r] != kLabelBorder && halo_near(a, px[r], py[r], ixs[r], iys[r]))
-                            lab[r] = kLabelBorder;
+                    for (int r = 0; r < R; r++) {
+                        const bool f = lab[r] == kLabelNoneHalo || (lab[r] < kLabelMaxId && (lab[r] & fb));
+                        lab[r] = lab[r] == kLabelNoneHalo ? kLabelNone : (f ? lab[r] & ~fb : lab[r]);
+                        if (f && halo_near(a, px[r], py[r], ixs[r], iys[r])) lab[r] = kLabelBorder;
+                    }
                 }
 #pragma unroll
                 for (int r = 0; r < R; r++) key[r] = lab[r] < kLabelMaxId ? (int)lab[r] : -1;

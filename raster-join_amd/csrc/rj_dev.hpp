@@ -14,7 +14,11 @@ constexpr int kSubShift = 8;
 constexpr uint32_t kLabelNone = 0xFFFFFFFFu;    // pixel center in no polygon
 constexpr uint32_t kLabelBorder = 0xFFFFFFFEu;  // pixel handled by the exact test
 constexpr uint32_t kLabelMixed = 0xFFFFFFFDu;   // tile map: the tile holds several labels
+constexpr uint32_t kLabelNoneHalo = 0xFFFFFFFCu;  // hybrid map: no polygon, a touched pixel next to it
 constexpr uint32_t kLabelMaxId = 0xFFFFFFF0u;
+// hybrid maps in the halo mode flag the labels of pixels next to a touched
+// pixel with this bit (16-bit maps: 0x8000, so ids < 0x7FF0 there)
+constexpr uint32_t kHaloFlag16 = 0x8000u, kHaloFlag32 = 0x40000000u;
 constexpr int kTileShiftDefault = 4;              // label tiles of 16 x 16 pixels (3..5 allowed)
 constexpr int kLabelRowAlign = 32;               // label-map rows are padded to 32 pixels
 
@@ -198,7 +202,7 @@ void launch_band(hipStream_t s, int mode, const Band* bands, int nbands, const D
 // null) receives the count of labeled pixels.
 void launch_tile_labels(hipStream_t s, const void* label, int label16, const DevCanvas& c, int tshift,
                         uint32_t* tiles, unsigned long long* labeled);
-void launch_tile_halo(hipStream_t s, const DevCanvas& c, int tshift, uint32_t* tiles, const uint32_t* mask);
+void launch_halo_flags(hipStream_t s, const uint32_t* mask, const DevCanvas& c, void* label, int label16);
 void launch_touch(hipStream_t s, const int32_t* X, const int32_t* Y, const int32_t* prev,
                   int64_t nverts, const DevCanvas& c, uint32_t* mask_bits);
 void launch_mark_rects(hipStream_t s, const int32_t* rects, int nrects, const DevCanvas& c,
