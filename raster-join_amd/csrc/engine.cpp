@@ -148,6 +148,7 @@ rj_ctx::~rj_ctx()
         if (S.h_block) (void)hipHostFree(S.h_block);
         if (S.h_res) (void)hipHostFree(S.h_res);
         if (S.done) (void)hipEventDestroy(S.done);
+        if (S.blk) (void)hipEventDestroy(S.blk);
     }
     if (h_res) (void)hipHostFree(h_res);
     if (h_stage) (void)hipHostFree(h_stage);
@@ -786,22 +787,13 @@ int rj_ctx::ensure_canvas_buffers(const CanvasPlan& plan, bool sums)
     });
 }
 
-// A point pass. For a host-resident table the rows [rb, re) stream through
-// the device in chunks, as the reference's point passes do
-// (GLFunction.cpp:329-337 sizes them, RasterJoin.cpp:214-236 loops over them):
-// the copy of chunk k + 1 to the device runs on its own stream while the pass
-// over chunk k runs, two device chunk buffers alternate, and the per-polygon
-// sums (or the canvas) accumulate across chunks. Pinned table memory is
-// copied from directly; pageable memory goes through two pinned staging
-// buffers filled by host threads.
-int rj_ctx::point_pass(const char* name, int mode, PointArgs pa, rj_table* t, int64_t rb, int64_t re)
+// Streaming of host tables. The chunk layout holds x,y and the columns a
+// pass reads (need[c + 1]), each part 256-byte aligned; the others are not
+// copied. Two device chunk buffers alternate: the copy into a buffer waits
+// for the pass that last read it (ev_pass), the pass over it waits for its
+// copy (ev_h2d).
+int rj_ctx::stream_setup(rj_table* t, const bool* need, int64_t rows, StreamLayout* L)
 {
-    if (!t->host) {
-        pa.begin = rb;
-        pa.end = re;
-        return launch(name, [&] { launch_points(stream, mode, pa); });
-    }
-    if (re <= rb) return RJ_OK;
     const size_t rowb = t->row_bytes();
     int64_t chunk = opt_stream_rows;
     if (chunk <= 0) {
@@ -811,23 +803,17 @@ int rj_ctx::point_pass(const char* name, int mode, PointArgs pa, rj_table* t, in
             chunk = std::min<int64_t>(chunk, avail / 2 / (int64_t)rowb - 256);
         }
     }
-    chunk = std::min<int64_t>(std::max<int64_t>(chunk, 1024), re - rb);
-    chunk = (chunk + 255) / 256 * 256;
-    // chunk layout: xy, then the columns the query reads, each part
-    // 256-byte aligned; the other columns are not copied
-    bool need[RJ_MAX_COLUMNS + 1] = {true};
-    for (int k = 0; k < pa.q.nslots; k++) need[pa.q.slot_col[k] + 1] = true;
-    int nparts = 0;
-    int part_of[RJ_MAX_COLUMNS + 1];  // table part (0: xy, c + 1: column c) of each chunk part
-    for (int part = 0; part <= t->ncols; part++)
-        if (need[part]) part_of[nparts++] = part;
-    size_t off[RJ_MAX_COLUMNS + 1], width[RJ_MAX_COLUMNS + 1];
-    size_t total = 0;
-    for (int i = 0; i < nparts; i++) {
-        const int part = part_of[i];
-        width[i] = part == 0 ? 8 : type_bytes(t->types[part - 1]);
-        off[i] = total;
-        total += al256((size_t)chunk * width[i]);
+    chunk = std::min<int64_t>(std::max<int64_t>(chunk, 1024), std::max<int64_t>(rows, 1));
+    L->chunk = (chunk + 255) / 256 * 256;
+    L->nparts = 0;
+    L->total = 0;
+    for (int part = 0; part <= t->ncols; part++) {
+        if (!need[part]) continue;
+        const int i = L->nparts++;
+        L->part_of[i] = part;
+        L->width[i] = part == 0 ? 8 : type_bytes(t->types[part - 1]);
+        L->off[i] = L->total;
+        L->total += al256((size_t)L->chunk * L->width[i]);
     }
     int rc;
     hipError_t e = hipSuccess;
@@ -835,15 +821,16 @@ int rj_ctx::point_pass(const char* name, int mode, PointArgs pa, rj_table* t, in
         e = hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking);
         if (e != hipSuccess) return hip_fail(e, "copy stream");
     }
-    if ((d_chunk[0].p && d_chunk[0].bytes < total) || (h_chunk[0] && !t->hpinned && h_chunk_bytes[0] < total)) {
+    if ((d_chunk[0].p && d_chunk[0].bytes < L->total) ||
+        (h_chunk[0] && !t->hpinned && h_chunk_bytes[0] < L->total)) {
         // buffers about to be replaced may still be read by earlier copies
         e = hipStreamSynchronize(copy_stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         if (e != hipSuccess) return hip_fail(e, "streamed chunk");
     }
     for (int k = 0; k < 2; k++) {
-        if ((rc = alloc(d_chunk[k], total, "streamed chunk"))) return rc;
-        if (!t->hpinned && (rc = pinned(&h_chunk[k], &h_chunk_bytes[k], total, "pinned chunk"))) return rc;
+        if ((rc = alloc(d_chunk[k], L->total, "streamed chunk"))) return rc;
+        if (!t->hpinned && (rc = pinned(&h_chunk[k], &h_chunk_bytes[k], L->total, "pinned chunk"))) return rc;
         if (!ev_h2d[k]) {
             e = hipEventCreateWithFlags(&ev_h2d[k], hipEventDisableTiming);
             if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_pass[k], hipEventDisableTiming);
@@ -853,44 +840,85 @@ int rj_ctx::point_pass(const char* name, int mode, PointArgs pa, rj_table* t, in
             if (e != hipSuccess) return hip_fail(e, "stream events");
         }
     }
-    int k = 0;
-    for (int64_t r = rb; r < re; r += chunk, k ^= 1) {
-        const int64_t n = std::min<int64_t>(chunk, re - r);
-        char* db = d_chunk[k].as<char>();
-        // device chunk k is free once the pass over its previous rows is done
-        e = hipStreamWaitEvent(copy_stream, ev_pass[k], 0);
-        if (t->hpinned) {
-            for (int i = 0; i < nparts && e == hipSuccess; i++)
-                e = hipMemcpyAsync(db + off[i], t->hdata + t->hoff[part_of[i]] + (size_t)r * width[i],
-                                   (size_t)n * width[i], hipMemcpyHostToDevice, copy_stream);
-        } else {
-            // staging buffer k is free once its previous copy is done
-            if (e == hipSuccess) e = hipEventSynchronize(ev_h2d[k]);
-            if (e != hipSuccess) return hip_fail(e, "streamed chunk");
-            char* hb = static_cast<char*>(h_chunk[k]);
-            const int nth = 8;
-            auto copy_rows = [&](int i) {
-                const int64_t a = n * i / nth, b = n * (i + 1) / nth;
-                for (int j = 0; j < nparts; j++)
-                    std::memcpy(hb + off[j] + (size_t)a * width[j],
-                                t->hdata + t->hoff[part_of[j]] + (size_t)(r + a) * width[j], (size_t)(b - a) * width[j]);
-            };
-            std::vector<std::thread> th;
-            for (int i = 0; i < nth; i++) th.emplace_back(copy_rows, i);
-            for (auto& x : th) x.join();
-            e = hipMemcpyAsync(db, hb, total, hipMemcpyHostToDevice, copy_stream);
-        }
-        if (e == hipSuccess) e = hipEventRecord(ev_h2d[k], copy_stream);
-        if (e == hipSuccess) e = hipStreamWaitEvent(stream, ev_h2d[k], 0);
+    return RJ_OK;
+}
+
+// Copy rows [r, r + n) of t into device chunk k on the copy stream and make
+// the context's stream wait for it. Pinned table memory is copied from
+// directly; pageable memory goes through pinned staging buffer k, filled by
+// host threads.
+int rj_ctx::stream_copy(rj_table* t, const StreamLayout& L, int k, int64_t r, int64_t n)
+{
+    char* db = d_chunk[k].as<char>();
+    hipError_t e = hipStreamWaitEvent(copy_stream, ev_pass[k], 0);
+    if (t->hpinned) {
+        for (int i = 0; i < L.nparts && e == hipSuccess; i++)
+            e = hipMemcpyAsync(db + L.off[i], t->hdata + t->hoff[L.part_of[i]] + (size_t)r * L.width[i],
+                               (size_t)n * L.width[i], hipMemcpyHostToDevice, copy_stream);
+    } else {
+        // staging buffer k is free once its previous copy is done
+        if (e == hipSuccess) e = hipEventSynchronize(ev_h2d[k]);
         if (e != hipSuccess) return hip_fail(e, "streamed chunk");
+        char* hb = static_cast<char*>(h_chunk[k]);
+        const int nth = 8;
+        auto copy_rows = [&](int i) {
+            const int64_t a = n * i / nth, b = n * (i + 1) / nth;
+            for (int j = 0; j < L.nparts; j++)
+                std::memcpy(hb + L.off[j] + (size_t)a * L.width[j],
+                            t->hdata + t->hoff[L.part_of[j]] + (size_t)(r + a) * L.width[j],
+                            (size_t)(b - a) * L.width[j]);
+        };
+        std::vector<std::thread> th;
+        for (int i = 0; i < nth; i++) th.emplace_back(copy_rows, i);
+        for (auto& x : th) x.join();
+        e = hipMemcpyAsync(db, hb, L.total, hipMemcpyHostToDevice, copy_stream);
+    }
+    if (e == hipSuccess) e = hipEventRecord(ev_h2d[k], copy_stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(stream, ev_h2d[k], 0);
+    if (e != hipSuccess) return hip_fail(e, "streamed chunk");
+    return RJ_OK;
+}
+
+// Point-pass arguments over device chunk k holding the rows from r0 on:
+// pointers rebased so that row indices stay those of the table (rows outside
+// the chunk are never read: the pass's row range and end lie inside it).
+void rj_ctx::stream_args(const rj_table* t, const StreamLayout& L, int k, int64_t r0, PointArgs* a) const
+{
+    char* db = d_chunk[k].as<char>();
+    a->xy = reinterpret_cast<const float*>(db + L.off[0]) - 2 * r0;
+    for (int col = 0; col < t->ncols; col++) a->cols.col[col] = nullptr;
+    for (int i = 1; i < L.nparts; i++) a->cols.col[L.part_of[i] - 1] = db + L.off[i] - (size_t)r0 * L.width[i];
+}
+
+// A point pass. For a host-resident table the rows [rb, re) stream through
+// the device in chunks, as the reference's point passes do
+// (GLFunction.cpp:329-337 sizes them, RasterJoin.cpp:214-236 loops over them):
+// the copy of chunk k + 1 runs on its own stream while the pass over chunk k
+// runs, and the per-polygon sums (or the canvas) accumulate across chunks.
+int rj_ctx::point_pass(const char* name, int mode, PointArgs pa, rj_table* t, int64_t rb, int64_t re)
+{
+    if (!t->host) {
+        pa.begin = rb;
+        pa.end = re;
+        return launch(name, [&] { launch_points(stream, mode, pa); });
+    }
+    if (re <= rb) return RJ_OK;
+    bool need[RJ_MAX_COLUMNS + 1] = {true};
+    for (int k = 0; k < pa.q.nslots; k++) need[pa.q.slot_col[k] + 1] = true;
+    StreamLayout L;
+    int rc = stream_setup(t, need, re - rb, &L);
+    if (rc) return rc;
+    // chunks start at multiples of 256 rows (aligned vector loads)
+    int k = 0;
+    for (int64_t c0 = rb - rb % 256; c0 < re; c0 += L.chunk, k ^= 1) {
+        const int64_t c1 = std::min<int64_t>(c0 + L.chunk, re);
+        if ((rc = stream_copy(t, L, k, c0, c1 - c0))) return rc;
         PointArgs c = pa;
-        c.xy = reinterpret_cast<const float*>(db);
-        for (int col = 0; col < t->ncols; col++) c.cols.col[col] = nullptr;
-        for (int i = 1; i < nparts; i++) c.cols.col[part_of[i] - 1] = db + off[i];
-        c.begin = 0;
-        c.end = n;
+        stream_args(t, L, k, c0, &c);
+        c.begin = std::max(rb, c0);
+        c.end = c1;
         if ((rc = launch(name, [&] { launch_points(stream, mode, c); }))) return rc;
-        e = hipEventRecord(ev_pass[k], stream);
+        const hipError_t e = hipEventRecord(ev_pass[k], stream);
         if (e != hipSuccess) return hip_fail(e, "streamed chunk");
         stream_passes++;
     }
@@ -1222,7 +1250,7 @@ int rj_ctx::batch_prepare(rj_table* t, const QuerySpec* qs, int nq, std::vector<
             (*status)[i] = rc;
             continue;
         }
-        if (s.exec == RJ_ERRORBOUNDS || (s.exec == RJ_RASTER && raster_path == 1) || t->host) {
+        if (s.exec == RJ_ERRORBOUNDS || (s.exec == RJ_RASTER && raster_path == 1)) {
             alone->push_back(i);
             continue;
         }
@@ -1290,6 +1318,52 @@ int rj_ctx::batch_prepare(rj_table* t, const QuerySpec* qs, int nq, std::vector<
 
 // Fill a pass slot's block for `members` and launch the fused pass (and the
 // copy of its results) on the stream. Returns without waiting.
+// Segments of a fused pass over rows [lo, hi): elementary intervals of the
+// members' row ranges with a constant query set, split into chunks of whole
+// row groups (opt_batch_chunk_rows rows, fewer when the rows are too few to
+// give every workgroup one).
+void rj_ctx::batch_segments(BatchBlock& B, const std::vector<int64_t>& qb, const std::vector<int64_t>& qe, int64_t lo,
+                            int64_t hi) const
+{
+    const size_t nm = qb.size();
+    std::vector<int64_t> cut;
+    for (size_t j = 0; j < nm; j++) {
+        const int64_t b = std::max(qb[j], lo), e = std::min(qe[j], hi);
+        if (e > b) {
+            cut.push_back(b);
+            cut.push_back(e);
+        }
+    }
+    std::sort(cut.begin(), cut.end());
+    cut.erase(std::unique(cut.begin(), cut.end()), cut.end());
+    int64_t union_rows = 0;
+    std::vector<BatchSeg> segs;
+    for (size_t k = 0; k + 1 < cut.size(); k++) {
+        unsigned long long m = 0;
+        for (size_t j = 0; j < nm; j++)
+            if (qb[j] <= cut[k] && qe[j] >= cut[k + 1] && qe[j] > qb[j]) m |= 1ull << j;
+        if (!m) continue;
+        if (!segs.empty() && segs.back().mask == m && segs.back().r1 == cut[k]) {
+            segs.back().r1 = cut[k + 1];
+        } else {
+            segs.push_back(BatchSeg{cut[k], cut[k + 1], m, 0});
+        }
+        union_rows += cut[k + 1] - cut[k];
+    }
+    const int64_t group = 256LL * opt_batch_rows;
+    int64_t chunk = std::min<int64_t>(opt_batch_chunk_rows, union_rows / std::max(1, opt_batch_wg));
+    chunk = std::max<int64_t>(group, (chunk + group - 1) / group * group);
+    int64_t nchunks = 0;
+    for (auto& g : segs) {
+        g.chunk0 = nchunks;
+        nchunks += (g.r1 - g.r0 + chunk - 1) / chunk;
+    }
+    B.nseg = (int)segs.size();
+    for (size_t k = 0; k < segs.size(); k++) B.seg[k] = segs[k];
+    B.nchunks = nchunks;
+    B.chunk_rows = chunk;
+}
+
 int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::vector<int>& members)
 {
     PassSlot& S = slots[slot];
@@ -1372,11 +1446,13 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
         narrow_rows(t, s, &qb[j], &qe[j]);
     }
     // border queues of the hybrid members: one region each, 1/8 of its rows
-    // (at least 65536); members beyond 2^28 queued points in all test inline
+    // (at least 65536); members beyond 2^28 queued points in all test inline.
+    // Passes over a host table test inline (a queue per streamed chunk would
+    // need its own exact pass).
     S.qcap.assign(members.size(), 0);
     S.qrows.assign(members.size(), 0);
     S.qcanvas.assign(members.size(), nullptr);
-    if (opt_queue) {
+    if (opt_queue && !t->host) {
         int64_t total = 0;
         for (size_t j = 0; j < members.size(); j++) {
             if (B.mode[j] != PT_LABEL_HYBRID) continue;
@@ -1406,44 +1482,6 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
             off += S.qcap[j];
         }
     }
-    // segments: elementary intervals of the row ranges with a constant query set
-    std::vector<int64_t> cut;
-    for (size_t j = 0; j < members.size(); j++)
-        if (qe[j] > qb[j]) {
-            cut.push_back(qb[j]);
-            cut.push_back(qe[j]);
-        }
-    std::sort(cut.begin(), cut.end());
-    cut.erase(std::unique(cut.begin(), cut.end()), cut.end());
-    int64_t union_rows = 0;
-    std::vector<BatchSeg> segs;
-    for (size_t k = 0; k + 1 < cut.size(); k++) {
-        unsigned long long m = 0;
-        for (size_t j = 0; j < members.size(); j++)
-            if (qb[j] <= cut[k] && qe[j] >= cut[k + 1] && qe[j] > qb[j]) m |= 1ull << j;
-        if (!m) continue;
-        if (!segs.empty() && segs.back().mask == m && segs.back().r1 == cut[k]) {
-            segs.back().r1 = cut[k + 1];
-        } else {
-            segs.push_back(BatchSeg{cut[k], cut[k + 1], m, 0});
-        }
-        union_rows += cut[k + 1] - cut[k];
-    }
-    // chunks: opt_batch_chunk_rows rows (whole row groups), fewer when the
-    // union is too small to give every workgroup one
-    const int rows = opt_batch_rows;
-    const int64_t group = 256LL * rows;
-    int64_t chunk = std::min<int64_t>(opt_batch_chunk_rows, union_rows / std::max(1, opt_batch_wg));
-    chunk = std::max<int64_t>(group, (chunk + group - 1) / group * group);
-    int64_t nchunks = 0;
-    for (auto& g : segs) {
-        g.chunk0 = nchunks;
-        nchunks += (g.r1 - g.r0 + chunk - 1) / chunk;
-    }
-    B.nseg = (int)segs.size();
-    for (size_t k = 0; k < segs.size(); k++) B.seg[k] = segs[k];
-    B.nchunks = nchunks;
-    B.chunk_rows = chunk;
 
     // LDS sums: direct for small collections, hashed for the others; sized
     // for the largest need among the members
@@ -1451,41 +1489,47 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
     B.lds_cap = cap;
     size_t lds = 0;
     for (int i : members) lds = std::max(lds, acc_lds_bytes(qs[i].P->ps.np, cap, true));
-    const void* xy = t->xy;
+    const int rows = opt_batch_rows;
     const int nwg = opt_batch_wg;
     const BatchBlock* dblock = S.d_block.as<BatchBlock>();
-    if (opt_graph) {
-        // the pass as one captured HIP graph: block upload, result clear and
-        // the fused kernel; everything that varies lives in the block
-        if (!S.graph || S.graph_key[0] != xy || S.graph_key[1] != dblock || S.graph_rows != rows ||
-            S.graph_wg != nwg || S.graph_nrows != t->n || S.graph_lds != lds) {
-            if (S.graph) (void)hipGraphExecDestroy(S.graph);
-            S.graph = nullptr;
-            hipGraph_t g = nullptr;
-            hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
-            if (e != hipSuccess) return hip_fail(e, "graph capture");
-            (void)hipMemcpyAsync(S.d_block.p, S.h_block, sizeof(BatchBlock), hipMemcpyHostToDevice, stream);
-            launch_batch(stream, dblock, counter, rows, nwg, lds);
-            e = hipStreamEndCapture(stream, &g);
-            if (e != hipSuccess) return hip_fail(e, "graph capture");
-            e = hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) {
-                S.graph = nullptr;
-                return hip_fail(e, "graph instantiate");
-            }
-            S.graph_key[0] = xy;
-            S.graph_key[1] = dblock;
-            S.graph_rows = rows;
-            S.graph_wg = nwg;
-            S.graph_nrows = t->n;
-            S.graph_lds = lds;
-        }
-        if ((rc = launch("batch", [&] { (void)hipGraphLaunch(S.graph, stream); }))) return rc;
+    if (t->host) {
+        if ((rc = batch_stream(S, t, qb, qe, counter, lds))) return rc;
     } else {
-        hipError_t e = hipMemcpyAsync(S.d_block.p, S.h_block, sizeof(BatchBlock), hipMemcpyHostToDevice, stream);
-        if (e != hipSuccess) return hip_fail(e, "batch block");
-        if ((rc = launch("batch", [&] { launch_batch(stream, dblock, counter, rows, nwg, lds); }))) return rc;
+        batch_segments(B, qb, qe, 0, t->n);
+        const void* xy = t->xy;
+        if (opt_graph) {
+            // the pass as one captured HIP graph: block upload, result clear and
+            // the fused kernel; everything that varies lives in the block
+            if (!S.graph || S.graph_key[0] != xy || S.graph_key[1] != dblock || S.graph_rows != rows ||
+                S.graph_wg != nwg || S.graph_nrows != t->n || S.graph_lds != lds) {
+                if (S.graph) (void)hipGraphExecDestroy(S.graph);
+                S.graph = nullptr;
+                hipGraph_t g = nullptr;
+                hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+                if (e != hipSuccess) return hip_fail(e, "graph capture");
+                (void)hipMemcpyAsync(S.d_block.p, S.h_block, sizeof(BatchBlock), hipMemcpyHostToDevice, stream);
+                launch_batch(stream, dblock, counter, rows, nwg, lds);
+                e = hipStreamEndCapture(stream, &g);
+                if (e != hipSuccess) return hip_fail(e, "graph capture");
+                e = hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (e != hipSuccess) {
+                    S.graph = nullptr;
+                    return hip_fail(e, "graph instantiate");
+                }
+                S.graph_key[0] = xy;
+                S.graph_key[1] = dblock;
+                S.graph_rows = rows;
+                S.graph_wg = nwg;
+                S.graph_nrows = t->n;
+                S.graph_lds = lds;
+            }
+            if ((rc = launch("batch", [&] { (void)hipGraphLaunch(S.graph, stream); }))) return rc;
+        } else {
+            hipError_t e = hipMemcpyAsync(S.d_block.p, S.h_block, sizeof(BatchBlock), hipMemcpyHostToDevice, stream);
+            if (e != hipSuccess) return hip_fail(e, "batch block");
+            if ((rc = launch("batch", [&] { launch_batch(stream, dblock, counter, rows, nwg, lds); }))) return rc;
+        }
     }
     if ((rc = pinned((void**)&S.h_res, &S.h_res_cap, ((size_t)words + kBatchCounters) * 8, "pinned batch results")))
         return rc;
@@ -1499,6 +1543,67 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
     }
     (void)hipEventRecord(S.done, stream);
     S.busy = true;
+    return RJ_OK;
+}
+
+// A fused pass over a host table: the union of the members' rows streams
+// through the device chunk buffers once for all members (the columns any
+// member reads), and the fused kernel runs over each chunk with the members'
+// arguments rebased onto it; results accumulate across chunks (cleared by the
+// first chunk's launch only).
+int rj_ctx::batch_stream(PassSlot& S, rj_table* t, const std::vector<int64_t>& qb, const std::vector<int64_t>& qe,
+                         uint32_t* counter, size_t lds)
+{
+    BatchBlock& B = *S.h_block;
+    const int nm = B.nq;
+    int64_t lo = t->n, hi = 0;
+    bool need[RJ_MAX_COLUMNS + 1] = {true};
+    for (int j = 0; j < nm; j++) {
+        if (qe[j] > qb[j]) {
+            lo = std::min(lo, qb[j]);
+            hi = std::max(hi, qe[j]);
+        }
+        for (int k = 0; k < B.q[j].q.nslots; k++) need[B.q[j].q.slot_col[k] + 1] = true;
+    }
+    if (hi <= lo) {
+        lo = hi = 0;
+    }
+    StreamLayout L;
+    int rc = stream_setup(t, need, hi - lo, &L);
+    if (rc) return rc;
+    hipError_t e = hipSuccess;
+    if (!S.blk) e = hipEventCreateWithFlags(&S.blk, hipEventDisableTiming);
+    if (e != hipSuccess) return hip_fail(e, "stream events");
+    std::vector<PointArgs> base(B.q, B.q + nm);
+    const int64_t words = B.res_words;
+    const BatchBlock* dblock = S.d_block.as<BatchBlock>();
+    int k = 0;
+    bool first = true;
+    for (int64_t c0 = lo - lo % 256; first || c0 < hi; c0 += L.chunk, k ^= 1) {
+        const int64_t c1 = std::min<int64_t>(c0 + L.chunk, hi);
+        if (c1 > c0 && (rc = stream_copy(t, L, k, c0, c1 - c0))) return rc;
+        // the previous chunk's block has reached the device
+        if (!first && (e = hipEventSynchronize(S.blk)) != hipSuccess) return hip_fail(e, "batch block");
+        for (int j = 0; j < nm; j++) {
+            B.q[j] = base[j];
+            stream_args(t, L, k, c0, &B.q[j]);
+            B.q[j].begin = c0;
+            B.q[j].end = c1;
+        }
+        batch_segments(B, qb, qe, c0, c1);
+        B.res_words = first ? words : 0;
+        e = hipMemcpyAsync(S.d_block.p, S.h_block, sizeof(BatchBlock), hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipEventRecord(S.blk, stream);
+        if (e != hipSuccess) return hip_fail(e, "batch block");
+        if ((rc = launch("batch", [&] { launch_batch(stream, dblock, counter, opt_batch_rows, opt_batch_wg, lds); })))
+            return rc;
+        if ((e = hipEventRecord(ev_pass[k], stream)) != hipSuccess) return hip_fail(e, "streamed chunk");
+        stream_passes++;
+        first = false;
+        if (c1 >= hi) break;
+    }
+    // the host block must not change while its last upload is pending
+    if ((e = hipEventSynchronize(S.blk)) != hipSuccess) return hip_fail(e, "batch block");
     return RJ_OK;
 }
 

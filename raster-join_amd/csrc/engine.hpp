@@ -213,6 +213,7 @@ struct rj_ctx {
         int64_t graph_nrows = 0;
         size_t graph_lds = 0;
         hipEvent_t done = nullptr;
+        hipEvent_t blk = nullptr;  // host-table passes: the last block upload
         std::vector<int> members;  // query indices of the pass
         std::vector<int64_t> res_off;
         std::vector<uint32_t> qcap;  // border-queue capacity per member (0: inline test)
@@ -300,6 +301,17 @@ struct rj_ctx {
     int ensure_canvas_buffers(const rj::CanvasPlan& plan, bool sums);
     // One point pass over rows [rb, re) of t (streamed in chunks for a host table).
     int point_pass(const char* name, int mode, rj::PointArgs pa, rj_table* t, int64_t rb, int64_t re);
+    // streaming of host tables through the two device chunk buffers
+    struct StreamLayout {
+        int64_t chunk = 0;  // rows per chunk
+        int nparts = 0;
+        int part_of[RJ_MAX_COLUMNS + 1];  // table part (0: x,y; c + 1: column c) of each chunk part
+        size_t off[RJ_MAX_COLUMNS + 1], width[RJ_MAX_COLUMNS + 1];
+        size_t total = 0;
+    };
+    int stream_setup(rj_table* t, const bool* need, int64_t rows, StreamLayout* L);
+    int stream_copy(rj_table* t, const StreamLayout& L, int k, int64_t r, int64_t n);
+    void stream_args(const rj_table* t, const StreamLayout& L, int k, int64_t r0, rj::PointArgs* a) const;
     int64_t estimate_bytes(const rj::QuerySpec& s) const;
 
     // the single-query path (plain kernels); out receives the results
@@ -313,6 +325,10 @@ struct rj_ctx {
     int batch_prepare(rj_table* t, const rj::QuerySpec* qs, int nq, std::vector<int>* fused,
                       std::vector<int>* alone, std::vector<int>* status);
     int batch_launch(int slot, rj_table* t, const rj::QuerySpec* qs, const std::vector<int>& members);
+    void batch_segments(rj::BatchBlock& B, const std::vector<int64_t>& qb, const std::vector<int64_t>& qe, int64_t lo,
+                        int64_t hi) const;
+    int batch_stream(PassSlot& S, rj_table* t, const std::vector<int64_t>& qb, const std::vector<int64_t>& qe,
+                     uint32_t* counter, size_t lds);
     int batch_finish(int slot, rj_table* t, const rj::QuerySpec* qs, rj::QueryOut* outs);
     // every query of the batch: fused passes, then the queries that run alone;
     // status[i] receives each query's own code

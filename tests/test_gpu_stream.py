@@ -126,6 +126,7 @@ def test_host_table_batch_and_oracle(handler, data):
         h.set_option(rj.OPT_STREAM_ROWS, 0)
     for a, b in zip(single, batch):
         assert b.status == 0 and same(a, b)
+        assert b.path == rj.PATH_FUSED  # one stream of the table for the whole batch
     # hybrid avg (CPUJoin semantics) against the oracle's exact join
     g = O.Grid(xy, off, 256)
     exp = O.join_exact(g, O.PointSet(pts, cols, COLS_TYPES, cons, agg_type=rj.AVG, agg_col=1))
@@ -158,3 +159,41 @@ def test_oversized_points_stream_in_passes():
     for fn, (a, b) in res.items():
         for x, y in zip(a, b):
             assert np.array_equal(x, y), fn
+
+
+@pytest.mark.parametrize("chunk", [0, 200_000])
+def test_host_table_fused_batch_mix(handler, data, chunk):
+    """A mixed batch (raster, hybrid, index; count and avg; time windows and a
+    filter; four collections) over a time-bucketed host table streamed in
+    chunks: every query equals the same query on the resident table."""
+    h = handler
+    pts, cols = data
+    dev = rj.Table(h, pts, cols, COLS_TYPES, time_col=0, time0=T0, bucket_seconds=7 * 86400)
+    host = rj.Table(h, pts, cols, COLS_TYPES, time_col=0, time0=T0, bucket_seconds=7 * 86400, host=True)
+    sets = [rj.PolySet(h, *polys(k)) for k in ("voronoi260", "tracts400", "voronoi20", "stars")]
+    rng = np.random.default_rng(9)
+    qs, nps = [], []
+    for i in range(24):
+        ps = sets[i % len(sets)]
+        length = int(np.exp(rng.uniform(np.log(86400), np.log(YEAR))))
+        lo = T0 + int(rng.uniform(0, YEAR - length))
+        cons = [(0, rj.GTE, lo), (0, rj.LTE, lo + length)]
+        if i % 5 == 0:
+            cons.append((2, rj.LTE, 3))
+        fn = (rj.RASTER, rj.HYBRID, rj.INDEX)[i % 3]
+        agg = rj.AVG if i % 2 else rj.COUNT
+        qs.append(rj.make_query(ps, fn, agg, 1 if agg == rj.AVG else -1, cons, eps=(10.0, 50.0)[i % 2],
+                                index_res=256))
+        nps.append(ps.np)
+    h.set_option(rj.OPT_STREAM_ROWS, chunk)
+    try:
+        batch = h.execute_batch(host, qs, nps)
+    finally:
+        h.set_option(rj.OPT_STREAM_ROWS, 0)
+    fused = 0
+    for i, (q, k, b) in enumerate(zip(qs, nps, batch)):
+        assert b.status == 0, i
+        assert same(h.execute_query(dev, q, k), b), i
+        fused += b.path == rj.PATH_FUSED
+    # raster queries over the overlapping stars run alone (canvas path)
+    assert fused >= 24 - 2
