@@ -239,6 +239,48 @@ def configs_section(h, pts, cols, args):
     return out
 
 
+def stream_section(h, pts, cols, lib, args):
+    """Out-of-core: a host-resident table (day buckets) streamed through the
+    device per pass. One raster count over every row, then the 64-query batch
+    one query at a time (each query streams its own rows) and fused (the
+    union of the windows streams once)."""
+    n = min(args.stream_rows, len(pts) // 2)
+    t0 = time.perf_counter()
+    ht = rj.Table(h, pts[:2 * n], tuple(c[:n] for c in cols), TYPES, time_col=0, time0=T0,
+                  bucket_seconds=86400, host=True)
+    load_s = time.perf_counter() - t0
+    ps = lib["v260"]
+    q = rj.make_query(ps, rj.RASTER, rj.COUNT, -1, [], eps=10.0)
+    h.execute_query(ht, q, ps.np)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        h.execute_query(ht, q, ps.np)
+        ts.append(time.perf_counter() - t0)
+    one = min(ts)
+    qs, nps = batch_queries(lib, 64, seed=41, nrows=n)
+    h.execute_batch(ht, qs, nps)
+    t0 = time.perf_counter()
+    single = [h.execute_query(ht, qq, k) for qq, k in zip(qs, nps)]
+    t_single = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    fused = h.execute_batch(ht, qs, nps)
+    t_fused = time.perf_counter() - t0
+    same = all(np.array_equal(a.counts, b.counts) and np.array_equal(a.hi, b.hi) and np.array_equal(a.lo, b.lo)
+               for a, b in zip(single, fused))
+    ht.close()
+    return {"workload": f"host-resident table of {n} rows (day buckets), streamed through the device per pass",
+            "table_load_s": round(load_s, 2),
+            "raster_count_all_rows_ms": round(one * 1e3, 1),
+            "raster_count_G_rows_per_s": round(n / one / 1e9, 2),
+            "h2d_GB_per_s": round(n * 8 / one / 1e9, 1),
+            "batch64_one_at_a_time_ms": round(t_single * 1e3, 1),
+            "batch64_fused_ms": round(t_fused * 1e3, 1),
+            "batch64_speedup": round(t_single / t_fused, 2),
+            "fused": sum(r.path == rj.PATH_FUSED for r in fused),
+            "identical_results": bool(same)}
+
+
 POLY_LIB = {  # name: (sites, seed, spacing, jitter)
     "v10": (10, 101, (400.0, 400.0), 30.0),
     "v40": (40, 102, (150.0, 150.0), 30.0),
@@ -447,8 +489,9 @@ def main():
                     help="queries of the plain one-at-a-time serving baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--headline-only", action="store_true")
-    ap.add_argument("--sections", default="headline,count,configs,batch,serving",
-                    help="comma list of headline, count, configs, batch, serving")
+    ap.add_argument("--sections", default="headline,count,configs,batch,serving,stream",
+                    help="comma list of headline, count, configs, batch, serving, stream")
+    ap.add_argument("--stream-rows", type=int, default=500_000_000, help="rows of the host-resident table")
     ap.add_argument("--calib", action="store_true",
                     help="first run a plain 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -469,7 +512,7 @@ def main():
     sections = set(args.sections.split(","))
     if args.headline_only:
         sections = {"headline"}
-    big = max(args.big, 0) if sections & {"count", "configs", "batch", "serving"} else 0
+    big = max(args.big, 0) if sections & {"count", "configs", "batch", "serving", "stream"} else 0
     nrows = max(big, args.points)
     cp = rj.CONFIG_POLYS["voronoi260"]
     t0 = time.perf_counter()
@@ -509,6 +552,7 @@ def main():
     if big and "configs" in sections:
         out.update(configs_section(h, pts, (tm, fare, pas), args))
         log("configs:", {k: out[k] for k in ("config1_index_1M", "config3_hybrid_500M")})
+    lib = poly_library(h, list(POLY_LIB)) if big and sections & {"batch", "serving", "stream"} else {}
     if big and sections & {"batch", "serving"}:
         t0 = time.perf_counter()
         table = rj.Table(h, pts[:2 * big], (tm[:big], fare[:big], pas[:big]), TYPES, time_col=0,
@@ -516,7 +560,6 @@ def main():
         h.sync()
         h.bind_table(table)
         log(f"resident {big}-row table (day buckets, Morton order) in {time.perf_counter() - t0:.1f} s")
-        lib = poly_library(h, list(POLY_LIB))
         if "batch" in sections:
             out["batch"] = batch_section(h, table, lib, args)
             log("batch:", out["batch"])
@@ -532,9 +575,12 @@ def main():
         out["p99_ms"] = top["p99_ms"]
         out["saturated_qps"] = srv["saturated"]["served_qps"]
     if big and sections & {"batch", "serving"}:
-        for ps in lib.values():
-            ps.close()
         table.close()
+    if big and "stream" in sections:
+        out["out_of_core"] = stream_section(h, pts, (tm, fare, pas), lib, args)
+        log("out of core:", out["out_of_core"])
+    for ps in lib.values():
+        ps.close()
     if not args.no_cpu and "headline" in sections:
         out["cpu_baseline"] = cpu_baseline(xy, off, pts[:2 * n], fare[:n])
     h.close()
