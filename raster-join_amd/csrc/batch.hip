@@ -5,7 +5,9 @@
 // set of reading queries is constant (a 64-bit mask each) and the segments
 // into chunks of consecutive rows. A fixed number of persistent workgroups
 // take chunks from a counter; for a chunk, a workgroup runs every query of the
-// segment's mask over the chunk's rows, one query after the other, with the
+// segment's mask (of its kernel's class: label-map raster, hybrid or index
+// queries run in separate kernels, each compiled for its own modes only) over
+// the chunk's rows, one query after the other, with the
 // single-query loop (point_rows): its filters, its key (label map, label map +
 // exact test on border pixels, or the CPUJoin grid + exact test) and its
 // per-polygon sums. A chunk is small (a few thousand rows of a spatially
@@ -39,9 +41,10 @@ __device__ __forceinline__ void query_rows(const PointArgs& a, int64_t r0, int64
     acc_flush(A, AVG);
 }
 
-template <int R>
-__global__ __launch_bounds__(kBlock) void k_batch(const BatchBlock* __restrict__ B, uint32_t* __restrict__ next)
+template <int R, int CLS>
+__global__ __launch_bounds__(kBlock) void k_batch(const BatchBlock* __restrict__ B, uint32_t* __restrict__ counters)
 {
+    uint32_t* next = counters + batch_chunk_counter(CLS);
     ConstBlock* CB = (ConstBlock*)B;
     __shared__ int64_t s_chunk0[kBatchSegs];
     __shared__ uint32_t s_take;
@@ -69,23 +72,26 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchBlock* __restrict__
         const int64_t r0 = S.r0 + ((int64_t)c - S.chunk0) * chunk_rows;
         const int64_t r1 = min(S.r1, r0 + chunk_rows);
         // uniform across the wave: the query arguments are read with scalar loads
-        unsigned long long m =
-            ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(S.mask >> 32)) << 32) |
-            (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)S.mask);
+        const unsigned long long sm = S.mask & CB->cls[CLS];
+        unsigned long long m = ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(sm >> 32)) << 32) |
+                               (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)sm);
         while (m) {
             const int q = __ffsll((long long)m) - 1;
             m &= m - 1;
             const PointArgs& a = *(const PointArgs*)(&CB->q[q]);
             const bool avg = a.q.agg_type != RJ_AGG_COUNT;
-            switch (CB->mode[q] * 2 + (avg ? 1 : 0)) {
-            case PT_LABEL * 2: query_rows<PT_LABEL, false, R>(a, r0, r1, sacc, lds_cap); break;
-            case PT_LABEL * 2 + 1: query_rows<PT_LABEL, true, R>(a, r0, r1, sacc, lds_cap); break;
-            case PT_LABEL_HYBRID * 2: query_rows<PT_LABEL_HYBRID, false, R>(a, r0, r1, sacc, lds_cap); break;
-            case PT_LABEL_HYBRID * 2 + 1: query_rows<PT_LABEL_HYBRID, true, R>(a, r0, r1, sacc, lds_cap); break;
-            case PT_HYBRID_QUEUE * 2: query_rows<PT_HYBRID_QUEUE, false, R>(a, r0, r1, sacc, lds_cap); break;
-            case PT_HYBRID_QUEUE * 2 + 1: query_rows<PT_HYBRID_QUEUE, true, R>(a, r0, r1, sacc, lds_cap); break;
-            case PT_INDEX * 2: query_rows<PT_INDEX, false, R>(a, r0, r1, sacc, lds_cap); break;
-            default: query_rows<PT_INDEX, true, R>(a, r0, r1, sacc, lds_cap); break;
+            if (CLS == BC_LABEL) {
+                if (avg) query_rows<PT_LABEL, true, R>(a, r0, r1, sacc, lds_cap);
+                else query_rows<PT_LABEL, false, R>(a, r0, r1, sacc, lds_cap);
+            } else if (CLS == BC_INDEX) {
+                if (avg) query_rows<PT_INDEX, true, R>(a, r0, r1, sacc, lds_cap);
+                else query_rows<PT_INDEX, false, R>(a, r0, r1, sacc, lds_cap);
+            } else if (CLS == BC_HYBRID_INLINE) {
+                if (avg) query_rows<PT_LABEL_HYBRID, true, R>(a, r0, r1, sacc, lds_cap);
+                else query_rows<PT_LABEL_HYBRID, false, R>(a, r0, r1, sacc, lds_cap);
+            } else {
+                if (avg) query_rows<PT_HYBRID_QUEUE, true, R>(a, r0, r1, sacc, lds_cap);
+                else query_rows<PT_HYBRID_QUEUE, false, R>(a, r0, r1, sacc, lds_cap);
             }
         }
     }
@@ -169,14 +175,27 @@ __global__ void k_batch_clear(const BatchBlock* __restrict__ B, uint32_t* __rest
 
 }  // namespace
 
-void launch_batch(hipStream_t s, const BatchBlock* dev_block, uint32_t* counters, int rows, int nwg, size_t lds)
+template <int CLS>
+static void launch_class(hipStream_t s, const BatchBlock* dev_block, uint32_t* counters, int rows, int nwg,
+                         size_t lds)
+{
+    if (rows == 4)
+        hipLaunchKernelGGL((k_batch<4, CLS>), dim3(nwg), dim3(kBlock), lds, s, dev_block, counters);
+    else
+        hipLaunchKernelGGL((k_batch<8, CLS>), dim3(nwg), dim3(kBlock), lds, s, dev_block, counters);
+}
+
+void launch_batch(hipStream_t s, const BatchBlock* dev_block, uint32_t* counters, const int* rows, int nwg,
+                  size_t lds, unsigned classes)
 {
     hipLaunchKernelGGL(k_batch_clear, dim3(256), dim3(kBlock), 0, s, dev_block, counters);
-    if (rows == 4)
-        hipLaunchKernelGGL(k_batch<4>, dim3(nwg), dim3(kBlock), lds, s, dev_block, counters);
-    else
-        hipLaunchKernelGGL(k_batch<8>, dim3(nwg), dim3(kBlock), lds, s, dev_block, counters);
-    hipLaunchKernelGGL(k_exact_batch, dim3(nwg), dim3(kBlock), lds, s, dev_block, counters);
+    if (classes & (1u << BC_LABEL)) launch_class<BC_LABEL>(s, dev_block, counters, rows[BC_LABEL], nwg, lds);
+    if (classes & (1u << BC_HYBRID)) launch_class<BC_HYBRID>(s, dev_block, counters, rows[BC_HYBRID], nwg, lds);
+    if (classes & (1u << BC_INDEX)) launch_class<BC_INDEX>(s, dev_block, counters, rows[BC_INDEX], nwg, lds);
+    if (classes & (1u << BC_HYBRID_INLINE))
+        launch_class<BC_HYBRID_INLINE>(s, dev_block, counters, rows[BC_HYBRID_INLINE], nwg, lds);
+    if (classes & (1u << BC_HYBRID))
+        hipLaunchKernelGGL(k_exact_batch, dim3(nwg), dim3(kBlock), lds, s, dev_block, counters);
 }
 
 }  // namespace rj

@@ -1350,7 +1350,7 @@ void rj_ctx::batch_segments(BatchBlock& B, const std::vector<int64_t>& qb, const
         }
         union_rows += cut[k + 1] - cut[k];
     }
-    const int64_t group = 256LL * opt_batch_rows;
+    const int64_t group = 256LL * 8;  // whole row groups at 4 or 8 rows per thread
     int64_t chunk = std::min<int64_t>(opt_batch_chunk_rows, union_rows / std::max(1, opt_batch_wg));
     chunk = std::max<int64_t>(group, (chunk + group - 1) / group * group);
     int64_t nchunks = 0;
@@ -1483,6 +1483,16 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
         }
     }
 
+    // one persistent kernel per class of members
+    unsigned classes = 0;
+    for (size_t j = 0; j < members.size(); j++) {
+        const int c = B.mode[j] == PT_LABEL    ? BC_LABEL
+                      : B.mode[j] == PT_INDEX  ? BC_INDEX
+                      : B.mode[j] == PT_HYBRID_QUEUE ? BC_HYBRID
+                                                     : BC_HYBRID_INLINE;
+        B.cls[c] |= 1ull << j;
+        classes |= 1u << c;
+    }
     // LDS sums: direct for small collections, hashed for the others; sized
     // for the largest need among the members
     const int cap = lds_direct_cap(true);
@@ -1490,10 +1500,11 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
     size_t lds = 0;
     for (int i : members) lds = std::max(lds, acc_lds_bytes(qs[i].P->ps.np, cap, true));
     const int rows = opt_batch_rows;
+    const int crow[BC_COUNT] = {rows, opt_batch_hybrid_rows, rows, rows};
     const int nwg = opt_batch_wg;
     const BatchBlock* dblock = S.d_block.as<BatchBlock>();
     if (t->host) {
-        if ((rc = batch_stream(S, t, qb, qe, counter, lds))) return rc;
+        if ((rc = batch_stream(S, t, qb, qe, counter, lds, classes))) return rc;
     } else {
         batch_segments(B, qb, qe, 0, t->n);
         const void* xy = t->xy;
@@ -1501,14 +1512,14 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
             // the pass as one captured HIP graph: block upload, result clear and
             // the fused kernel; everything that varies lives in the block
             if (!S.graph || S.graph_key[0] != xy || S.graph_key[1] != dblock || S.graph_rows != rows ||
-                S.graph_wg != nwg || S.graph_nrows != t->n || S.graph_lds != lds) {
+                S.graph_wg != nwg || S.graph_nrows != t->n || S.graph_lds != lds || S.graph_classes != classes) {
                 if (S.graph) (void)hipGraphExecDestroy(S.graph);
                 S.graph = nullptr;
                 hipGraph_t g = nullptr;
                 hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
                 if (e != hipSuccess) return hip_fail(e, "graph capture");
                 (void)hipMemcpyAsync(S.d_block.p, S.h_block, sizeof(BatchBlock), hipMemcpyHostToDevice, stream);
-                launch_batch(stream, dblock, counter, rows, nwg, lds);
+                launch_batch(stream, dblock, counter, crow, nwg, lds, classes);
                 e = hipStreamEndCapture(stream, &g);
                 if (e != hipSuccess) return hip_fail(e, "graph capture");
                 e = hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0);
@@ -1523,12 +1534,14 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
                 S.graph_wg = nwg;
                 S.graph_nrows = t->n;
                 S.graph_lds = lds;
+                S.graph_classes = classes;
             }
             if ((rc = launch("batch", [&] { (void)hipGraphLaunch(S.graph, stream); }))) return rc;
         } else {
             hipError_t e = hipMemcpyAsync(S.d_block.p, S.h_block, sizeof(BatchBlock), hipMemcpyHostToDevice, stream);
             if (e != hipSuccess) return hip_fail(e, "batch block");
-            if ((rc = launch("batch", [&] { launch_batch(stream, dblock, counter, rows, nwg, lds); }))) return rc;
+            if ((rc = launch("batch", [&] { launch_batch(stream, dblock, counter, crow, nwg, lds, classes); })))
+                return rc;
         }
     }
     if ((rc = pinned((void**)&S.h_res, &S.h_res_cap, ((size_t)words + kBatchCounters) * 8, "pinned batch results")))
@@ -1552,7 +1565,7 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
 // arguments rebased onto it; results accumulate across chunks (cleared by the
 // first chunk's launch only).
 int rj_ctx::batch_stream(PassSlot& S, rj_table* t, const std::vector<int64_t>& qb, const std::vector<int64_t>& qe,
-                         uint32_t* counter, size_t lds)
+                         uint32_t* counter, size_t lds, unsigned classes)
 {
     BatchBlock& B = *S.h_block;
     const int nm = B.nq;
@@ -1595,7 +1608,8 @@ int rj_ctx::batch_stream(PassSlot& S, rj_table* t, const std::vector<int64_t>& q
         e = hipMemcpyAsync(S.d_block.p, S.h_block, sizeof(BatchBlock), hipMemcpyHostToDevice, stream);
         if (e == hipSuccess) e = hipEventRecord(S.blk, stream);
         if (e != hipSuccess) return hip_fail(e, "batch block");
-        if ((rc = launch("batch", [&] { launch_batch(stream, dblock, counter, opt_batch_rows, opt_batch_wg, lds); })))
+        const int crow[BC_COUNT] = {opt_batch_rows, opt_batch_hybrid_rows, opt_batch_rows, opt_batch_rows};
+        if ((rc = launch("batch", [&] { launch_batch(stream, dblock, counter, crow, opt_batch_wg, lds, classes); })))
             return rc;
         if ((e = hipEventRecord(ev_pass[k], stream)) != hipSuccess) return hip_fail(e, "streamed chunk");
         stream_passes++;

@@ -170,7 +170,8 @@ struct rj_ctx {
     bool organize = true;
     int opt_cap = 0;                       // workgroups of a single-query pass (0: 4096)
     int opt_tshift = rj::kTileShiftDefault;  // label tile side 2^tshift
-    int opt_batch_rows = 8;                // rows per thread in fused passes
+    int opt_batch_rows = 4;                // rows per thread of the fused label-map and index kernels
+    int opt_batch_hybrid_rows = 4;         // rows per thread of the fused hybrid kernel
     int opt_batch_wg = 2048;               // persistent workgroups of a fused pass
     int opt_batch_chunk_rows = 8192;       // rows per chunk of a fused pass
     bool opt_graph = true;                 // fused passes replay a captured HIP graph
@@ -212,6 +213,7 @@ struct rj_ctx {
         int graph_rows = 0, graph_wg = 0;
         int64_t graph_nrows = 0;
         size_t graph_lds = 0;
+        unsigned graph_classes = 0;
         hipEvent_t done = nullptr;
         hipEvent_t blk = nullptr;  // host-table passes: the last block upload
         std::vector<int> members;  // query indices of the pass
@@ -328,7 +330,7 @@ struct rj_ctx {
     void batch_segments(rj::BatchBlock& B, const std::vector<int64_t>& qb, const std::vector<int64_t>& qe, int64_t lo,
                         int64_t hi) const;
     int batch_stream(PassSlot& S, rj_table* t, const std::vector<int64_t>& qb, const std::vector<int64_t>& qe,
-                     uint32_t* counter, size_t lds);
+                     uint32_t* counter, size_t lds, unsigned classes);
     int batch_finish(int slot, rj_table* t, const rj::QuerySpec* qs, rj::QueryOut* outs);
     // every query of the batch: fused passes, then the queries that run alone;
     // status[i] receives each query's own code

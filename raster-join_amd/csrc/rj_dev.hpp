@@ -139,9 +139,16 @@ struct PointArgs {
 
 constexpr int kBatchMax = 64;    // queries of one fused pass (one bit each in a segment mask)
 constexpr int kBatchSegs = 2 * kBatchMax + 1;
-// Device counters of a pass, after its block: the chunk counter, the exact-tile
-// counter, then the border-queue count of every query.
-constexpr int kBatchCounters = 2 + kBatchMax;
+// Query classes of a fused pass, each run by its own persistent kernel (a
+// kernel compiled for one class needs fewer registers than one for all):
+// label-map raster queries, hybrid queries with a border queue, per-point
+// grid (index) queries, hybrid queries testing border points inline.
+enum BatchClass { BC_LABEL = 0, BC_HYBRID = 1, BC_INDEX = 2, BC_HYBRID_INLINE = 3, BC_COUNT = 4 };
+// Device counters of a pass, after its block: the chunk counter of class
+// BC_LABEL, the exact-tile counter, the border-queue count of every query,
+// then the chunk counters of the other classes.
+constexpr int kBatchCounters = 2 + kBatchMax + (BC_COUNT - 1);
+__host__ __device__ constexpr int batch_chunk_counter(int cls) { return cls == 0 ? 0 : 2 + kBatchMax + cls - 1; }
 constexpr int kExactTile = 4096;  // queued border points per exact-pass work item
 
 // Rows [r0, r1) read by the queries of `mask`; split into chunks of
@@ -165,6 +172,7 @@ struct BatchBlock {
     unsigned long long* res;  // result words of all queries, cleared before the pass
     int64_t res_words;
     BatchSeg seg[kBatchSegs];
+    unsigned long long cls[BC_COUNT];  // members of each class (bit j: query j)
     int32_t mode[kBatchMax];
     PointArgs q[kBatchMax];
 };
@@ -219,7 +227,9 @@ void launch_exact(hipStream_t s, const PointArgs& a);
 // left; rows = rows per thread (4 or 8). The block is read-only on the device.
 // lds: dynamic LDS bytes of the per-polygon sums. Queries in PT_HYBRID_QUEUE
 // mode then get their queued border points tested by a second persistent
-// kernel. counters: kBatchCounters words.
-void launch_batch(hipStream_t s, const BatchBlock* dev_block, uint32_t* counters, int rows, int nwg, size_t lds);
+// kernel. counters: kBatchCounters words. One persistent kernel per class
+// with members (bit c of `classes`); rows[c] = its rows per thread (4 or 8).
+void launch_batch(hipStream_t s, const BatchBlock* dev_block, uint32_t* counters, const int* rows, int nwg,
+                  size_t lds, unsigned classes);
 
 }  // namespace rj
