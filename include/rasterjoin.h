@@ -248,7 +248,10 @@ int rj_canvas_info(rj_ctx* ctx, int32_t* W, int32_t* H);
  *   RJ_OPT_TILE_SHIFT:      label tile side 2^v pixels, v in 3..5 (default 4)
  *   RJ_OPT_PASS_WORKGROUPS: workgroups of a single-query point pass (0: 4096)
  *   RJ_OPT_BATCH_ROWS:      rows per thread of the fused label-map and index kernels, 4 (default) or 8
- *   RJ_OPT_BATCH_HYBRID_ROWS: rows per thread of the fused hybrid kernel, 4 (default) or 8
+ *   RJ_OPT_BATCH_HYBRID_ROWS: rows per thread of the fused hybrid kernel, 4 or 8 (default)
+ *   RJ_OPT_BATCH_SHARE:     1 (default) fused raster queries on the same label map (same
+ *                           collection and accuracy) compute each row's label once per chunk;
+ *                           0 every query computes its own
  *   RJ_OPT_BATCH_WORKGROUPS: persistent workgroups of fused passes (64..65536)
  *   RJ_OPT_BATCH_CHUNK_ROWS: rows per chunk of fused passes (1024 .. 2^22)
  *   RJ_OPT_GRAPH:           1 (default) fused passes replay a captured HIP graph
@@ -263,7 +266,8 @@ int rj_canvas_info(rj_ctx* ctx, int32_t* W, int32_t* H);
 enum { RJ_OPT_RASTER_PATH = 1, RJ_OPT_TIMING = 2, RJ_OPT_HYBRID_DILATION = 3, RJ_OPT_CACHE = 4,
        RJ_OPT_ORGANIZE = 5, RJ_OPT_TILE_SHIFT = 6, RJ_OPT_PASS_WORKGROUPS = 7, RJ_OPT_BATCH_ROWS = 8,
        RJ_OPT_BATCH_WORKGROUPS = 9, RJ_OPT_BATCH_CHUNK_ROWS = 10, RJ_OPT_GRAPH = 11,
-       RJ_OPT_INDEX_PATH = 12, RJ_OPT_EXACT_QUEUE = 13, RJ_OPT_STREAM_ROWS = 14, RJ_OPT_BATCH_HYBRID_ROWS = 15 };
+       RJ_OPT_INDEX_PATH = 12, RJ_OPT_EXACT_QUEUE = 13, RJ_OPT_STREAM_ROWS = 14, RJ_OPT_BATCH_HYBRID_ROWS = 15,
+       RJ_OPT_BATCH_SHARE = 16 };
 int rj_set_option(rj_ctx* ctx, int32_t key, int64_t value);
 
 /* Per-kernel device time of the last execute (timing on): name i, total ms and

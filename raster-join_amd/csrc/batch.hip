@@ -41,6 +41,120 @@ __device__ __forceinline__ void query_rows(const PointArgs& a, int64_t r0, int64
     acc_flush(A, AVG);
 }
 
+// Group keys of one chunk (rows [r0, r1)): every row's label in the group's
+// map as 16-bit keys in LDS: the polygon id, 0xFFFE for a hybrid border
+// point, 0xFFFF for none. No filter applies here: the members apply their own.
+constexpr uint16_t kKeyBorder = 0xFFFE, kKeyNone = 0xFFFF;
+
+template <bool HYB, int R>
+__device__ __forceinline__ void group_keys(const PointArgs& a, int64_t r0, int64_t r1, uint16_t* skey)
+{
+    const int64_t stride = (int64_t)blockDim.x * R;
+    const int64_t start = r0 - r0 % R;
+    float nx[R], ny[R];
+    load_xy<R>(a.xy, start + (int64_t)threadIdx.x * R, r1, a.end, nx, ny);
+    for (int64_t base = start; base < r1; base += stride) {
+        const int64_t i0 = base + (int64_t)threadIdx.x * R;
+        float px[R], py[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            px[r] = nx[r];
+            py[r] = ny[r];
+        }
+        if (i0 >= r1) continue;
+        int pix[R], ixs[R], iys[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            pix[r] = pixel_xy(a.cv, px[r], py[r], ixs[r], iys[r]);
+            if (i0 + r < r0 || i0 + r >= r1) pix[r] = -1;
+        }
+        uint32_t lab[R];
+        row_labels<HYB, R>(a, px, py, pix, ixs, iys, lab, [&] { load_xy<R>(a.xy, i0 + stride, r1, a.end, nx, ny); });
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            if (i0 + r >= r0 && i0 + r < r1)
+                skey[i0 + r - r0] = lab[r] < kLabelMaxId ? (uint16_t)lab[r] : (lab[r] == kLabelBorder ? kKeyBorder : kKeyNone);
+    }
+}
+
+// One member of a group over the chunk: its filters and aggregate column,
+// the keys from LDS.
+template <bool AVG, int R>
+__device__ __forceinline__ void member_rows(const PointArgs& a, int64_t r0, int64_t r1, const uint16_t* skey,
+                                            const Acc& A, Run& run)
+{
+    const DevQuery& q = a.q;
+    const int agg_t = AVG ? a.cols.type[q.slot_col[q.agg_slot < 0 ? 0 : q.agg_slot]] : RJ_ATTR_UINT;
+    const int64_t stride = (int64_t)blockDim.x * R;
+    for (int64_t base = r0 - r0 % R; base < r1; base += stride) {
+        const int64_t i0 = base + (int64_t)threadIdx.x * R;
+        if (i0 >= r1) continue;
+        const bool full = (i0 + R <= a.end);
+        bool ok[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) ok[r] = (i0 + r >= r0) && (i0 + r < r1);
+        uint32_t vraw[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) vraw[r] = 0;
+#pragma unroll
+        for (int s = 0; s < kMaxSlots; s++) {
+            if (s >= q.nslots) break;
+            uint32_t v[R];
+            const int col = q.slot_col[s];
+            load_rows<R>(a.cols, col, i0, full, a.end, v);
+            const int t = a.cols.type[col];
+            for (int k = 0; k < q.ncons; k++) {
+                if (q.con_slot[k] != s) continue;
+                const int op = q.con_op[k];
+                const uint32_t cv = q.con_val[k];
+#pragma unroll
+                for (int r = 0; r < R; r++) ok[r] = ok[r] && cmp_bits(t, op, v[r], cv);
+            }
+            if (AVG && s == q.agg_slot) {
+#pragma unroll
+                for (int r = 0; r < R; r++) vraw[r] = v[r];
+            }
+        }
+        int key[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const uint32_t k16 = ok[r] ? skey[i0 + r - r0] : kKeyNone;
+            key[r] = k16 < 0xFFF0u ? (int)k16 : -1;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int k = key[r];
+            if (k < 0) continue;
+            if (k != run.ck) {
+                if (run.ck >= 0) run_flush<PT_LABEL, AVG>(a, A, run.ck, run.cc, run.ch, run.cl);
+                run.ck = k;
+                run.cc = 0;
+                run.ch = run.cl = 0;
+            }
+            run.cc++;
+            if (AVG) {
+                unsigned long long h, l;
+                agg_bits(agg_t, vraw[r], h, l);
+                run.ch += h;
+                run.cl += l;
+            }
+        }
+    }
+}
+
+template <bool AVG, int R>
+__device__ __forceinline__ void member_query(const PointArgs& a, int64_t r0, int64_t r1, const uint16_t* skey,
+                                             unsigned long long* sacc, int lds_cap)
+{
+    Acc A;
+    __syncthreads();  // the previous query's sums have been flushed
+    acc_init(A, sacc, a.P.np, lds_cap, AVG, a.res);
+    Run run{-1, 0, 0, 0};
+    member_rows<AVG, R>(a, r0, r1, skey, A, run);
+    if (run.ck >= 0) run_flush<PT_LABEL, AVG>(a, A, run.ck, run.cc, run.ch, run.cl);
+    acc_flush(A, AVG);
+}
+
 template <int R, int CLS>
 __global__ __launch_bounds__(kBlock) void k_batch(const BatchBlock* __restrict__ B, uint32_t* __restrict__ counters)
 {
@@ -77,6 +191,27 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchBlock* __restrict__
                                (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)sm);
         while (m) {
             const int q = __ffsll((long long)m) - 1;
+            if (CLS == BC_LABEL) {
+                // the members of q's group in this chunk share its labels
+                const int g = CB->grp[q];
+                const unsigned long long gm = g >= 0 ? (m & CB->gmask[g]) : 0ull;
+                if (__popcll(gm) >= 2) {
+                    m &= ~gm;
+                    uint16_t* skey = reinterpret_cast<uint16_t*>(reinterpret_cast<char*>(sacc) + B->key_off);
+                    __syncthreads();  // the previous group's members have read their keys
+                    group_keys<false, R>(*(const PointArgs*)(&CB->q[q]), r0, r1, skey);
+                    unsigned long long mm = gm;
+                    while (mm) {
+                        const int j = __ffsll((long long)mm) - 1;
+                        mm &= mm - 1;
+                        const PointArgs& aj = *(const PointArgs*)(&CB->q[j]);
+                        const bool avg = aj.q.agg_type != RJ_AGG_COUNT;
+                        if (avg) member_query<true, R>(aj, r0, r1, skey, sacc, lds_cap);
+                        else member_query<false, R>(aj, r0, r1, skey, sacc, lds_cap);
+                    }
+                    continue;
+                }
+            }
             m &= m - 1;
             const PointArgs& a = *(const PointArgs*)(&CB->q[q]);
             const bool avg = a.q.agg_type != RJ_AGG_COUNT;

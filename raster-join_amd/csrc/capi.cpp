@@ -525,6 +525,7 @@ int rj_set_option(rj_ctx* ctx, int32_t key, int64_t value)
         if (value != 4 && value != 8) return ctx->fail(RJ_ERR_ARG, "rows per thread is 4 or 8");
         ctx->opt_batch_rows = (int)value;
         return RJ_OK;
+    case RJ_OPT_BATCH_SHARE: ctx->opt_batch_share = value != 0; return RJ_OK;
     case RJ_OPT_BATCH_HYBRID_ROWS:
         if (value != 4 && value != 8) return ctx->fail(RJ_ERR_ARG, "rows per thread is 4 or 8");
         ctx->opt_batch_hybrid_rows = (int)value;

@@ -1446,7 +1446,7 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
         narrow_rows(t, s, &qb[j], &qe[j]);
     }
     // border queues of the hybrid members: one region each, 1/8 of its rows
-    // (at least 65536); members beyond 2^28 queued points in all test inline.
+    // (at least 65536); members beyond kQueueEntriesMax queued points in all test inline.
     // Passes over a host table test inline (a queue per streamed chunk would
     // need its own exact pass).
     S.qcap.assign(members.size(), 0);
@@ -1463,7 +1463,7 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
             S.qcanvas[j] = C;
             S.qrows[j] = qe[j] - qb[j];
             const int64_t cap = queue_cap(C, qe[j] - qb[j]);
-            if (total + cap > (1LL << 28)) continue;
+            if (total + cap > kQueueEntriesMax) continue;
             S.qcap[j] = (uint32_t)cap;
             total += cap;
         }
@@ -1493,12 +1493,35 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
         B.cls[c] |= 1ull << j;
         classes |= 1u << c;
     }
+    // groups: raster members that read the same 16-bit label map share each
+    // chunk's labels (computed once into LDS)
+    const int64_t chunk_max = (opt_batch_chunk_rows + 2047) / 2048 * 2048;
+    B.ngroups = 0;
+    for (size_t j = 0; j < members.size(); j++) B.grp[j] = -1;
+    if (opt_batch_share && chunk_max <= kGroupChunkMax) {
+        for (size_t j = 0; j < members.size(); j++) {
+            // raster members only: hybrid members spend their time on their own
+            // filters, border queues and exact tests, and measured no gain
+            if (B.mode[j] != PT_LABEL || !B.q[j].label16 || B.grp[j] >= 0) continue;
+            unsigned long long m = 1ull << j;
+            for (size_t i = j + 1; i < members.size(); i++)
+                if (B.mode[i] == B.mode[j] && B.q[i].label == B.q[j].label) m |= 1ull << i;
+            if (__builtin_popcountll(m) < 2) continue;
+            const int g = B.ngroups++;
+            B.gmask[g] = m;
+            for (size_t i = j; i < members.size(); i++)
+                if ((m >> i) & 1ull) B.grp[i] = g;
+        }
+    }
     // LDS sums: direct for small collections, hashed for the others; sized
-    // for the largest need among the members
+    // for the largest need among the members; then the group keys
     const int cap = lds_direct_cap(true);
     B.lds_cap = cap;
     size_t lds = 0;
     for (int i : members) lds = std::max(lds, acc_lds_bytes(qs[i].P->ps.np, cap, true));
+    lds = (lds + 15) / 16 * 16;
+    B.key_off = (int32_t)lds;
+    if (B.ngroups) lds += (size_t)chunk_max * 2;
     const int rows = opt_batch_rows;
     const int crow[BC_COUNT] = {rows, opt_batch_hybrid_rows, rows, rows};
     const int nwg = opt_batch_wg;
@@ -1512,7 +1535,8 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
             // the pass as one captured HIP graph: block upload, result clear and
             // the fused kernel; everything that varies lives in the block
             if (!S.graph || S.graph_key[0] != xy || S.graph_key[1] != dblock || S.graph_rows != rows ||
-                S.graph_wg != nwg || S.graph_nrows != t->n || S.graph_lds != lds || S.graph_classes != classes) {
+                S.graph_wg != nwg || S.graph_nrows != t->n || S.graph_lds != lds || S.graph_classes != classes ||
+                S.graph_hrows != opt_batch_hybrid_rows) {
                 if (S.graph) (void)hipGraphExecDestroy(S.graph);
                 S.graph = nullptr;
                 hipGraph_t g = nullptr;
@@ -1535,6 +1559,7 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
                 S.graph_nrows = t->n;
                 S.graph_lds = lds;
                 S.graph_classes = classes;
+                S.graph_hrows = opt_batch_hybrid_rows;
             }
             if ((rc = launch("batch", [&] { (void)hipGraphLaunch(S.graph, stream); }))) return rc;
         } else {
@@ -1652,12 +1677,34 @@ int rj_ctx::batch_finish(int slot, rj_table* t, const QuerySpec* qs, QueryOut* o
     return RJ_OK;
 }
 
-// Fused queries in passes of at most max_batch queries, in query order.
-static void group_passes(const std::vector<int>& fused, int max_batch, std::vector<std::vector<int>>* passes)
+// Fused queries in passes of at most max_batch queries, in query order; a
+// pass also ends where its hybrid members' border queues would pass
+// kQueueEntriesMax entries in all (members beyond it would test inline, which
+// is slower than another pass).
+void rj_ctx::group_passes(rj_table* t, const QuerySpec* qs, const std::vector<int>& fused, int max_batch,
+                          std::vector<std::vector<int>>* passes)
 {
     passes->clear();
-    for (size_t k = 0; k < fused.size(); k += (size_t)max_batch)
-        passes->emplace_back(fused.begin() + k, fused.begin() + std::min(fused.size(), k + (size_t)max_batch));
+    int64_t entries = 0;
+    for (int i : fused) {
+        const QuerySpec& s = qs[i];
+        int64_t cap = 0;
+        if (s.exec == RJ_HYBRID && opt_queue && !t->host) {
+            CanvasPlan plan;
+            make_plan(s.P->ps.bbox, canvas_eps(s), s.ground_res, &plan);
+            auto ci = s.P->canvases.find(plan.cell);
+            int64_t rb = s.rb, re = s.re;
+            narrow_rows(t, s, &rb, &re);
+            if (ci != s.P->canvases.end()) cap = queue_cap(ci->second.get(), re - rb);
+        }
+        if (passes->empty() || (int)passes->back().size() >= max_batch ||
+            (entries + cap > kQueueEntriesMax && entries > 0)) {
+            passes->emplace_back();
+            entries = 0;
+        }
+        passes->back().push_back(i);
+        entries += cap;
+    }
 }
 
 int rj_ctx::execute_batch(rj_table* t, const QuerySpec* qs, int nq, QueryOut* outs, int* status)
@@ -1668,7 +1715,7 @@ int rj_ctx::execute_batch(rj_table* t, const QuerySpec* qs, int nq, QueryOut* ou
     int rc = batch_prepare(t, qs, nq, &fused, &alone, &st);
     if (rc) return rc;
     std::vector<std::vector<int>> passes;
-    group_passes(fused, kBatchMax, &passes);
+    group_passes(t, qs, fused, kBatchMax, &passes);
     for (size_t p = 0; p < passes.size() && !rc; p++) {
         const int slot = (int)(p & 1);
         rc = batch_finish(slot, t, qs, outs);
@@ -1768,6 +1815,20 @@ int rj_ctx::autotune(rj_table* t, const QuerySpec* qs_in, int nq, const std::vec
     opt_batch_rows = bestc[1];
     opt_batch_wg = bestc[2];
     opt_batch_chunk_rows = bestc[3];
+    // rows per thread of the hybrid kernel, with the rest fixed
+    const int base_hrows = opt_batch_hybrid_rows;
+    for (int hr : {4, 8}) {
+        if (rc || hr == base_hrows) continue;
+        opt_batch_hybrid_rows = hr;
+        double ms = 0;
+        if ((rc = run(&ms))) break;
+        if (!same()) {
+            rc = fail(RJ_ERR_DEVICE, "autotune: results differ between launch shapes");
+            break;
+        }
+        if (ms >= best) opt_batch_hybrid_rows = base_hrows;
+        else best = ms;
+    }
     if (!rc && !auto_res_q.empty()) {
         // grid resolution of queries that leave it to the engine: a
         // resolution is kept only where the results equal those at 256
@@ -1794,6 +1855,7 @@ int rj_ctx::autotune(rj_table* t, const QuerySpec* qs_in, int nq, const std::vec
     timing = saved_timing;
     if (rc) {
         opt_tshift = base_tshift;
+        opt_batch_hybrid_rows = base_hrows;
         opt_batch_rows = base_rows;
         opt_batch_wg = base_wg;
         opt_batch_chunk_rows = base_chunks;

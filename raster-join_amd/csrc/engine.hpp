@@ -171,7 +171,8 @@ struct rj_ctx {
     int opt_cap = 0;                       // workgroups of a single-query pass (0: 4096)
     int opt_tshift = rj::kTileShiftDefault;  // label tile side 2^tshift
     int opt_batch_rows = 4;                // rows per thread of the fused label-map and index kernels
-    int opt_batch_hybrid_rows = 4;         // rows per thread of the fused hybrid kernel
+    int opt_batch_hybrid_rows = 8;         // rows per thread of the fused hybrid kernel
+    bool opt_batch_share = true;           // fused members on one label map share its lookups
     int opt_batch_wg = 2048;               // persistent workgroups of a fused pass
     int opt_batch_chunk_rows = 8192;       // rows per chunk of a fused pass
     bool opt_graph = true;                 // fused passes replay a captured HIP graph
@@ -214,6 +215,7 @@ struct rj_ctx {
         int64_t graph_nrows = 0;
         size_t graph_lds = 0;
         unsigned graph_classes = 0;
+        int graph_hrows = 0;
         hipEvent_t done = nullptr;
         hipEvent_t blk = nullptr;  // host-table passes: the last block upload
         std::vector<int> members;  // query indices of the pass
@@ -327,6 +329,9 @@ struct rj_ctx {
     int batch_prepare(rj_table* t, const rj::QuerySpec* qs, int nq, std::vector<int>* fused,
                       std::vector<int>* alone, std::vector<int>* status);
     int batch_launch(int slot, rj_table* t, const rj::QuerySpec* qs, const std::vector<int>& members);
+    static constexpr int64_t kQueueEntriesMax = 1LL << 28;  // border-queue entries of one fused pass
+    void group_passes(rj_table* t, const rj::QuerySpec* qs, const std::vector<int>& fused, int max_batch,
+                      std::vector<std::vector<int>>* passes);
     void batch_segments(rj::BatchBlock& B, const std::vector<int64_t>& qb, const std::vector<int64_t>& qe, int64_t lo,
                         int64_t hi) const;
     int batch_stream(PassSlot& S, rj_table* t, const std::vector<int64_t>& qb, const std::vector<int64_t>& qe,

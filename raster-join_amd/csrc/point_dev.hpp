@@ -484,6 +484,55 @@ __device__ __forceinline__ void queue_border(const PointArgs& a, const float (&p
     }
 }
 
+// Labels of R rows from their pixels (pix < 0: no label), as point_rows
+// computes them (kept apart from point_rows, whose code generation is tuned
+// on its own): the tile map answers for whole tiles, mixed tiles read the
+// pixel, and a row in the same tile (pixel) as the row before it takes that
+// row's label without a load.
+// HYB: hybrid maps in the halo mode, whose flagged pixels (next to a touched
+// pixel) send points within the float margin of the touched pixel to the
+// exact test (kLabelBorder). `between` runs after the tile loads are issued
+// and before their values are used (a prefetch placed there does not delay
+// them: a wave's vector loads complete in order).
+template <bool HYB, int R, class F>
+__device__ __forceinline__ void row_labels(const PointArgs& a, const float (&px)[R], const float (&py)[R],
+                                           const int (&pix)[R], const int (&ixs)[R], const int (&iys)[R],
+                                           uint32_t (&lab)[R], F&& between)
+{
+    int tix[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) tix[r] = pix[r] < 0 ? -1 : (iys[r] >> a.tshift) * a.tiles_w + (ixs[r] >> a.tshift);
+    uint32_t tl[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        tl[r] = kLabelNone;
+        if (tix[r] >= 0 && (r == 0 || tix[r] != tix[r - 1])) tl[r] = a.tiles[tix[r]];
+    }
+    between();
+#pragma unroll
+    for (int r = 0; r < R; r++) lab[r] = (r == 0 || tix[r] != tix[r - 1]) ? tl[r] : lab[r > 0 ? r - 1 : 0];
+    uint32_t pl[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        pl[r] = 0;
+        if (lab[r] == kLabelMixed && (r == 0 || pix[r] != pix[r - 1]))
+            pl[r] = label_at(a.label, a.label16, (int64_t)iys[r] * a.cv.LW + ixs[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++)
+        if (lab[r] == kLabelMixed) lab[r] = (r == 0 || pix[r] != pix[r - 1]) ? pl[r] : lab[r > 0 ? r - 1 : 0];
+    if (HYB && a.halo) {
+        // only pixels flagged next to a touched pixel take the halo test
+        const uint32_t fb = a.label16 ? kHaloFlag16 : kHaloFlag32;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const bool f = lab[r] == kLabelNoneHalo || (lab[r] < kLabelMaxId && (lab[r] & fb));
+            lab[r] = lab[r] == kLabelNoneHalo ? kLabelNone : (f ? lab[r] & ~fb : lab[r]);
+            if (f && halo_near(a, px[r], py[r], ixs[r], iys[r])) lab[r] = kLabelBorder;
+        }
+    }
+}
+
 // Rows [lo, hi) of one point pass, read in groups of R consecutive rows per
 // thread from `start` (a multiple of R, <= lo). Every row gets one key
 // (polygon id, or pixel for PT_SPLAT) and the lane keeps a running

@@ -173,9 +173,18 @@ struct BatchBlock {
     int64_t res_words;
     BatchSeg seg[kBatchSegs];
     unsigned long long cls[BC_COUNT];  // members of each class (bit j: query j)
+    // members that read the same 16-bit label map form a group: a chunk's
+    // labels are computed once for the group into LDS (key_off: byte offset
+    // of that buffer in the dynamic LDS) and each member reads them there
+    int32_t key_off;
+    int32_t ngroups;
+    int32_t grp[kBatchMax];                // group of each member, -1: none
+    unsigned long long gmask[kBatchMax];   // members of each group
     int32_t mode[kBatchMax];
     PointArgs q[kBatchMax];
 };
+// rows of a chunk whose group keys fit the LDS key buffer
+constexpr int64_t kGroupChunkMax = 16384;
 
 // Per-polygon sums of a point pass in LDS: indexed by polygon when np <=
 // lds_cap, else kHashSlots hashed slots with a 4-byte tag each.

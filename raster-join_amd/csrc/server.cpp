@@ -186,10 +186,9 @@ void rj_server::run_job(std::shared_ptr<Job> J)
     }
     for (int i = 0; i < n; i++)
         if (J->st[i] != RJ_OK) complete(*J, i, J->st[i]);
-    // passes of at most max_batch queries
+    // passes of at most max_batch queries (and border-queue entries)
     std::vector<std::vector<int>> passes;
-    for (size_t k = 0; k < fused.size(); k += (size_t)max_batch)
-        passes.emplace_back(fused.begin() + k, fused.begin() + std::min(fused.size(), k + (size_t)max_batch));
+    ctx->group_passes(t, J->specs.data(), fused, max_batch, &passes);
     J->passes_left = (int)passes.size() + 1;  // + 1 until the queries that run alone are done
     for (auto& p : passes) {
         const int s = next_slot;

@@ -29,7 +29,7 @@ COUNT, AVG, SUM = 0, 1, 2
 OPT_RASTER_PATH, OPT_TIMING, OPT_HYBRID_DILATION, OPT_CACHE, OPT_ORGANIZE = 1, 2, 3, 4, 5
 OPT_TILE_SHIFT, OPT_PASS_WORKGROUPS, OPT_BATCH_ROWS, OPT_BATCH_WORKGROUPS, OPT_BATCH_CHUNK_ROWS, OPT_GRAPH = \
     6, 7, 8, 9, 10, 11
-OPT_INDEX_PATH, OPT_EXACT_QUEUE, OPT_STREAM_ROWS, OPT_BATCH_HYBRID_ROWS = 12, 13, 14, 15
+OPT_INDEX_PATH, OPT_EXACT_QUEUE, OPT_STREAM_ROWS, OPT_BATCH_HYBRID_ROWS, OPT_BATCH_SHARE = 12, 13, 14, 15, 16
 # how a query ran
 PATH_SINGLE, PATH_FUSED = 0, 1
 

@@ -300,3 +300,34 @@ def test_server_memory_admission_and_eviction(handler, data):
         assert b is not None and b.status == 0, i
         assert same(a, b), i
     assert st["completed"] == len(qs)
+
+
+@pytest.mark.parametrize("share", [1, 0])
+def test_groups_on_one_label_map_equal_single(handler, data, share):
+    """Fused raster queries on the same label map (same collection and
+    accuracy) share each chunk's labels; each still applies its own window,
+    filter and aggregate. Mixed with hybrid and index queries of the same
+    collections, results equal the queries run alone, sharing on and off."""
+    h = handler
+    tb = bucketed(h, data)
+    sets = polysets(h, ("voronoi260", "tracts400"))
+    rng = np.random.default_rng(23)
+    qs, nps = [], []
+    for i in range(30):
+        ps = sets[i % 2]
+        fn = (rj.RASTER, rj.RASTER, rj.HYBRID, rj.INDEX)[i % 4]
+        length = int(np.exp(rng.uniform(np.log(3600), np.log(YEAR))))
+        cons = window(rng.uniform(0, 1 - length / YEAR), length)
+        if i % 3 == 0:
+            cons += [(2, rj.LTE, int(rng.integers(1, 7)))]
+        agg = rj.AVG if i % 5 < 2 else rj.COUNT
+        qs.append(rj.make_query(ps, fn, agg, 1 if agg == rj.AVG else -1, cons, eps=20.0, index_res=256))
+        nps.append(ps.np)
+    h.set_option(rj.OPT_BATCH_SHARE, share)
+    try:
+        batch = h.execute_batch(tb, qs, nps)
+    finally:
+        h.set_option(rj.OPT_BATCH_SHARE, 1)
+    for i, (q, k, b) in enumerate(zip(qs, nps, batch)):
+        assert b.status == 0 and b.path == rj.PATH_FUSED, i
+        assert same(h.execute_query(tb, q, k), b), i
