@@ -406,7 +406,42 @@ def batch_section(h, table, lib, args):
         res["fused_pass"] = {"kernel": "batch", "ms": round(bms, 3), "achieved": round(ach, 1),
                              "unit": "GB/s", "peak": HBM_PEAK_GBS, "frac": round(ach / HBM_PEAK_GBS, 4),
                              "alg_bytes": int(union * ROW_BYTES)}
+    res["same_collection"] = same_collection_batch(h, table, lib)
     return res
+
+
+def same_collection_batch(h, table, lib):
+    """16 raster queries on one collection and accuracy (windows of 30-100% of
+    the table, half with a passengers filter, count or avg): fused with the
+    labels shared per chunk, fused without sharing, one at a time."""
+    ps = lib["v260"]
+    rng = np.random.default_rng(3)
+    qs = []
+    for i in range(16):
+        cons = window_rows_frac(rng, 0.3, 1.0)
+        if i % 2:
+            cons.append((2, rj.LTE, 3))
+        agg = rj.AVG if i % 4 < 2 else rj.COUNT
+        qs.append(rj.make_query(ps, rj.RASTER, agg, 1 if agg == rj.AVG else -1, cons, eps=10.0))
+    nps = [ps.np] * 16
+    single = [h.execute_query(table, q, ps.np) for q in qs]
+    t0 = time.perf_counter()
+    for q in qs:
+        h.execute_query(table, q, ps.np)
+    seq_s = time.perf_counter() - t0
+    out = {}
+    for share in (1, 0):
+        h.set_option(rj.OPT_BATCH_SHARE, share)
+        h.execute_batch(table, qs, nps)
+        t0 = time.perf_counter()
+        r = h.execute_batch(table, qs, nps)
+        out[share] = (time.perf_counter() - t0, r)
+    h.set_option(rj.OPT_BATCH_SHARE, 1)
+    ok = all(same(a, b) and same(a, c) for a, b, c in zip(single, out[1][1], out[0][1]))
+    return {"workload": "16 raster queries, 260 polygons, eps=10 m, windows 30-100% of the 1B rows",
+            "fused_shared_ms": round(out[1][0] * 1e3, 2), "fused_unshared_ms": round(out[0][0] * 1e3, 2),
+            "one_at_a_time_ms": round(seq_s * 1e3, 2), "speedup": round(seq_s / out[1][0], 2),
+            "identical_results": bool(ok)}
 
 
 def serve(h, table, qs, nps, rate, seed, max_batch=64):
