@@ -186,6 +186,34 @@ __global__ __launch_bounds__(kBlock) void k_band(const Band* __restrict__ bands,
 
     unsigned long long s_cnt = 0, s_hi = 0, s_lo = 0;
     unsigned long long s_fn = 0, s_fp = 0, s_fn1 = 0, s_fp1 = 0;
+    if (MODE == BAND_LABEL && label16) {
+        // 16-bit label map: aligned groups of 8 pixels, one 16-byte store
+        // where the polygon covers all 8 (rows of the map are 64-byte aligned)
+        const int g0 = c0 >> 3, ng = (c1 >> 3) - g0 + 1;
+        const uint32_t pp = (uint32_t)p | ((uint32_t)p << 16);
+        for (int idx = threadIdx.x; idx < nr * ng; idx += blockDim.x) {
+            const int row = idx / ng, g = g0 + idx % ng;
+            const uint32_t* trow = tog + row * wpr;
+            uint32_t m = 0;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int lc = 8 * g + k - c0;
+                if (lc >= 0 && lc < bw && bit_at(trow, lc)) m |= 1u << k;
+            }
+            if (!m) continue;
+            uint16_t* dst = static_cast<uint16_t*>(label) + (int64_t)(r0 + row) * cv.LW + 8 * g;
+            if (m == 0xFFu) {
+                *reinterpret_cast<uint4*>(dst) = make_uint4(pp, pp, pp, pp);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    if ((m >> k) & 1u) dst[k] = (uint16_t)p;
+            }
+            s_cnt += __popc(m);
+        }
+        block_add4(s_cnt, 0, 0, 0, aux, nullptr, nullptr, nullptr);
+        return;
+    }
     // Threads cover 2^lg columns of (256 >> lg) rows at a time: no division.
     const int lg = bw >= kBlock ? 8 : (32 - __clz(bw - 1));
     const int colstep = 1 << lg;
