@@ -688,6 +688,8 @@ DevGrid rj_ctx::dev_grid(const Grid* G) const
     g.minx = G->g.minx; g.miny = G->g.miny; g.width = G->g.width; g.height = G->g.height;
     g.cell_off = G->d_cell_off.as<int32_t>();
     g.cand = G->d_cand.as<int32_t>();
+    g.band_h = G->g.band_h;
+    g.band_rows = G->g.band_rows;
     const int32_t* b = G->d_bands.as<int32_t>();
     const size_t np = G->g.band_lo.size();
     g.band_lo = b;

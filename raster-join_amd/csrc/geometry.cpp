@@ -294,43 +294,36 @@ void parallel_polys(int np, F f)
 // Grid row of a y coordinate, as the point pass computes it (getYIndex,
 // GridIndexF.cpp:44-47, with the in-grid test of getRegion): -1 below the
 // grid, ys above it. Monotone in t.
-static int grid_row(float t, float miny, float height, int ys)
-{
-    const float fy = (t - miny) / height;
-    if (!(fy > -1.0f)) return -1;
-    if (!(fy < (float)ys)) return ys;
-    return (int)fy;
-}
-
-// Row bands of every polygon (see GridIndex). Edge (vi, vj) toggles the
+// Edge bands of every polygon (see GridIndex). Edge (vi, vj) toggles the
 // crossing test of a point at ty iff min(vi.y, vj.y) <= ty < max(vi.y, vj.y)
-// ((vi.y > ty) != (vj.y > ty)); the rows of those ty run from
-// grid_row(min) to grid_row(the largest float below max), as grid_row is
-// monotone.
-static void build_bands(const PolySet& ps, GridIndex* g)
+// ((vi.y > ty) != (vj.y > ty)); the bands of those ty run from band_row(min)
+// to band_row(the largest float below max), as band_row is monotone. The
+// bands are the grid rows split k ways, k (a power of two up to 16) chosen so
+// that a band holds about 8 edges of a polygon on average.
+static void bands_for(const PolySet& ps, float miny, float band_h, int rows, GridIndex* g)
 {
-    const int ys = g->ys;
-    const float miny = g->miny, height = g->height;
     const int np = ps.np;
+    g->band_h = band_h;
+    g->band_rows = rows;
     g->band_lo.assign(np, 0);
     g->band_hi.assign(np, -1);
     g->pband.assign(np + 1, 0);
-    std::vector<std::vector<int32_t>> lo_hi(np);  // per polygon: (row0, row1) per edge
+    std::vector<std::vector<int32_t>> lo_hi(np);  // per polygon: (band0, band1) per edge
     parallel_polys(np, [&](int t, int nt) {
         for (int p = t; p < np; p += nt) {
             const float* v = ps.xy.data() + 2 * (size_t)ps.off[p];
             const int n = ps.off[p + 1] - ps.off[p];
             std::vector<int32_t>& r = lo_hi[p];
             r.resize(2 * (size_t)n);
-            int blo = ys, bhi = -1;
+            int blo = rows, bhi = -1;
             for (int i = 0; i < n; i++) {
                 const int j = i ? i - 1 : n - 1;
                 const float y0 = std::min(v[2 * i + 1], v[2 * j + 1]);
                 const float y1 = std::max(v[2 * i + 1], v[2 * j + 1]);
                 int r0 = 0, r1 = -1;
                 if (y0 < y1) {
-                    r0 = std::max(0, grid_row(y0, miny, height, ys));
-                    r1 = std::min(ys - 1, grid_row(std::nextafter(y1, -INFINITY), miny, height, ys));
+                    r0 = band_row(y0, miny, band_h, rows);
+                    r1 = band_row(std::nextafter(y1, -INFINITY), miny, band_h, rows);
                 }
                 r[2 * i] = r0;
                 r[2 * i + 1] = r1;
@@ -367,6 +360,16 @@ static void build_bands(const PolySet& ps, GridIndex* g)
             }
         }
     }
+}
+
+static void build_bands(const PolySet& ps, GridIndex* g)
+{
+    bands_for(ps, g->miny, g->height, g->ys, g);
+    const size_t nb = std::max<size_t>(1, g->band_off.size() - 1);
+    const double per = (double)g->band_off.back() / (double)nb;  // edges per band at the grid's rows
+    int k = 1;
+    while (k < 64 && per / k > 4.0) k *= 2;
+    if (k > 1) bands_for(ps, g->miny, g->height / (float)k, g->ys * k, g);
 }
 
 void build_grid(const PolySet& ps, int res, GridIndex* g)

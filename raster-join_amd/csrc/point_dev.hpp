@@ -333,9 +333,10 @@ template <bool AVG, bool LDS = true>
 __device__ __forceinline__ int exact_first(const DevGrid& g, const DevPolys& P, const Acc& A, float x,
                                            float y, int agg_t, uint32_t vraw)
 {
-    int row;
-    const int cell = grid_cell(g, x, y, row);
+    int grow;
+    const int cell = grid_cell(g, x, y, grow);
     if (cell < 0) return -1;
+    const int row = band_row(y, g.miny, g.band_h, g.band_rows);
     int first = -1;
     const int k1 = g.cell_off[cell + 1];
     for (int k = g.cell_off[cell]; k < k1; k++) {

@@ -60,12 +60,24 @@ struct DevGrid {
     float minx, miny, width, height;
     const int32_t* cell_off;
     const int32_t* cand;
-    const int32_t* band_lo;   // per polygon: first and last row with edges
+    float band_h;             // edge bands: height and count (band_row)
+    int32_t band_rows;
+    const int32_t* band_lo;   // per polygon: first and last band with edges
     const int32_t* band_hi;
     const int32_t* pband;     // per polygon: its first band
     const int32_t* band_off;  // per band: first edge
     const float4* edges;      // (vi.x, vi.y, vj.x, vj.y)
 };
+
+// Edge band of a y (GridIndex): the same float operations on the host and
+// the device; monotone in t, clamped to [0, rows).
+__host__ __device__ inline int band_row(float t, float miny, float band_h, int rows)
+{
+    const float fy = (t - miny) / band_h;
+    if (!(fy >= 0.0f)) return 0;
+    if (!(fy < (float)rows)) return rows - 1;
+    return (int)fy;
+}
 
 // Polygon collection on the device.
 struct DevPolys {
