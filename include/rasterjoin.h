@@ -245,7 +245,8 @@ int rj_canvas_info(rj_ctx* ctx, int32_t* W, int32_t* H);
  *                           polygons, label maps, grid index) across executes;
  *                           0 rebuilds them on every execute as the reference does
  *   RJ_OPT_ORGANIZE:        1 (default) rj_set_points orders rows spatially
- *   RJ_OPT_TILE_SHIFT:      label tile side 2^v pixels, v in 3..5 (default 4)
+ *   RJ_OPT_TILE_SHIFT:      label tile side 2^v pixels, v in 3..5; 0 (default): 8 px for
+ *                           raster label maps, 16 px for hybrid maps
  *   RJ_OPT_PASS_WORKGROUPS: workgroups of a single-query point pass (0: 4096)
  *   RJ_OPT_BATCH_ROWS:      rows per thread of the fused label-map and index kernels, 4 (default) or 8
  *   RJ_OPT_BATCH_HYBRID_ROWS: rows per thread of the fused hybrid kernel, 4 or 8 (default)

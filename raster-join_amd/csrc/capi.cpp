@@ -514,7 +514,7 @@ int rj_set_option(rj_ctx* ctx, int32_t key, int64_t value)
     case RJ_OPT_CACHE: ctx->cache = value != 0; return RJ_OK;
     case RJ_OPT_ORGANIZE: ctx->organize = value != 0; return RJ_OK;
     case RJ_OPT_TILE_SHIFT:
-        if (value < 3 || value > 5) return ctx->fail(RJ_ERR_ARG, "tile shift is 3, 4 or 5");
+        if (value != 0 && (value < 3 || value > 5)) return ctx->fail(RJ_ERR_ARG, "tile shift is 0 (auto), 3, 4 or 5");
         ctx->opt_tshift = (int)value;
         return RJ_OK;
     case RJ_OPT_PASS_WORKGROUPS:

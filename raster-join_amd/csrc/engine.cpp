@@ -503,16 +503,17 @@ int rj_ctx::build_labels(rj_polyset* P, Canvas* C, void* lab, int label16, DevBu
 int rj_ctx::raster_labels(rj_polyset* P, Canvas* C, bool* fresh)
 {
     *fresh = false;
-    if (C->lab_built && cache && C->lab_tshift == opt_tshift) return RJ_OK;
+    const int tshift = raster_tshift();
+    if (C->lab_built && cache && C->lab_tshift == tshift) return RJ_OK;
     const DevCanvas cv = dev_canvas(C->plan);
     const size_t npx = (size_t)cv.LW * cv.H;
     const int l16 = P->ps.np < (int)0xFFF0 ? 1 : 0;
     int rc = alloc(C->d_lab, npx * (l16 ? 2 : 4) + 16, "label map");
     if (rc) return rc;
-    if ((rc = build_labels(P, C, C->d_lab.p, l16, C->d_tiles, opt_tshift, true))) return rc;
+    if ((rc = build_labels(P, C, C->d_lab.p, l16, C->d_tiles, tshift, true))) return rc;
     C->lab_built = true;
     C->lab16 = l16;
-    C->lab_tshift = opt_tshift;
+    C->lab_tshift = tshift;
     C->lab_state = 0;  // confirmed when the counters are read
     *fresh = true;
     return RJ_OK;
@@ -564,7 +565,8 @@ void rj_ctx::hybrid_args(const Canvas* C, PointArgs* pa) const
 // dilated by whole pixels, as the 3x3 dilation of hybrid.glsl:174-185.
 int rj_ctx::hybrid_labels(rj_polyset* P, Canvas* C, Grid* G)
 {
-    if (C->hyb_res == G->g.xs && cache && C->hyb_tshift == opt_tshift) return RJ_OK;
+    const int tshift = hybrid_tshift();
+    if (C->hyb_res == G->g.xs && cache && C->hyb_tshift == tshift) return RJ_OK;
     const double delta = hybrid_delta(C->plan);
     const bool halo = hybrid_dilation == 0 && delta <= 0.25;
     const int r = halo ? 0 : hybrid_radius(C->plan);
@@ -583,9 +585,9 @@ int rj_ctx::hybrid_labels(rj_polyset* P, Canvas* C, Grid* G)
     int l16 = P->ps.np < (int)(kHaloFlag16 - 16) ? 1 : 0;
     if (P->ps.np >= (int)kHaloFlag32) return fail(RJ_ERR_LIMIT, "too many polygons for a hybrid label map");
     if ((rc = alloc(C->d_hlab, npx * (l16 ? 2 : 4) + 16, "hybrid label map"))) return rc;
-    if ((rc = build_labels(P, C, C->d_hlab.p, l16, C->d_htiles, opt_tshift, true))) return rc;
+    if ((rc = build_labels(P, C, C->d_hlab.p, l16, C->d_htiles, tshift, true))) return rc;
     unsigned long long a[2] = {0, 0};
-    hipError_t e = hipMemcpyAsync(a, tiles_aux(C->d_htiles, cv, opt_tshift), 16, hipMemcpyDeviceToHost, stream);
+    hipError_t e = hipMemcpyAsync(a, tiles_aux(C->d_htiles, cv, tshift), 16, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "overlap test");
     if (a[0] != a[1]) {
@@ -613,11 +615,11 @@ int rj_ctx::hybrid_labels(rj_polyset* P, Canvas* C, Grid* G)
     if ((rc = launch("dilate", [&] { launch_dilate_to_label(stream, mask, cv, r, lab, l16); }))) return rc;
     if (halo && (rc = launch("halo_flags", [&] { launch_halo_flags(stream, mask, cv, lab, l16); }))) return rc;
     uint32_t* t = C->d_htiles.as<uint32_t>();
-    if ((rc = launch("tile_labels", [&] { launch_tile_labels(stream, lab, l16, cv, opt_tshift, t, nullptr); })))
+    if ((rc = launch("tile_labels", [&] { launch_tile_labels(stream, lab, l16, cv, tshift, t, nullptr); })))
         return rc;
     C->hyb16 = l16;
     C->hyb_res = G->g.xs;
-    C->hyb_tshift = opt_tshift;
+    C->hyb_tshift = tshift;
     C->hyb_halo = halo;
     C->hyb_delta = (float)delta;
     return RJ_OK;
@@ -1764,7 +1766,7 @@ int rj_ctx::autotune(rj_table* t, const QuerySpec* qs_in, int nq, const std::vec
     std::vector<int> st(nq);
     bool have_ref = false;
     double best = 1e30;
-    int32_t bestc[5] = {base_tshift, base_rows, base_wg, base_chunks, base_res};
+    int32_t bestc[5] = {raster_tshift(), base_rows, base_wg, base_chunks, base_res};
     auto run = [&](double* ms) -> int {
         double m = 1e30;
         for (int k = 0; k < 3; k++) {

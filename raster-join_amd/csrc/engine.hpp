@@ -169,7 +169,9 @@ struct rj_ctx {
     bool cache = true;
     bool organize = true;
     int opt_cap = 0;                       // workgroups of a single-query pass (0: 4096)
-    int opt_tshift = rj::kTileShiftDefault;  // label tile side 2^tshift
+    int opt_tshift = 0;  // label tile side 2^tshift (0: 8 px for raster maps, 16 px for hybrid maps)
+    int raster_tshift() const { return opt_tshift ? opt_tshift : 3; }
+    int hybrid_tshift() const { return opt_tshift ? opt_tshift : rj::kTileShiftDefault; }
     int opt_batch_rows = 4;                // rows per thread of the fused label-map and index kernels
     int opt_batch_hybrid_rows = 8;         // rows per thread of the fused hybrid kernel
     bool opt_batch_share = true;           // fused members on one label map share its lookups

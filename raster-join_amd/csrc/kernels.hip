@@ -359,57 +359,61 @@ __global__ void k_dilate_to_label(const uint32_t* __restrict__ mask, int W, int 
 // to a multiple of 32 pixels, so a tile's threads are all active or all
 // idle). The map is small enough to stay in L2 during the point pass.
 template <int TS>
-__global__ void k_tile_labels(const void* __restrict__ label, int label16, int LW, int H, int TW,
+__global__ void k_tile_labels(const void* __restrict__ label, int label16, int LW, int H, int TW, int TH,
                               uint32_t* __restrict__ tiles, unsigned long long* __restrict__ labeled)
 {
     constexpr int T = 1 << TS;
+    constexpr int KT = TS >= 5 ? 1 : (32 >> TS) / 2;  // tile rows per workgroup (16 pixel rows, at least one tile)
     const int per = label16 ? 8 : 4;  // labels per vector
     const int tpt = T / per;          // threads per tile
     const int nvec = LW / per;
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ty = blockIdx.y;
     const bool active = v < nvec;
-    uint32_t first = 0;
-    int uni = 1;
     unsigned long long nlab = 0;
-    if (active) {
-        const int y0 = ty << TS;
-        const int rows = min(T, H - y0);
-        const size_t row_bytes = (size_t)LW * (label16 ? 2 : 4);
-        const char* base = static_cast<const char*>(label) + (size_t)y0 * row_bytes;
-        const uint4 w0 = reinterpret_cast<const uint4*>(base)[v];
-        const uint32_t a0 = w0.x & 0xFFFFu;
-        first = label16 ? (a0 >= 0xFFF0u ? (0xFFFF0000u | a0) : a0) : w0.x;
+    for (int kt = 0; kt < KT; kt++) {
+        const int ty = blockIdx.y * KT + kt;
+        if (ty >= TH) break;
+        uint32_t first = 0;
+        int uni = 1;
+        if (active) {
+            const int y0 = ty << TS;
+            const int rows = min(T, H - y0);
+            const size_t row_bytes = (size_t)LW * (label16 ? 2 : 4);
+            const char* base = static_cast<const char*>(label) + (size_t)y0 * row_bytes;
+            const uint4 w0 = reinterpret_cast<const uint4*>(base)[v];
+            const uint32_t a0 = w0.x & 0xFFFFu;
+            first = label16 ? (a0 >= 0xFFF0u ? (0xFFFF0000u | a0) : a0) : w0.x;
 #pragma unroll 8
-        for (int k = 0; k < T; k++) {
-            if (k < rows) {
-                const uint4 w = reinterpret_cast<const uint4*>(base + (size_t)k * row_bytes)[v];
-                const uint32_t x[4] = {w.x, w.y, w.z, w.w};
-                if (label16) {
+            for (int k = 0; k < T; k++) {
+                if (k < rows) {
+                    const uint4 w = reinterpret_cast<const uint4*>(base + (size_t)k * row_bytes)[v];
+                    const uint32_t x[4] = {w.x, w.y, w.z, w.w};
+                    if (label16) {
 #pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const uint32_t a = x[j] & 0xFFFFu, b = x[j] >> 16;
-                        const uint32_t va = a >= 0xFFF0u ? (0xFFFF0000u | a) : a;
-                        const uint32_t vb = b >= 0xFFF0u ? (0xFFFF0000u | b) : b;
-                        uni &= (va == first) & (vb == first);
-                        nlab += (va != kLabelNone) + (vb != kLabelNone);
-                    }
-                } else {
+                        for (int j = 0; j < 4; j++) {
+                            const uint32_t a = x[j] & 0xFFFFu, b = x[j] >> 16;
+                            const uint32_t va = a >= 0xFFF0u ? (0xFFFF0000u | a) : a;
+                            const uint32_t vb = b >= 0xFFF0u ? (0xFFFF0000u | b) : b;
+                            uni &= (va == first) & (vb == first);
+                            nlab += (va != kLabelNone) + (vb != kLabelNone);
+                        }
+                    } else {
 #pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        uni &= (x[j] == first);
-                        nlab += (x[j] != kLabelNone);
+                        for (int j = 0; j < 4; j++) {
+                            uni &= (x[j] == first);
+                            nlab += (x[j] != kLabelNone);
+                        }
                     }
                 }
             }
         }
+        for (int d = 1; d < tpt; d <<= 1) {
+            const uint32_t of = __shfl_xor(first, d, 64);
+            const int ou = __shfl_xor(uni, d, 64);
+            uni = uni && ou && (of == first);
+        }
+        if (active && (v % tpt) == 0) tiles[(size_t)ty * TW + v / tpt] = uni ? first : kLabelMixed;
     }
-    for (int d = 1; d < tpt; d <<= 1) {
-        const uint32_t of = __shfl_xor(first, d, 64);
-        const int ou = __shfl_xor(uni, d, 64);
-        uni = uni && ou && (of == first);
-    }
-    if (active && (v % tpt) == 0) tiles[(size_t)ty * TW + v / tpt] = uni ? first : kLabelMixed;
     if (labeled) block_add4(nlab, 0, 0, 0, labeled, nullptr, nullptr, nullptr);
 }
 
@@ -560,13 +564,17 @@ void launch_tile_labels(hipStream_t s, const void* label, int label16, const Dev
 {
     const int TW = c.LW >> tshift, TH = (c.H + (1 << tshift) - 1) >> tshift;
     const int nvec = c.LW / (label16 ? 8 : 4);
-    const dim3 grid((unsigned)((nvec + kBlock - 1) / kBlock), (unsigned)TH);
+    const int kt = tshift >= 5 ? 1 : (32 >> tshift) / 2;  // tile rows per workgroup, as the kernel
+    const dim3 grid((unsigned)((nvec + kBlock - 1) / kBlock), (unsigned)((TH + kt - 1) / kt));
     if (tshift == 3)
-        hipLaunchKernelGGL(k_tile_labels<3>, grid, dim3(kBlock), 0, s, label, label16, c.LW, c.H, TW, tiles, labeled);
+        hipLaunchKernelGGL(k_tile_labels<3>, grid, dim3(kBlock), 0, s, label, label16, c.LW, c.H, TW, TH, tiles,
+                           labeled);
     else if (tshift == 5)
-        hipLaunchKernelGGL(k_tile_labels<5>, grid, dim3(kBlock), 0, s, label, label16, c.LW, c.H, TW, tiles, labeled);
+        hipLaunchKernelGGL(k_tile_labels<5>, grid, dim3(kBlock), 0, s, label, label16, c.LW, c.H, TW, TH, tiles,
+                           labeled);
     else
-        hipLaunchKernelGGL(k_tile_labels<4>, grid, dim3(kBlock), 0, s, label, label16, c.LW, c.H, TW, tiles, labeled);
+        hipLaunchKernelGGL(k_tile_labels<4>, grid, dim3(kBlock), 0, s, label, label16, c.LW, c.H, TW, TH, tiles,
+                           labeled);
 }
 
 // Hybrid label maps in the halo mode: every pixel that is not itself border
