@@ -90,7 +90,7 @@ def timed_steps(h, fn, steps, warmup):
 
 
 def headline(h, pts, cols, xy, off, args):
-    """Config 2: raster eps=10 m, count + avg(fare), 100M points."""
+    """Raster join at eps = 10 m, count + avg(fare), 100M points, 260 polygons."""
     n = len(pts) // 2
     t0 = time.perf_counter()
     table = rj.Table(h, pts, cols, TYPES, organize=True)
@@ -172,11 +172,11 @@ def count_1b(h, table, xy, off, args):
                          "kernel_ms": round(k_ms, 4), "alg_bytes": int(alg)}}
 
 
-def configs_section(h, pts, cols, args):
+def workloads_section(h, pts, cols, args):
     """The index join of 1M points (grid 256, both index paths) and the
     hybrid join of 500M points with two filters over 2200 tracts."""
     out = {}
-    cp = rj.CONFIG_POLYS["voronoi260"]
+    cp = rj.POLY_SETS["voronoi260"]
     xy, off = rj.gen_voronoi(cp["nsites"], seed=cp["seed"], spacing=cp["spacing"], jitter=cp["jitter"])
     n1 = 1_000_000
     t1 = rj.Table(h, pts[:2 * n1], (), (), organize=False)
@@ -197,11 +197,11 @@ def configs_section(h, pts, cols, args):
         cfg1[name] = {"ms_min": round(min(ts) * 1e3, 3), "ms_median": round(float(np.median(ts)) * 1e3, 3),
                       "G_points_per_s": round(n1 / min(ts) / 1e9, 3)}
     h.set_option(rj.OPT_INDEX_PATH, 0)
-    out["config1_index_1M"] = cfg1
+    out["index_1M"] = cfg1
     t1.close()
 
     n3 = min(500_000_000, len(pts) // 2)
-    cp = rj.CONFIG_POLYS["tracts2200"]
+    cp = rj.POLY_SETS["tracts2200"]
     txy, toff = rj.gen_voronoi(cp["nsites"], seed=cp["seed"], spacing=cp["spacing"], jitter=cp["jitter"])
     t3 = rj.Table(h, pts[:2 * n3], tuple(c[:n3] for c in cols), TYPES, organize=True)
     h.set_polygons(txy, toff)
@@ -226,7 +226,7 @@ def configs_section(h, pts, cols, args):
     alg = n3 * 17 + 8192 * 8192 * 2
     ach = alg / (k_ms * 1e-3) / 1e9
     h.set_option(rj.OPT_TIMING, 0)
-    out["config3_hybrid_500M"] = {
+    out["hybrid_500M_filters"] = {
         "workload": f"hybrid eps=20 m, count()+avg(fare), {n3} points (Morton order), time window 30% "
                     "and passengers<=2, 2200 tract polygons, grid 1024, polygon structures cached",
         "ms_per_join": round(dt / steps * 1e3, 3), "G_points_per_s": round(n3 * steps / dt / 1e9, 2),
@@ -308,7 +308,7 @@ def window_rows_frac(rng, lo_frac, hi_frac):
 
 
 def batch_queries(lib, n, seed, nrows):
-    """Config 4: raster + hybrid, polygon sets of 10-2200, windows selecting
+    """The 64-query batch: raster + hybrid, polygon sets of 10-2200, windows selecting
     1M-1B points, eps in {5, 10, 50, 100}."""
     rng = np.random.default_rng(seed)
     names = ["v10", "v40", "v260", "v900", "t2200"]
@@ -325,7 +325,7 @@ def batch_queries(lib, n, seed, nrows):
 
 
 def serving_queries(lib, n, seed):
-    """Config 5: 60/30/10 raster/hybrid/index, 10-5000 polygons, windows
+    """The serving mix: 60/30/10 raster/hybrid/index, 10-5000 polygons, windows
     log-uniform from 1 hour to 1 year, an optional passengers filter."""
     rng = np.random.default_rng(seed)
     names = list(POLY_LIB)
@@ -524,8 +524,8 @@ def main():
                     help="queries of the plain one-at-a-time serving baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--headline-only", action="store_true")
-    ap.add_argument("--sections", default="headline,count,configs,batch,serving,stream",
-                    help="comma list of headline, count, configs, batch, serving, stream")
+    ap.add_argument("--sections", default="headline,count,workloads,batch,serving,stream",
+                    help="comma list of headline, count, workloads, batch, serving, stream")
     ap.add_argument("--stream-rows", type=int, default=500_000_000, help="rows of the host-resident table")
     ap.add_argument("--calib", action="store_true",
                     help="first run a plain 1 GiB device copy (HBM counter calibration)")
@@ -547,9 +547,9 @@ def main():
     sections = set(args.sections.split(","))
     if args.headline_only:
         sections = {"headline"}
-    big = max(args.big, 0) if sections & {"count", "configs", "batch", "serving", "stream"} else 0
+    big = max(args.big, 0) if sections & {"count", "workloads", "batch", "serving", "stream"} else 0
     nrows = max(big, args.points)
-    cp = rj.CONFIG_POLYS["voronoi260"]
+    cp = rj.POLY_SETS["voronoi260"]
     t0 = time.perf_counter()
     xy, off = rj.gen_voronoi(cp["nsites"], seed=cp["seed"], spacing=cp["spacing"], jitter=cp["jitter"])
     pts, tm, fare, pas = rj.gen_points(nrows, params=rj.point_params(seed=7))
@@ -584,9 +584,9 @@ def main():
         out["raster_count_1B"] = count_1b(h, mt, xy, off, args)
         log("raster count 1B:", out["raster_count_1B"])
         mt.close()  # the context's bound table is replaced before any further join
-    if big and "configs" in sections:
-        out.update(configs_section(h, pts, (tm, fare, pas), args))
-        log("configs:", {k: out[k] for k in ("config1_index_1M", "config3_hybrid_500M")})
+    if big and "workloads" in sections:
+        out.update(workloads_section(h, pts, (tm, fare, pas), args))
+        log("workloads:", {k: out[k] for k in ("index_1M", "hybrid_500M_filters")})
     lib = poly_library(h, list(POLY_LIB)) if big and sections & {"batch", "serving", "stream"} else {}
     if big and sections & {"batch", "serving"}:
         t0 = time.perf_counter()

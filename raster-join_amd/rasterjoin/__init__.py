@@ -182,7 +182,7 @@ def gen_voronoi(nsites, seed=1, spacing=(60.0, 60.0), jitter=30.0, box=50000.0):
 
 
 # The benchmark configurations (point counts, polygon sets, accuracy).
-CONFIG_POLYS = {
+POLY_SETS = {
     "voronoi260": dict(nsites=260, seed=1, spacing=(60.0, 60.0), jitter=30.0),
     "tracts2200": dict(nsites=2200, seed=3, spacing=(2.2, 140.0), jitter=30.0),
 }

@@ -9,10 +9,10 @@ import rasterjoin as rj
 @functools.lru_cache(maxsize=None)
 def polys(name):
     if name == "voronoi260":
-        cp = rj.CONFIG_POLYS["voronoi260"]
+        cp = rj.POLY_SETS["voronoi260"]
         return rj.gen_voronoi(cp["nsites"], seed=cp["seed"], spacing=cp["spacing"], jitter=cp["jitter"])
     if name == "tracts2200":
-        cp = rj.CONFIG_POLYS["tracts2200"]
+        cp = rj.POLY_SETS["tracts2200"]
         return rj.gen_voronoi(cp["nsites"], seed=cp["seed"], spacing=cp["spacing"], jitter=cp["jitter"])
     if name == "tracts400":
         return rj.gen_voronoi(400, seed=17, spacing=(2.2, 140.0), jitter=30.0)

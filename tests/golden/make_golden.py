@@ -6,7 +6,7 @@ per-polygon counts of GridIndexF::getRegion as CPUJoin::execute counts them):
   toy_squares.npz    the 5-point / 2-square sample of RasterJoin/test/baseline.hpp:139-217
   squares_edges.npz  axis-aligned squares with points on vertices and edges
   voronoi20.npz      20 Voronoi cells (shared jittered edges), 6000 points, grid 64
-  config1.npz        counts for the 1M-point / 260-polygon / grid-256 workload,
+  index_1M_voronoi260.npz        counts for the 1M-point / 260-polygon / grid-256 workload,
                      with checksums of the regenerated inputs (inputs are not stored)
 Run from the repository root: python tests/golden/make_golden.py
 """
@@ -79,12 +79,12 @@ def voronoi20():
                         cell_off=cell_off, cand=cand, counts=counts)
 
 
-def config1():
-    cp = rj.CONFIG_POLYS["voronoi260"]
+def index_1M_voronoi260():
+    cp = rj.POLY_SETS["voronoi260"]
     xy, off = rj.gen_voronoi(cp["nsites"], seed=cp["seed"], spacing=cp["spacing"], jitter=cp["jitter"])
     pts, _, _, _ = rj.gen_points(1_000_000, params=rj.point_params(seed=7), columns=False)
     cell_off, cand, counts = ref_outputs(xy, off, pts, 256)
-    np.savez_compressed(os.path.join(OUT, "config1.npz"), res=256, counts=counts,
+    np.savez_compressed(os.path.join(OUT, "index_1M_voronoi260.npz"), res=256, counts=counts,
                         ncand=len(cand), cand_sha=sha(cell_off, cand),
                         poly_sha=sha(xy, off), pts_sha=sha(pts), npts=1_000_000, point_seed=7)
 
@@ -95,7 +95,7 @@ if __name__ == "__main__":
     toy()
     squares_edges()
     voronoi20()
-    config1()
+    index_1M_voronoi260()
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)), "bytes")

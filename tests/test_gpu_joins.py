@@ -60,9 +60,9 @@ def test_index_golden_reference_fixture(ih, name):
     assert np.array_equal(c, d["counts"])
 
 
-def test_index_config1_matches_reference_counts(ih):
+def test_index_1M_voronoi260_matches_reference_counts(ih):
     """1M points, 260 polygons, grid 256 (the CPUJoin-comparable workload)."""
-    d = np.load(os.path.join(GOLD, "config1.npz"))
+    d = np.load(os.path.join(GOLD, "index_1M_voronoi260.npz"))
     xy, off = polys("voronoi260")
     pts = points(1_000_000, 7)[0]
     c, _, _, _ = run(ih, rj.INDEX, xy, off, pts, res=256)
@@ -210,8 +210,8 @@ def test_hybrid_and_index_points_near_edges(ih, name):
         assert np.array_equal(hi, ohi), fn
 
 
-def test_hybrid_count_config1_reference(handler):
-    d = np.load(os.path.join(GOLD, "config1.npz"))
+def test_hybrid_count_1M_voronoi260_reference(handler):
+    d = np.load(os.path.join(GOLD, "index_1M_voronoi260.npz"))
     xy, off = polys("voronoi260")
     pts = points(1_000_000, 7)[0]
     c, _, _, _ = run(handler, rj.HYBRID, xy, off, pts, eps=20.0, res=256)

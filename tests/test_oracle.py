@@ -46,10 +46,10 @@ def test_golden_fixture(name):
     assert np.array_equal(counts, d["counts"])
 
 
-def test_config1_golden():
+def test_index_1M_voronoi260_golden():
     """1M points, 260 Voronoi polygons, grid 256: the reference's counts."""
-    d = np.load(os.path.join(GOLD, "config1.npz"))
-    cp = rj.CONFIG_POLYS["voronoi260"]
+    d = np.load(os.path.join(GOLD, "index_1M_voronoi260.npz"))
+    cp = rj.POLY_SETS["voronoi260"]
     xy, off = rj.gen_voronoi(cp["nsites"], seed=cp["seed"], spacing=cp["spacing"], jitter=cp["jitter"])
     pts, _, _, _ = rj.gen_points(int(d["npts"]), params=rj.point_params(seed=int(d["point_seed"])),
                                  columns=False)
@@ -215,7 +215,7 @@ def test_tessellation_covers_each_pixel_center_once():
 def test_pixel_formula_and_plan():
     bbox = np.array([0.0, 0.0, 50000.0, 50000.0])
     p = O.plan(bbox, 10.0)
-    assert (p.W, p.H) == (7072, 7072)          # SURVEY/GL: ceil(50000/(10/sqrt2))
+    assert (p.W, p.H) == (7072, 7072)          # ceil(50000 / (10 / sqrt(2))) pixels a side
     p5 = O.plan(bbox, 5.0)
     assert (p5.W, p5.splitx) == (14144, 2)     # two 7072 tiles per axis
     assert O.pixel(p, 0.0, 0.0) == 0
@@ -323,7 +323,7 @@ def test_boundary_rule_is_a_tight_superset_of_touched_pixels(seed):
 
 @pytest.mark.parametrize("eps", [20.0, 100.0])
 def test_errorbounds_contain_exact_counts(eps):
-    cp = rj.CONFIG_POLYS["voronoi260"]
+    cp = rj.POLY_SETS["voronoi260"]
     xy, off = rj.gen_voronoi(cp["nsites"], seed=cp["seed"], spacing=cp["spacing"], jitter=cp["jitter"])
     pts, _, _, _ = rj.gen_points(300_000, params=rj.point_params(seed=13), columns=False)
     p = O.plan(O.poly_bbox(xy).astype(np.float64), eps)
