@@ -84,3 +84,32 @@ def test_voronoi_cells_share_exact_vertex_chains():
 
 def test_constraint_record_is_16_bytes():
     assert C.sizeof(rj.Constraint) == 16
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes records of the Python binding have the sizes and field
+    offsets of the header's structs, as a C compiler lays them out."""
+    records = {"rj_constraint": rj.Constraint, "rj_table_options": rj.TableOptions, "rj_query": rj.Query,
+               "rj_result": rj.Result, "rj_server_config": rj.ServerConfig, "rj_server_stats": rj.ServerStats}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "rasterjoin.h"', "int main(void) {"]
+    for cname, py in records.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f in py._fields_:
+            fname = f[0]
+            if fname == "val":  # the union of rj_constraint
+                continue
+            lines.append(f'printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0; }"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = dict(l.split() for l in out if l)
+    for cname, py in records.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for f in py._fields_:
+            if f[0] == "val":
+                continue
+            assert int(got[f"{cname}.{f[0]}"]) == getattr(py, f[0]).offset, (cname, f[0])
