@@ -300,3 +300,41 @@ def test_table_order_does_not_change_results(handler):
     handler.set_option(rj.OPT_ORGANIZE, 1)
     for a, b in zip(res[0], res[1]):
         assert np.array_equal(a, b)
+
+
+def jittered_grid_polygons(nx, ny, box=50000.0, seed=4):
+    """nx * ny quadrilaterals tiling the box: a grid whose interior nodes are
+    jittered, each cell one polygon sharing its edges with its neighbours."""
+    rng = np.random.default_rng(seed)
+    gx = np.linspace(0.0, box, nx + 1)
+    gy = np.linspace(0.0, box, ny + 1)
+    X, Y = np.meshgrid(gx, gy, indexing="ij")
+    jx, jy = 0.3 * box / nx, 0.3 * box / ny
+    X[1:-1, 1:-1] += rng.uniform(-jx, jx, (nx - 1, ny - 1))
+    Y[1:-1, 1:-1] += rng.uniform(-jy, jy, (nx - 1, ny - 1))
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    i, j = i.ravel(), j.ravel()
+    cx = np.stack([X[i, j], X[i + 1, j], X[i + 1, j + 1], X[i, j + 1]], 1)
+    cy = np.stack([Y[i, j], Y[i + 1, j], Y[i + 1, j + 1], Y[i, j + 1]], 1)
+    xy = np.stack([cx, cy], 2).astype(np.float32).ravel()
+    off = (np.arange(len(i) + 1) * 4).astype(np.int32)
+    return xy, off
+
+
+def test_many_polygons_32bit_label_maps(handler):
+    """75,000 polygons: more than a 16-bit label map holds, so the raster and
+    hybrid maps are 32-bit; raster against the pixel-center emulation,
+    hybrid and index against CPUJoin."""
+    xy, off = jittered_grid_polygons(300, 250)
+    pts, t, fare, pas = points(500_000, 27)
+    cols = (t, fare, pas)
+    p = O.plan(O.poly_bbox(xy).astype(np.float64), 20.0)
+    oc, ohi, olo = O.raster_join(p, xy, off, O.PointSet(pts, cols, COLS_TYPES, (), agg_type=rj.AVG, agg_col=1))
+    c, _, hi, lo = run(handler, rj.RASTER, xy, off, pts, cols, (), rj.AVG, 1, eps=20.0)
+    assert np.array_equal(c, oc) and np.array_equal(hi, ohi) and np.array_equal(lo, olo)
+    ec, ehi, elo, _ = oracle_exact(xy, off, pts, cols, (), rj.AVG, 1, res=256)
+    for fn in (rj.HYBRID, rj.INDEX):
+        c, _, hi, lo = run(handler, fn, xy, off, pts, cols, (), rj.AVG, 1, eps=20.0, res=256)
+        assert np.array_equal(c, ec), fn
+        assert np.array_equal(hi, ehi) and np.array_equal(lo, elo), fn
+    assert ec.sum() == len(pts) // 2  # the cells tile the box: every point is in one
