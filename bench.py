@@ -68,9 +68,17 @@ def cpu_baseline(xy, off, pts, fare, target_s=10.0, max_n=None):
         O.join_exact(g, ps, nthreads=threads)
         dt = time.perf_counter() - t
         best = dt if best is None else min(best, dt)
+    # one core (CPUJoin's --singleCore, CPUJoin.cpp:84-89) on a tenth of that sample
+    n1 = max(1, min(n, int(rate / threads * target_s / 4)))
+    ps1 = O.PointSet(pts[:2 * n1], [fare[:n1]], [rj.FLOAT], (), agg_type=rj.AVG, agg_col=0)
+    t = time.perf_counter()
+    O.join_exact(g, ps1, nthreads=1)
+    one = time.perf_counter() - t
     return {"value": n / best / 1e9, "unit": "G points/s", "cores": threads, "kind": "port",
             "sample": f"first {n} of the {n_total} points, exact CPUJoin join (grid 1024) with "
-                      f"avg(fare), min of 2 runs, {best:.2f} s"}
+                      f"avg(fare), min of 2 runs, {best:.2f} s",
+            "single_core": {"value": n1 / one / 1e9, "cores": 1,
+                            "sample": f"first {n1} points, one run, {one:.2f} s"}}
 
 
 def timed_steps(h, fn, steps, warmup):
