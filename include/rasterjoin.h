@@ -247,7 +247,8 @@ int rj_canvas_info(rj_ctx* ctx, int32_t* W, int32_t* H);
  *   RJ_OPT_ORGANIZE:        1 (default) rj_set_points orders rows spatially
  *   RJ_OPT_TILE_SHIFT:      label tile side 2^v pixels, v in 3..5; 0 (default): 8 px for
  *                           raster label maps, 16 px for hybrid maps
- *   RJ_OPT_PASS_WORKGROUPS: workgroups of a single-query point pass (0: 4096)
+ *   RJ_OPT_PASS_WORKGROUPS: workgroups of a single-query point pass; 0 (default): one per
+ *                           12288 rows of a spatially ordered table (4096..49152), else 4096
  *   RJ_OPT_BATCH_ROWS:      rows per thread of the fused label-map and index kernels, 4 (default) or 8
  *   RJ_OPT_BATCH_HYBRID_ROWS: rows per thread of the fused hybrid kernel, 4 or 8 (default)
  *   RJ_OPT_BATCH_SHARE:     1 (default) fused raster queries on the same label map (same

@@ -971,6 +971,7 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
     pa.res = res;
     pa.lds_np = lds_direct_cap(avg);
     pa.cap = opt_cap;
+    pa.wg_rows = t->organized ? kPassRowsPerWorkgroup : 0;
     Canvas* C = nullptr;
     Grid* G = nullptr;
     const unsigned long long* check_aux = nullptr;  // overlap counters of a fresh label map

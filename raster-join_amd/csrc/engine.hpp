@@ -168,7 +168,7 @@ struct rj_ctx {
     int hybrid_dilation = 0;
     bool cache = true;
     bool organize = true;
-    int opt_cap = 0;                       // workgroups of a single-query pass (0: 4096)
+    int opt_cap = 0;                       // workgroups of a single-query pass (0: by rows, kPassRowsPerWorkgroup)
     int opt_tshift = 0;  // label tile side 2^tshift (0: 8 px for raster maps, 16 px for hybrid maps)
     int raster_tshift() const { return opt_tshift ? opt_tshift : 3; }
     int hybrid_tshift() const { return opt_tshift ? opt_tshift : rj::kTileShiftDefault; }

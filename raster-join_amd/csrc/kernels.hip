@@ -633,7 +633,10 @@ void launch_points(hipStream_t s, int mode, PointArgs a)
     a.start = a.begin - a.begin % kRows;
     const int64_t n = a.end - a.start;
     const int64_t iter_rows = (int64_t)kBlock * kRows;
-    const int64_t cap = a.cap > 0 ? a.cap : 4096;
+    int64_t cap = a.cap;
+    if (cap <= 0)
+        cap = a.wg_rows > 0 ? std::min<int64_t>(kPassWorkgroupsMax, std::max<int64_t>(kPassWorkgroupsMin, n / a.wg_rows))
+                            : kPassWorkgroupsMin;
     int64_t per = (n + cap - 1) / cap;
     per = (per + iter_rows - 1) / iter_rows * iter_rows;
     a.per_block = per;

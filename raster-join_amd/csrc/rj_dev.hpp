@@ -110,6 +110,13 @@ enum BandMode { BAND_SUM = 0, BAND_LABEL = 1, BAND_BOUNDS = 2, BAND_LABEL_ATOMIC
 // for k_exact instead of testing them itself.
 enum PointMode { PT_INDEX = 0, PT_LABEL = 1, PT_LABEL_HYBRID = 2, PT_SPLAT = 3, PT_HYBRID_QUEUE = 4 };
 
+// Workgroups of a point pass when not set: spatially ordered tables take one
+// workgroup per kPassRowsPerWorkgroup rows within these bounds (short row
+// ranges per workgroup balance the tail; ordered rows touch few polygons, so
+// the per-workgroup flush stays small), other tables the minimum (each
+// workgroup flushes about every polygon).
+constexpr int kPassWorkgroupsMin = 4096, kPassWorkgroupsMax = 49152, kPassRowsPerWorkgroup = 12288;
+
 // Arguments of one point pass.
 struct PointArgs {
     const float* xy;
@@ -143,8 +150,9 @@ struct PointArgs {
     uint32_t* cnt;        // canvas (PT_SPLAT)
     unsigned long long* chi;
     unsigned long long* clo;
-    int cap;              // workgroups of the pass (0: 4096)
-    int pad;
+    int cap;              // workgroups of the pass (0: from wg_rows)
+    int wg_rows;          // cap 0: one workgroup per wg_rows rows, kPassWorkgroupsMin..Max
+                          // (wg_rows 0: kPassWorkgroupsMin)
 };
 
 // ---- fused passes over many queries (batch.hip) ----
