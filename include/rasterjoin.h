@@ -238,7 +238,9 @@ int rj_canvas_info(rj_ctx* ctx, int32_t* W, int32_t* H);
 
 /* Tuning and path selection (no reference counterpart).
  *   RJ_OPT_RASTER_PATH:     0 auto, 1 canvas splat + polygon scan, 2 polygon label map
- *   RJ_OPT_TIMING:          1 records device time per kernel with HIP events
+ *   RJ_OPT_TIMING:          1 records device time per kernel of each execute with HIP events;
+ *                           2 only the kernels that stream the table (point passes, fused
+ *                           passes), totals kept over executes until the option is set again
  *   RJ_OPT_HYBRID_DILATION: border dilation radius in pixels, 0 = derived from the
  *                           float error bound of the exact test (at least 1)
  *   RJ_OPT_CACHE:           1 (default) keeps polygon-side structures (snapped
@@ -272,8 +274,9 @@ enum { RJ_OPT_RASTER_PATH = 1, RJ_OPT_TIMING = 2, RJ_OPT_HYBRID_DILATION = 3, RJ
        RJ_OPT_BATCH_SHARE = 16 };
 int rj_set_option(rj_ctx* ctx, int32_t key, int64_t value);
 
-/* Per-kernel device time of the last execute (timing on): name i, total ms and
- * launches. Returns the number of kernels recorded, or < 0. */
+/* Per-kernel device time of the last execute (timing 1), or of every execute
+ * since the option was set (timing 2): name i, total ms and launches. Returns
+ * the number of kernels recorded, or < 0. */
 int rj_kernel_times(rj_ctx* ctx, int32_t i, char* name, size_t name_len, double* ms, int32_t* launches);
 
 /* Synchronize the context's stream. */

@@ -503,7 +503,12 @@ int rj_set_option(rj_ctx* ctx, int32_t key, int64_t value)
         if (value < 0 || value > 2) return ctx->fail(RJ_ERR_ARG, "raster path is 0, 1 or 2");
         ctx->raster_path = (int)value;
         return RJ_OK;
-    case RJ_OPT_TIMING: ctx->timing = value != 0; return RJ_OK;
+    case RJ_OPT_TIMING:
+        if (value < 0 || value > 2) return ctx->fail(RJ_ERR_ARG, "timing 0, 1 or 2");
+        ctx->timing = value != 0;
+        ctx->timing_pass_only = value == 2;
+        ctx->kstats.clear();
+        return RJ_OK;
     case RJ_OPT_HYBRID_DILATION:
         if (value < 0 || value > 31) return ctx->fail(RJ_ERR_ARG, "dilation radius is 0 (auto) .. 31");
         if (ctx->hybrid_dilation != (int)value)

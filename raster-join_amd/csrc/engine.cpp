@@ -238,7 +238,7 @@ hipEvent_t rj_ctx::event()
 
 void rj_ctx::collect_times(double* pt_ms, double* poly_ms)
 {
-    kstats.clear();
+    if (!timing_pass_only) kstats.clear();
     *pt_ms = *poly_ms = 0;
     for (auto& l : launches) {
         float ms = 0;
@@ -251,7 +251,7 @@ void rj_ctx::collect_times(double* pt_ms, double* poly_ms)
                 found = true;
             }
         if (!found) kstats.push_back({l.name, ms, 1});
-        if (l.name.rfind("points", 0) == 0 || l.name.rfind("splat", 0) == 0 || l.name.rfind("batch", 0) == 0)
+        if (pass_kernel(l.name))
             *pt_ms += ms;
         else
             *poly_ms += ms;
@@ -1759,8 +1759,9 @@ int rj_ctx::autotune(rj_table* t, const QuerySpec* qs_in, int nq, const std::vec
 {
     if (nq <= 0) return fail(RJ_ERR_ARG, "nothing to tune");
     std::vector<QuerySpec> qs(qs_in, qs_in + nq);
-    const bool saved_timing = timing;
+    const bool saved_timing = timing, saved_pass_only = timing_pass_only;
     timing = true;
+    timing_pass_only = false;
     const int base_tshift = opt_tshift, base_rows = opt_batch_rows, base_wg = opt_batch_wg,
               base_chunks = opt_batch_chunk_rows, base_res = opt_auto_res;
     std::vector<QueryOut> ref(nq), cur(nq);
@@ -1858,6 +1859,7 @@ int rj_ctx::autotune(rj_table* t, const QuerySpec* qs_in, int nq, const std::vec
         opt_auto_res = bestc[4];
     }
     timing = saved_timing;
+    timing_pass_only = saved_pass_only;
     if (rc) {
         opt_tshift = base_tshift;
         opt_batch_hybrid_rows = base_hrows;

@@ -165,6 +165,7 @@ struct rj_ctx {
     // options
     int raster_path = 0;
     bool timing = false;
+    bool timing_pass_only = false;  // timing 2: point passes only, totals kept across executes
     int hybrid_dilation = 0;
     bool cache = true;
     bool organize = true;
@@ -262,11 +263,17 @@ struct rj_ctx {
     int upload(rj::DevBuf& b, const void* src, size_t bytes, const char* what);
     int pinned(void** p, size_t* cur, size_t bytes, const char* what);
     hipEvent_t event();
+    // the kernels that stream the table (point passes, splats, fused passes)
+    static bool pass_kernel(const std::string& name)
+    {
+        return name.rfind("points", 0) == 0 || name.rfind("splat", 0) == 0 || name.rfind("batch", 0) == 0;
+    }
     template <class F>
     int launch(const char* name, F f)
     {
         hipEvent_t a = nullptr, b = nullptr;
-        if (timing) {
+        const bool timed = timing && (!timing_pass_only || pass_kernel(name));
+        if (timed) {
             a = event();
             b = event();
             (void)hipEventRecord(a, stream);
@@ -274,7 +281,7 @@ struct rj_ctx {
         f();
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, name);
-        if (timing) {
+        if (timed) {
             (void)hipEventRecord(b, stream);
             launches.push_back({name, a, b});
         }
