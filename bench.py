@@ -39,6 +39,10 @@ import rasterjoin as rj  # noqa: E402
 
 METRIC = "G points joined/s at ε=10m + served queries/s with p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32
+# one crossing-number edge test (isInsidePoly): two compares and their xor,
+# two subtracts, a multiply, a divide, an add and a compare
+FLOPS_PER_EDGE_TEST = 8
 T0 = 1230768000
 YEAR = 31536000
 TYPES = (rj.UINT, rj.FLOAT, rj.UINT8)   # time, fare, passengers
@@ -79,6 +83,79 @@ def cpu_baseline(xy, off, pts, fare, target_s=10.0, max_n=None):
                       f"avg(fare), min of 2 runs, {best:.2f} s",
             "single_core": {"value": n1 / one / 1e9, "cores": 1,
                             "sample": f"first {n1} points, one run, {one:.2f} s"}}
+
+
+def cell_edge_work(xy, off, res):
+    """Edge tests per grid cell of the reference's index join: a point in a
+    cell tests every candidate polygon of the cell edge by edge (GridIndexF::
+    getRegion, isInsidePoly over all n_v edges). A polygon is a candidate of
+    the cells of its bounding-box range that it meets (GridIndexF::buildGrid:
+    a vertex in the cell, a cell corner inside the polygon, or an edge through
+    the cell), on res x res cells over the polygons' bbox."""
+    x, y = xy[0::2].astype(np.float32), xy[1::2].astype(np.float32)
+    minx, miny = np.float32(x.min()), np.float32(y.min())
+    w = np.float32((np.float32(x.max()) - minx) / np.float32(res))
+    hh = np.float32((np.float32(y.max()) - miny) / np.float32(res))
+    work = np.zeros((res, res), np.int64)
+    for p in range(len(off) - 1):
+        px, py = x[off[p]:off[p + 1]], y[off[p]:off[p + 1]]
+        n = len(px)
+        if n < 3:
+            continue
+        qx, qy = np.roll(px, -1), np.roll(py, -1)
+        x0, x1 = max(0, int((px.min() - minx) / w)), min(res - 1, int((px.max() - minx) / w) + 1)
+        y0, y1 = max(0, int((py.min() - miny) / hh)), min(res - 1, int((py.max() - miny) / hh) + 1)
+        hit = np.zeros((y1 - y0 + 1, x1 - x0 + 1), bool)
+        # cell corners inside the polygon (crossing number along each grid line)
+        gx = np.arange(x0, x1 + 2, dtype=np.float32) * w + minx
+        gy = (np.arange(y0, y1 + 2, dtype=np.float32) * hh + miny)[:, None]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            xc = (qx - px)[None, :] * (gy - py[None, :]) / (qy - py)[None, :] + px[None, :]
+        xc = np.where((py[None, :] > gy) != (qy[None, :] > gy), xc, -np.inf)
+        inside = ((xc[:, None, :] > gx[None, :, None]).sum(-1) & 1).astype(bool)
+        hit |= inside[:-1, :-1] | inside[1:, 1:] | inside[:-1, 1:] | inside[1:, :-1]
+        # cells holding a vertex
+        hit[np.clip(((py - miny) / hh).astype(np.int64), y0, y1) - y0,
+            np.clip(((px - minx) / w).astype(np.int64), x0, x1) - x0] = True
+        # cells an edge passes through: in the edge's bbox, with the cell's
+        # corners not all strictly on one side of the edge's line
+        ex0 = np.clip(((np.minimum(px, qx) - minx) / w).astype(np.int64), x0, x1)
+        ex1 = np.clip(((np.maximum(px, qx) - minx) / w).astype(np.int64), x0, x1)
+        ey0 = np.clip(((np.minimum(py, qy) - miny) / hh).astype(np.int64), y0, y1)
+        ey1 = np.clip(((np.maximum(py, qy) - miny) / hh).astype(np.int64), y0, y1)
+        cnt = (ex1 - ex0 + 1) * (ey1 - ey0 + 1)
+        e = np.repeat(np.arange(n), cnt)
+        k = np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+        wd = (ex1 - ex0 + 1)[e]
+        cx, cy = ex0[e] + k % wd, ey0[e] + k // wd
+        lx, rx = cx.astype(np.float32) * w + minx, (cx + 1).astype(np.float32) * w + minx
+        ly, ry = cy.astype(np.float32) * hh + miny, (cy + 1).astype(np.float32) * hh + miny
+        ax, ay, dx, dy = px[e], py[e], (qx - px)[e], (qy - py)[e]
+        side = [dx * (v - ay) - dy * (u - ax) for u, v in ((lx, ly), (rx, ry), (lx, ry), (rx, ly))]
+        one_side = ((side[0] > 0) & (side[1] > 0) & (side[2] > 0) & (side[3] > 0)) | \
+                   ((side[0] < 0) & (side[1] < 0) & (side[2] < 0) & (side[3] < 0))
+        hit[cy[~one_side] - y0, cx[~one_side] - x0] = True
+        work[y0:y1 + 1, x0:x1 + 1] += np.where(hit, n, 0)
+    return work, (minx, miny, w, hh)
+
+
+def edge_tests(cell_work, pts):
+    """Sum of the edge tests of every point (cell_edge_work)."""
+    work, (minx, miny, w, hh) = cell_work
+    res = work.shape[0]
+    with np.errstate(invalid="ignore"):
+        cx = np.floor((pts[0::2] - minx) / w)
+        cy = np.floor((pts[1::2] - miny) / hh)
+    ok = (cx >= 0) & (cx < res) & (cy >= 0) & (cy < res)
+    return int(work[cy[ok].astype(np.int64), cx[ok].astype(np.int64)].sum())
+
+
+def valu_roofline(kernel, tests, ms):
+    ach = tests * FLOPS_PER_EDGE_TEST / (ms * 1e-3) / 1e12
+    return {"kernel": kernel, "bound": "valu", "edge_tests": int(tests),
+            "G_edge_tests_per_s": round(tests / (ms * 1e-3) / 1e9, 2), "flops_per_edge_test": FLOPS_PER_EDGE_TEST,
+            "achieved": round(ach, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(ach / FP32_PEAK_TFLOPS, 5), "ms": round(ms, 4)}
 
 
 def timed_steps(h, fn, steps, warmup):
@@ -220,6 +297,13 @@ def workloads_section(h, pts, cols, args):
         cfg1[name] = {"ms_min": round(min(ts) * 1e3, 3), "ms_median": round(float(np.median(ts)) * 1e3, 3),
                       "G_points_per_s": round(n1 / min(ts) / 1e9, 3)}
     h.set_option(rj.OPT_INDEX_PATH, 0)
+    # the reference's per-point edge tests (cell_edge_work)
+    tests = edge_tests(cell_edge_work(xy, off, 256), pts[:2 * n1])
+    cfg1["ref_edge_tests"] = tests
+    for name in ("via_labels", "grid_pip"):
+        cfg1[name]["G_ref_edge_tests_per_s"] = round(tests / (cfg1[name]["ms_min"] * 1e-3) / 1e9, 2)
+    cfg1["valu_roofline"] = valu_roofline("index join (whole execute, via labels)", tests,
+                                          cfg1["via_labels"]["ms_min"])
     out["index_1M"] = cfg1
     t1.close()
 
@@ -249,6 +333,17 @@ def workloads_section(h, pts, cols, args):
     alg = n3 * 17 + 8192 * 8192 * 2
     ach = alg / (k_ms * 1e-3) / 1e9
     h.set_option(rj.OPT_TIMING, 0)
+    # the border pass: queued points tested against the edges of their cell's
+    # candidates, priced at the reference's per-point edge tests (averaged over
+    # a sample of the rows)
+    border = h.exact_points()
+    sample = min(n3, 5_000_000)
+    per_pt = edge_tests(cell_edge_work(txy, toff, 1024), pts[:2 * sample]) / sample
+    exact = None
+    if border > 0 and "exact_queue" in ktot:
+        e_ms = ktot["exact_queue"][0] / ktot["exact_queue"][1]
+        exact = valu_roofline("exact_queue", border * per_pt, e_ms)
+        exact["border_points"] = border
     out["hybrid_500M_filters"] = {
         "workload": f"hybrid eps=20 m, count()+avg(fare), {n3} points (Morton order), time window 30% "
                     "and passengers<=2, 2200 tract polygons, grid 1024, polygon structures cached",
@@ -257,7 +352,8 @@ def workloads_section(h, pts, cols, args):
         "kernels_ms": kms,
         "roofline": {"kernel": kname, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "kernel_ms": round(k_ms, 4),
-                     "alg_bytes": int(alg)}}
+                     "alg_bytes": int(alg)},
+        "border_roofline": exact}
     t3.close()
     return out
 

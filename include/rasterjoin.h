@@ -279,6 +279,13 @@ int rj_set_option(rj_ctx* ctx, int32_t key, int64_t value);
  * the number of kernels recorded, or < 0. */
 int rj_kernel_times(rj_ctx* ctx, int32_t i, char* name, size_t name_len, double* ms, int32_t* launches);
 
+/* Points the last rj_execute queued for the separate exact pass (the border
+ * points of a hybrid or label-accelerated index join that the exact kernel
+ * tests against the polygons' edges), or -1 when it queued none (other join
+ * types, the exact test inline). A measurement aid with no reference
+ * counterpart: the work of the border pass. */
+int64_t rj_exact_points(rj_ctx* ctx);
+
 /* Synchronize the context's stream. */
 int rj_sync(rj_ctx* ctx);
 

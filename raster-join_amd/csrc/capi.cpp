@@ -683,6 +683,12 @@ int rj_kernel_times(rj_ctx* ctx, int32_t i, char* name, size_t name_len, double*
     return (int)ctx->kstats.size();
 }
 
+int64_t rj_exact_points(rj_ctx* ctx)
+{
+    RJ_GUARD(ctx);
+    return ctx->last_exact;
+}
+
 int rj_sync(rj_ctx* ctx)
 {
     RJ_GUARD(ctx);

@@ -977,6 +977,7 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
     const unsigned long long* check_aux = nullptr;  // overlap counters of a fresh label map
     const uint32_t* check_queue = nullptr;          // border-queue count of a hybrid pass
     out->path = RJ_PATH_SINGLE;
+    last_exact = -1;
 
     auto canvas_pass = [&](bool sums) -> int {
         int r2 = ensure_canvas_buffers(C->plan, sums);
@@ -1096,10 +1097,12 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
         e = hipMemcpyAsync(h_res + res_words + 2, check_queue, 4, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "execute");
+    if (check_queue) last_exact = (uint32_t)h_res[res_words + 2];
     if (check_queue && re0 - rb0 >= 65536)  // remember the border share for the next queue
         C->hyb_border = std::max(0.01, (double)(uint32_t)h_res[res_words + 2] / (double)(re0 - rb0));
     if (check_queue && (uint32_t)h_res[res_words + 2] > pa.qcap) {
         // the queue overflowed: the whole pass again with the exact test inline
+        last_exact = -1;
         PointArgs ia = pa;
         ia.queue = nullptr;
         ia.qcount = nullptr;

@@ -166,6 +166,7 @@ struct rj_ctx {
     int raster_path = 0;
     bool timing = false;
     bool timing_pass_only = false;  // timing 2: point passes only, totals kept across executes
+    int64_t last_exact = -1;        // points the last single query queued for the exact pass
     int hybrid_dilation = 0;
     bool cache = true;
     bool organize = true;

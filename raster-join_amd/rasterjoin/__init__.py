@@ -130,6 +130,8 @@ def lib():
         L.rj_last_error.restype = C.c_char_p
         L.rj_polys_free.argtypes = [vp]
         L.rj_polys_free.restype = None
+        L.rj_exact_points.argtypes = [vp]
+        L.rj_exact_points.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -485,6 +487,11 @@ class Handler:
             lib().rj_kernel_times(self.ctx, i, name, 64, C.byref(ms), C.byref(n))
             out[name.value.decode()] = (ms.value, n.value)
         return out
+
+    def exact_points(self):
+        """Points the last execute queued for the separate exact pass (border
+        points of hybrid and label-accelerated index joins), or -1."""
+        return int(lib().rj_exact_points(self.ctx))
 
     def sync(self):
         self._check(lib().rj_sync(self.ctx))

@@ -1,6 +1,7 @@
-"""Kernel timing (RJ_OPT_TIMING): 1 records every kernel of the last execute;
-2 records only the kernels that stream the table, summed over every execute
-since the option was set, and leaves the results unchanged."""
+"""Measurement aids. Kernel timing (RJ_OPT_TIMING): 1 records every kernel
+of the last execute; 2 records only the kernels that stream the table, summed
+over every execute since the option was set, and leaves the results
+unchanged. rj_exact_points: the size of the last execute's border queue."""
 import numpy as np
 import pytest
 
@@ -40,3 +41,22 @@ def test_timing_modes(handler):
     finally:
         h.set_option(rj.OPT_TIMING, 0)
         h.set_option(rj.OPT_CACHE, 1)
+
+
+def test_exact_points_counts_the_border_queue(handler):
+    """rj_exact_points: the points a hybrid join queued for the exact pass
+    (some, never all, of the selected rows); -1 after a raster join."""
+    h = handler
+    xy, off = polys("voronoi260")
+    pts = points(400_000, 4)[0]
+    h.set_polygons(xy, off)
+    h.set_points(pts, (), ())
+    h.set_query_constraints(())
+    h.set_aggregation(rj.COUNT, -1)
+    h.set_accuracy_distance(20.0)
+    h.set_poly_index_resolution(256)
+    c, _ = h.execute_function(rj.HYBRID)
+    k = h.exact_points()
+    assert 0 < k < c.sum()
+    h.execute_function(rj.RASTER)
+    assert h.exact_points() == -1
