@@ -247,6 +247,32 @@ def profiled_traffic():
         return json.load(f).get("traffic_bytes")
 
 
+def stream_copy_gbs(gib=2, reps=10):
+    """The device's achievable copy rate (bytes read + written per second,
+    best of `reps` copies of `gib` GiB), timed with events on torch's stream;
+    None where torch has no device."""
+    try:
+        import torch
+        if not torch.cuda.is_available():
+            return None
+        src = torch.ones((gib << 30) // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        best = float("inf")
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            dst.copy_(src)
+            b.record()
+            b.synchronize()
+            best = min(best, a.elapsed_time(b))
+        del src, dst
+        torch.cuda.empty_cache()
+        return 2 * (gib << 30) / (best * 1e-3) / 1e9
+    except Exception as e:  # reported, not fatal: the copy rate is context
+        log("stream copy not measured:", e)
+        return None
+
+
 def count_1b(h, table, xy, off, args):
     """Raster count at eps = 10 m over every row of the 1B table."""
     n = table.n
@@ -264,12 +290,15 @@ def count_1b(h, table, xy, off, args):
     alg = n * 8 + LW * H * 2
     ach = alg / (k_ms * 1e-3) / 1e9
     h.set_option(rj.OPT_TIMING, 0)
+    copy = args.stream_copy
     return {"workload": f"raster count eps=10 m over {n} points (Morton-ordered location table), "
                         "260 polygons, label map cached",
             "G_points_per_s": round(n * steps / dt / 1e9, 2), "ms_per_join": round(dt / steps * 1e3, 3),
             "roofline": {"kernel": "points_label", "bound": "hbm", "achieved": round(ach, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                         "kernel_ms": round(k_ms, 4), "alg_bytes": int(alg)}}
+                         "kernel_ms": round(k_ms, 4), "alg_bytes": int(alg)},
+            "stream_copy_GBps": None if copy is None else round(copy, 1),
+            "frac_of_stream_copy": None if copy is None else round(ach / copy, 4)}
 
 
 def workloads_section(h, pts, cols, args):
@@ -666,6 +695,11 @@ def main():
     sections = set(args.sections.split(","))
     if args.headline_only:
         sections = {"headline"}
+    # the device's copy rate, context for the 1B count's roofline (measured
+    # before the library touches the device)
+    args.stream_copy = stream_copy_gbs() if "count" in sections else None
+    if args.stream_copy:
+        log(f"device copy: {args.stream_copy:.0f} GB/s (read + write)")
     big = max(args.big, 0) if sections & {"count", "workloads", "batch", "serving", "stream"} else 0
     nrows = max(big, args.points)
     cp = rj.POLY_SETS["voronoi260"]
