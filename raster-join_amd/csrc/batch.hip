@@ -102,14 +102,7 @@ __device__ __forceinline__ void member_rows(const PointArgs& a, int64_t r0, int6
             uint32_t v[R];
             const int col = q.slot_col[s];
             load_rows<R>(a.cols, col, i0, full, a.end, v);
-            const int t = a.cols.type[col];
-            for (int k = 0; k < q.ncons; k++) {
-                if (q.con_slot[k] != s) continue;
-                const int op = q.con_op[k];
-                const uint32_t cv = q.con_val[k];
-#pragma unroll
-                for (int r = 0; r < R; r++) ok[r] = ok[r] && cmp_bits(t, op, v[r], cv);
-            }
+            slot_filter<R>(a.cols.type[col], q.slot_lo[s], q.slot_hi[s], v, ok);
             if (AVG && s == q.agg_slot) {
 #pragma unroll
                 for (int r = 0; r < R; r++) vraw[r] = v[r];

@@ -702,6 +702,31 @@ DevGrid rj_ctx::dev_grid(const Grid* G) const
     return g;
 }
 
+// Key range (col_key) of the values of a column type that satisfy `value op
+// c`, the comparison of index.glsl:184-208 / points.vert:138-149 with typed
+// operands (Float as IEEE floats: NaN satisfies nothing, -0 == +0). Returns
+// false when no value does.
+static bool constraint_range(int type, int op, uint32_t c, uint32_t* lo, uint32_t* hi)
+{
+    uint32_t kmin = 0, kmax = UINT32_MAX, kl = col_key(type, c), ku = kl;
+    if (type == RJ_ATTR_FLOAT) {
+        kmin = col_key(type, 0xFF800000u);  // -inf
+        kmax = col_key(type, 0x7F800000u);  // +inf
+        if ((c & 0x7FFFFFFFu) > 0x7F800000u) return false;  // NaN constant
+        if ((c & 0x7FFFFFFFu) == 0) {  // either zero: both zeros
+            kl = col_key(type, 0x80000000u);
+            ku = col_key(type, 0u);
+        }
+    }
+    switch (op) {
+    case RJ_EQ: *lo = kl; *hi = ku; return true;
+    case RJ_LT: *lo = kmin; *hi = kl - 1; return kl > kmin;
+    case RJ_LTE: *lo = kmin; *hi = ku; return true;
+    case RJ_GT: *lo = ku + 1; *hi = kmax; return ku < kmax;
+    default: *lo = kl; *hi = kmax; return true;  // RJ_GTE
+    }
+}
+
 // Filters and aggregate of a query on a table's columns (GLFunction::setupPoints,
 // GLFunction.cpp:228-323: the columns a query reads, at most 5; at most 10
 // constraints).
@@ -717,18 +742,30 @@ int rj_ctx::build_query(const rj_table* t, const QuerySpec& s, DevQuery* q)
         q->slot_col[q->nslots] = col;
         return q->nslots++;
     };
+    for (int k = 0; k < kMaxSlots; k++) {
+        q->slot_lo[k] = 0;
+        q->slot_hi[k] = UINT32_MAX;
+    }
     for (size_t k = 0; k < s.cons.size(); k++) {
         const rj_constraint& c = s.cons[k];
         if (c.attrib_id < 0 || c.attrib_id >= t->ncols)
             return fail(RJ_ERR_ARG, "constraint column " + std::to_string(c.attrib_id) + " out of range");
         if (c.type < RJ_EQ || c.type > RJ_GTE) return fail(RJ_ERR_ARG, "bad constraint type");
-        q->con_col[k] = c.attrib_id;
-        q->con_op[k] = c.type;
-        q->con_val[k] = c.val.uval;
-        q->con_slot[k] = slot_of(c.attrib_id);
-        if (q->con_slot[k] < 0) return fail(RJ_ERR_LIMIT, "a query may use at most 5 attributes");
+        const int slot = slot_of(c.attrib_id);
+        if (slot < 0) return fail(RJ_ERR_LIMIT, "a query may use at most 5 attributes");
+        uint32_t lo, hi;
+        if (constraint_range(t->types[c.attrib_id], c.type, c.val.uval, &lo, &hi)) {
+            q->slot_lo[slot] = std::max(q->slot_lo[slot], lo);
+            q->slot_hi[slot] = std::min(q->slot_hi[slot], hi);
+        } else {
+            q->slot_lo[slot] = 1;  // no value passes
+            q->slot_hi[slot] = 0;
+        }
     }
     q->ncons = (int)s.cons.size();
+    q->plain = 1;
+    for (int k = 0; k < q->nslots; k++)
+        if (q->slot_lo[k] != 0 || q->slot_hi[k] != UINT32_MAX) q->plain = 0;
     q->agg_type = s.agg_type;
     q->agg_col = 0;
     if (s.agg_type != RJ_AGG_COUNT && s.fn != RJ_ERRORBOUNDS) {

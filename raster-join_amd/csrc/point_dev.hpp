@@ -30,20 +30,21 @@ __device__ __forceinline__ int64_t ceil_div(int64_t a, int64_t b)
     return q;
 }
 
-template <class T>
-__device__ __forceinline__ T cmp_op(int op, T a, T b)
+// The constraints of one slot on R rows: ok[r] stays true when the row's
+// value lies in the slot's key range (DevQuery). Typed comparisons: Float
+// columns compare as floats, Int as signed and Uint (and u8) as unsigned
+// integers; the reference's float-bit equality on LTE/GTE
+// (points.vert:99,107) is not kept. A slot without constraints (its whole
+// key range) is skipped.
+template <int R>
+__device__ __forceinline__ void slot_filter(int type, uint32_t lo, uint32_t hi, const uint32_t (&v)[R], bool (&ok)[R])
 {
-    return op == RJ_EQ ? a == b : op == RJ_LT ? a < b : op == RJ_LTE ? a <= b : op == RJ_GT ? a > b : a >= b;
-}
-
-// Typed comparison of 32-bit column patterns: Float columns compare as
-// floats, Int as signed and Uint (and u8) as unsigned integers; the
-// reference's float-bit equality on LTE/GTE (points.vert:99,107) is not kept.
-__device__ __forceinline__ bool cmp_bits(int type, int op, uint32_t a, uint32_t v)
-{
-    if (type == RJ_ATTR_FLOAT) return cmp_op<float>(op, __uint_as_float(a), __uint_as_float(v));
-    if (type == RJ_ATTR_INT) return cmp_op<int32_t>(op, (int32_t)a, (int32_t)v);
-    return cmp_op<uint32_t>(op, a, v);
+    if (lo == 0u && hi == 0xFFFFFFFFu) return;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t k = col_key(type, v[r]);
+        ok[r] = ok[r] & (k >= lo) & (k <= hi);
+    }
 }
 
 // Exact fixed-point aggregate value of a 32-bit pattern: hi = floor(v),
@@ -571,20 +572,18 @@ __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, in
         uint32_t vraw[R];
 #pragma unroll
         for (int r = 0; r < R; r++) vraw[r] = 0;
-#pragma unroll
+        if (q.plain) {
+            // no constraints: at most the aggregate column
+            if (AVG) load_rows<R>(a.cols, q.slot_col[q.agg_slot], i0, full, a.end, vraw);
+        } else
+        // one slot at a time: fewer registers than every slot's loads in flight
+#pragma unroll 1
         for (int s = 0; s < kMaxSlots; s++) {
             if (s >= q.nslots) break;
             uint32_t v[R];
             const int col = q.slot_col[s];
             load_rows<R>(a.cols, col, i0, full, a.end, v);
-            const int t = a.cols.type[col];
-            for (int k = 0; k < q.ncons; k++) {
-                if (q.con_slot[k] != s) continue;
-                const int op = q.con_op[k];
-                const uint32_t cv = q.con_val[k];
-#pragma unroll
-                for (int r = 0; r < R; r++) ok[r] = ok[r] && cmp_bits(t, op, v[r], cv);
-            }
+            slot_filter<R>(a.cols.type[col], q.slot_lo[s], q.slot_hi[s], v, ok);
             if (AVG && s == q.agg_slot) {
 #pragma unroll
                 for (int r = 0; r < R; r++) vraw[r] = v[r];

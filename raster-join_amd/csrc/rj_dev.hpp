@@ -30,19 +30,31 @@ struct DevCols {
 
 constexpr int kMaxSlots = 5;  // distinct columns one query may read (GLFunction.cpp:250-253)
 
+// Order-preserving 32-bit key of a typed column value: a < b as the type
+// compares them (Uint and u8 unsigned, Int signed, Float as IEEE floats)
+// iff key(a) < key(b) unsigned. Float NaNs fall outside [key(-inf),
+// key(+inf)] (negative NaNs below, positive above); -0 and +0 are adjacent.
+__host__ __device__ inline uint32_t col_key(int type, uint32_t bits)
+{
+    if (type == RJ_ATTR_FLOAT) return (bits & 0x80000000u) ? ~bits : (bits | 0x80000000u);
+    if (type == RJ_ATTR_INT) return bits ^ 0x80000000u;
+    return bits;
+}
+
 // Filters and aggregate of one query. The columns it reads are numbered as
-// slots; every constraint and the aggregate name their slot.
+// slots; the aggregate names its slot. The constraints on a slot's column
+// (a conjunction) become one key range: a row passes when slot_lo[s] <=
+// col_key(type, value) <= slot_hi[s] for every slot (lo > hi: no row).
 struct DevQuery {
     int32_t ncons;
     int32_t agg_type;  // RJ_AGG_*
     int32_t agg_col;
     int32_t nslots;
-    int32_t con_col[RJ_MAX_CONSTRAINTS];
-    int32_t con_op[RJ_MAX_CONSTRAINTS];
-    uint32_t con_val[RJ_MAX_CONSTRAINTS];
-    int32_t con_slot[RJ_MAX_CONSTRAINTS];
     int32_t slot_col[kMaxSlots];
+    uint32_t slot_lo[kMaxSlots];
+    uint32_t slot_hi[kMaxSlots];
     int32_t agg_slot;  // -1 for count
+    int32_t plain;     // 1: no slot has constraints
 };
 
 // Canvas geometry (same numbers as the host plan).

@@ -1,0 +1,105 @@
+"""Attribute filters (QueryConstraint, GLFunction.cpp:250-323) on every
+column type and comparison, with the values where typed comparisons differ
+from bit comparisons: NaN, signed zeros, infinities, denormals, the integer
+extremes. Index-join counts (exact membership, so the filter alone decides)
+against the oracle's CPUJoin restatement, which compares as
+index.glsl:184-208 does; single queries and one fused batch."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+import rasterjoin as rj
+from common import points, polys
+
+pytestmark = pytest.mark.gpu
+N = 200_000
+TYPES = (rj.UINT, rj.FLOAT, rj.INT, rj.UINT8)
+F32_SPECIAL = [np.nan, -np.nan, np.inf, -np.inf, 0.0, -0.0, 1e-45, -1e-45, 1.1754944e-38,
+               -1.1754944e-38, 3.4028235e38, -3.4028235e38, 1.0, -1.0, 2.5, 7.25]
+U32_SPECIAL = [0, 1, 2, 0x7FFFFFFF, 0x80000000, 0xFFFFFFFE, 0xFFFFFFFF, 1234567]
+I32_SPECIAL = [0, 1, -1, -2147483648, -2147483647, 2147483647, 2147483646, -77]
+U8_SPECIAL = [0, 1, 2, 127, 128, 254, 255, 256, 1000]
+
+
+def columns(seed):
+    rng = np.random.default_rng(seed)
+    u = rng.integers(0, 1 << 32, N, dtype=np.uint64).astype(np.uint32)
+    pick = rng.random(N) < 0.3
+    u[pick] = rng.choice(np.array(U32_SPECIAL, np.uint32), pick.sum())
+    f = rng.normal(0, 3, N).astype(np.float32)
+    pick = rng.random(N) < 0.4
+    f[pick] = rng.choice(np.array(F32_SPECIAL, np.float32), pick.sum())
+    i = rng.integers(-(1 << 31), 1 << 31, N, dtype=np.int64).astype(np.int32)
+    pick = rng.random(N) < 0.3
+    i[pick] = rng.choice(np.array(I32_SPECIAL, np.int32), pick.sum())
+    b = rng.integers(0, 256, N).astype(np.uint8)
+    return u, f, i, b
+
+
+def constraint_sets():
+    ops = (rj.EQ, rj.LT, rj.LTE, rj.GT, rj.GTE)
+    out = []
+    for op in ops:
+        out += [[(0, op, v)] for v in U32_SPECIAL]
+        out += [[(1, op, float(v))] for v in F32_SPECIAL]
+        out += [[(2, op, v)] for v in I32_SPECIAL]
+        out += [[(3, op, v)] for v in U8_SPECIAL]
+    # several constraints on one column (a range, an empty range, contradictory
+    # equalities) and on several columns
+    out += [
+        [(1, rj.GT, -1.0), (1, rj.LTE, 2.5)],
+        [(1, rj.GTE, 0.0), (1, rj.LTE, -0.0)],
+        [(1, rj.GT, 0.0), (1, rj.LT, 1e-45)],
+        [(1, rj.GTE, -np.inf), (1, rj.LTE, np.inf)],
+        [(2, rj.GT, -2147483648), (2, rj.LT, 2147483647), (2, rj.EQ, -77)],
+        [(2, rj.EQ, 1), (2, rj.EQ, -1)],
+        [(0, rj.GTE, 0x80000000), (0, rj.LT, 0xFFFFFFFF)],
+        [(3, rj.GTE, 2), (3, rj.LTE, 127), (0, rj.GT, 1), (1, rj.LT, 7.25), (2, rj.GTE, -1)],
+        [(3, rj.GT, 255)],
+        [(0, rj.LT, 0)],
+    ]
+    return out
+
+
+@pytest.fixture(scope="module")
+def data():
+    pts = points(N, 21)[0]
+    return pts, columns(5)
+
+
+def oracle_counts(g, pts, cols, cons):
+    return O.join_exact(g, O.PointSet(pts, cols, TYPES, cons))[0]
+
+
+def test_filters_single_queries(handler, data):
+    pts, cols = data
+    xy, off = polys("voronoi20")
+    g = O.Grid(xy, off, 64)
+    h = handler
+    h.set_polygons(xy, off)
+    h.set_points(pts, cols, TYPES)
+    h.set_aggregation(rj.COUNT, -1)
+    h.set_poly_index_resolution(64)
+    for path in (0, 1):
+        h.set_option(rj.OPT_INDEX_PATH, path)
+        for cons in constraint_sets():
+            h.set_query_constraints(cons)
+            c, _ = h.execute_function(rj.INDEX)
+            assert np.array_equal(c, oracle_counts(g, pts, cols, cons)), (path, cons)
+    h.set_option(rj.OPT_INDEX_PATH, 0)
+
+
+def test_filters_fused_batch(handler, data):
+    pts, cols = data
+    xy, off = polys("voronoi20")
+    g = O.Grid(xy, off, 64)
+    h = handler
+    table = rj.Table(h, pts, cols, TYPES)
+    ps = rj.PolySet(h, xy, off)
+    sets = constraint_sets()[::3]
+    qs = [rj.make_query(ps, rj.INDEX, rj.COUNT, -1, cons, index_res=64) for cons in sets]
+    for k in range(0, len(qs), 48):
+        res = h.execute_batch(table, qs[k:k + 48], [ps.np] * len(qs[k:k + 48]))
+        for cons, r in zip(sets[k:k + 48], res):
+            assert r.status == 0
+            assert np.array_equal(r.counts, oracle_counts(g, pts, cols, cons)), cons
