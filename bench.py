@@ -163,14 +163,19 @@ def timed_steps(h, fn, steps, warmup):
     region only the kernels that stream the table carry HIP events (timing 2,
     totals over the steps); the other kernels are timed afterwards over a few
     more executes (timing 1) and scaled to `steps`. Returns the wall time of
-    the timed region and {kernel: [total ms, launches]} over `steps`."""
+    the timed region, {kernel: [total ms, launches]} over `steps` and the
+    min and median wall time of one execute (each returns its results, so
+    each is synchronous), in ms."""
     for _ in range(warmup):
         h.execute_function(fn)
     h.set_option(rj.OPT_TIMING, 2)
     h.sync()
+    per = []
     t0 = time.perf_counter()
     for _ in range(steps):
+        t1 = time.perf_counter()
         h.execute_function(fn)
+        per.append(time.perf_counter() - t1)
     h.sync()
     dt = time.perf_counter() - t0
     ktot = {k: [ms, cnt] for k, (ms, cnt) in h.kernel_times().items()}
@@ -186,7 +191,7 @@ def timed_steps(h, fn, steps, warmup):
                 a[1] += cnt
     for k, (ms, cnt) in other.items():
         ktot[k] = [ms * steps / extra, cnt * steps // extra]
-    return dt, ktot
+    return dt, ktot, {"step_ms_min": round(min(per) * 1e3, 4), "step_ms_median": round(float(np.median(per)) * 1e3, 4)}
 
 
 def headline(h, pts, cols, xy, off, args):
@@ -206,7 +211,7 @@ def headline(h, pts, cols, xy, off, args):
     h.set_option(rj.OPT_CACHE, 0)
     h.set_option(rj.OPT_TIMING, 1)
     W, H = h.canvas_size()
-    dt, ktot = timed_steps(h, rj.RASTER, args.steps, args.warmup)
+    dt, ktot, per = timed_steps(h, rj.RASTER, args.steps, args.warmup)
     counts, _ = h.execute_function(rj.RASTER)
     path = "label" if "points_label" in ktot else "canvas"
     log("kernels (ms total, launches):", {k: (round(v[0], 3), v[1]) for k, v in ktot.items()})
@@ -234,6 +239,7 @@ def headline(h, pts, cols, xy, off, args):
                      "traffic": profiled_traffic(), "kernel_ms": round(k_ms, 4),
                      "alg_bytes": int(alg_bytes)},
         "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in ktot.items()},
+        "step_ms": per,
     }
 
 
@@ -283,7 +289,7 @@ def count_1b(h, table, xy, off, args):
     h.set_accuracy_distance(10.0)
     h.set_option(rj.OPT_TIMING, 1)
     W, H = h.canvas_size()
-    dt, ktot = timed_steps(h, rj.RASTER, max(3, args.steps // 2), 1)
+    dt, ktot, per = timed_steps(h, rj.RASTER, max(3, args.steps // 2), 1)
     steps = max(3, args.steps // 2)
     k_ms = ktot["points_label"][0] / ktot["points_label"][1]
     LW = (W + 31) // 32 * 32
@@ -293,7 +299,7 @@ def count_1b(h, table, xy, off, args):
     copy = args.stream_copy
     return {"workload": f"raster count eps=10 m over {n} points (Morton-ordered location table), "
                         "260 polygons, label map cached",
-            "G_points_per_s": round(n * steps / dt / 1e9, 2), "ms_per_join": round(dt / steps * 1e3, 3),
+            "G_points_per_s": round(n * steps / dt / 1e9, 2), "ms_per_join": round(dt / steps * 1e3, 3), **per,
             "roofline": {"kernel": "points_label", "bound": "hbm", "achieved": round(ach, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                          "kernel_ms": round(k_ms, 4), "alg_bytes": int(alg)},
@@ -353,7 +359,7 @@ def workloads_section(h, pts, cols, args):
     cold_ms = (time.perf_counter() - t0) * 1e3
     cold_k = {k: round(v[0], 3) for k, v in h.kernel_times().items()}
     steps = max(6, args.steps // 2)
-    dt, ktot = timed_steps(h, rj.HYBRID, steps, 1)
+    dt, ktot, per = timed_steps(h, rj.HYBRID, steps, 1)
     kms = {k: round(v[0] / v[1], 4) for k, v in ktot.items()}
     # the pass reads x,y, time, passengers and fare of every row (17 B) and at
     # most once each 2-byte label of the hybrid map (8192 pixels a side)
@@ -376,7 +382,7 @@ def workloads_section(h, pts, cols, args):
     out["hybrid_500M_filters"] = {
         "workload": f"hybrid eps=20 m, count()+avg(fare), {n3} points (Morton order), time window 30% "
                     "and passengers<=2, 2200 tract polygons, grid 1024, polygon structures cached",
-        "ms_per_join": round(dt / steps * 1e3, 3), "G_points_per_s": round(n3 * steps / dt / 1e9, 2),
+        "ms_per_join": round(dt / steps * 1e3, 3), **per, "G_points_per_s": round(n3 * steps / dt / 1e9, 2),
         "selected_points": int(counts.sum()), "cold_ms": round(cold_ms, 1), "cold_kernels_ms": cold_k,
         "kernels_ms": kms,
         "roofline": {"kernel": kname, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
@@ -728,6 +734,8 @@ def main():
         "config": head["config"],
         "roofline": head["roofline"],
         "kernels_ms_per_step": head["kernels_ms_per_step"],
+        # min and median of one step's wall time (the reference reports the min)
+        "step_ms": head.get("step_ms"),
     }
     if big and "count" in sections:
         t0 = time.perf_counter()

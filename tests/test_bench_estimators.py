@@ -1,6 +1,7 @@
-"""The bench's count of the reference's edge tests (cell_edge_work): the
-vertices of GridIndexF's candidate lists, cell by cell, against the oracle's
-grid (the CPUJoin restatement)."""
+"""bench.py on the CPU: its count of the reference's edge tests
+(cell_edge_work: the vertices of GridIndexF's candidate lists, cell by cell)
+against the oracle's grid (the CPUJoin restatement), and its one-GPU
+contract."""
 import os
 import sys
 
@@ -48,3 +49,13 @@ def test_edge_tests_counts_points_in_cells():
     # points outside the grid, NaN and infinities test nothing
     odd = np.array([np.nan, 1.0, -1e9, 5.0, np.inf, 3.0, 1e9, 1e9], np.float32)
     assert bench.edge_tests(cw, odd) == 0
+
+
+def test_bench_measures_one_gpu_only():
+    """bench.py --gpus N>1 exits with a message instead of a figure for
+    several GPUs (this build has no multi-GPU path)."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
