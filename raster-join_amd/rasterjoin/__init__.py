@@ -137,7 +137,9 @@ def lib():
 
 
 def _p(a):
-    return None if a is None else a.ctypes.data_as(C.c_void_p)
+    """Address of a numpy array's data for a void* argument (an int: cheaper
+    than a ctypes pointer object on every call)."""
+    return None if a is None else a.ctypes.data
 
 
 # ---------------------------------------------------------------------------
