@@ -421,9 +421,14 @@ __global__ void k_tile_labels(const void* __restrict__ label, int label16, int L
 // point pass: index join, label-map raster join, hybrid join, canvas splat
 // ---------------------------------------------------------------------------
 
+// The hybrid pass with the border queue over many rows takes 4 rows per
+// thread (its per-row state at kRows costs occupancy); short passes keep
+// kRows (fewer iterations of dependent loads per workgroup).
+constexpr int64_t kQueuePassRows4 = 1 << 24;
+
 // One pass over rows [begin, end): each workgroup takes a contiguous block of
 // rows (point_rows) and sums per polygon in LDS (acc_init / acc_flush).
-template <int MODE, bool AVG>
+template <int MODE, bool AVG, int R>
 __global__ __launch_bounds__(kBlock) void k_points(PointArgs a)
 {
     extern __shared__ unsigned long long sacc[];
@@ -439,7 +444,7 @@ __global__ __launch_bounds__(kBlock) void k_points(PointArgs a)
     const int64_t b0 = a.start + (int64_t)blockIdx.x * a.per_block;
     const int64_t b1 = min(a.end, b0 + a.per_block);
     Run run{-1, 0, 0, 0};
-    point_rows<MODE, AVG, kRows>(a, b0, max(a.begin, b0), b1, A, run);
+    point_rows<MODE, AVG, R>(a, b0, max(a.begin, b0), b1, A, run);
     if (run.ck >= 0) run_flush<MODE, AVG>(a, A, run.ck, run.cc, run.ch, run.cl);
     if (MODE != PT_SPLAT) acc_flush(A, AVG);
 }
@@ -632,7 +637,8 @@ void launch_points(hipStream_t s, int mode, PointArgs a)
     // Row groups start at a multiple of kRows so that vector loads are aligned.
     a.start = a.begin - a.begin % kRows;
     const int64_t n = a.end - a.start;
-    const int64_t iter_rows = (int64_t)kBlock * kRows;
+    const int rows = (mode == PT_HYBRID_QUEUE && n >= kQueuePassRows4) ? 4 : kRows;
+    const int64_t iter_rows = (int64_t)kBlock * rows;
     int64_t cap = a.cap;
     if (cap <= 0)
         cap = a.wg_rows > 0 ? std::min<int64_t>(kPassWorkgroupsMax, std::max<int64_t>(kPassWorkgroupsMin, n / a.wg_rows))
@@ -644,7 +650,7 @@ void launch_points(hipStream_t s, int mode, PointArgs a)
     const bool avg = a.q.agg_type != RJ_AGG_COUNT;
     const size_t lds = (mode == PT_SPLAT) ? 0 : acc_lds_bytes(a.P.np, a.lds_np, avg);
 #define RJ_LAUNCH_PTS(M, A) \
-    hipLaunchKernelGGL((k_points<M, A>), dim3(blocks), dim3(kBlock), lds, s, a)
+    hipLaunchKernelGGL((k_points<M, A, kRows>), dim3(blocks), dim3(kBlock), lds, s, a)
     switch (mode) {
     case PT_INDEX: if (avg) RJ_LAUNCH_PTS(PT_INDEX, true); else RJ_LAUNCH_PTS(PT_INDEX, false); break;
     case PT_LABEL: if (avg) RJ_LAUNCH_PTS(PT_LABEL, true); else RJ_LAUNCH_PTS(PT_LABEL, false); break;
@@ -652,7 +658,12 @@ void launch_points(hipStream_t s, int mode, PointArgs a)
         if (avg) RJ_LAUNCH_PTS(PT_LABEL_HYBRID, true); else RJ_LAUNCH_PTS(PT_LABEL_HYBRID, false);
         break;
     case PT_HYBRID_QUEUE:
-        if (avg) RJ_LAUNCH_PTS(PT_HYBRID_QUEUE, true); else RJ_LAUNCH_PTS(PT_HYBRID_QUEUE, false);
+        if (rows == 4) {
+            if (avg) hipLaunchKernelGGL((k_points<PT_HYBRID_QUEUE, true, 4>), dim3(blocks), dim3(kBlock), lds, s, a);
+            else hipLaunchKernelGGL((k_points<PT_HYBRID_QUEUE, false, 4>), dim3(blocks), dim3(kBlock), lds, s, a);
+        } else {
+            if (avg) RJ_LAUNCH_PTS(PT_HYBRID_QUEUE, true); else RJ_LAUNCH_PTS(PT_HYBRID_QUEUE, false);
+        }
         break;
     default: if (avg) RJ_LAUNCH_PTS(PT_SPLAT, true); else RJ_LAUNCH_PTS(PT_SPLAT, false); break;
     }
