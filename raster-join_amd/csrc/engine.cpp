@@ -993,8 +993,10 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
     unsigned long long* res_base = d_res.as<unsigned long long>();
     DevResult res{res_base, (long long*)(res_base + np), res_base + 2 * (size_t)np};
     long long* bnd = (long long*)(res_base + 3 * (size_t)np);
+    // the results and, in the word after them, the border-queue count of a
+    // hybrid pass (read back with them)
     auto clear_results = [&] {
-        return launch("fill_results", [&] { launch_fill_u32(stream, (uint32_t*)res_base, res_words * 2, 0); });
+        return launch("fill_results", [&] { launch_fill_u32(stream, (uint32_t*)res_base, (res_words + 1) * 2, 0); });
     };
     if ((rc = clear_results())) return rc;
 
@@ -1077,10 +1079,8 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
         const uint32_t qcap = queue_cap(C, rows);
         if (opt_queue && !t->host && alloc(d_queue, (size_t)qcap * 16 + 256, "border queue") == RJ_OK) {
             pa.queue = d_queue.as<float4>();
-            pa.qcount = reinterpret_cast<uint32_t*>(d_queue.as<char>() + (size_t)qcap * 16);
+            pa.qcount = reinterpret_cast<uint32_t*>(res_base + res_words);  // zeroed by clear_results
             pa.qcap = qcap;
-            uint32_t* qc = pa.qcount;
-            if ((rc = launch("fill_queue", [&] { launch_fill_u32(stream, qc, 1, 0); }))) return rc;
             if ((rc = launch("points_hybrid", [&] { launch_points(stream, PT_HYBRID_QUEUE, pa); }))) return rc;
             if ((rc = launch("exact_queue", [&] { launch_exact(stream, pa); }))) return rc;
             check_queue = pa.qcount;
@@ -1123,21 +1123,21 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
             return rc;
     }
 
-    // one pinned copy of [results | bounds] (+ the overlap counters, the
-    // border-queue count)
+    // one pinned copy of [results | bounds] (+ the overlap counters of a
+    // raster pass, or the border-queue count of a hybrid pass in the word
+    // after the results: never both)
     const size_t res_bytes = (res_words + 3) * 8;
     if ((rc = pinned((void**)&h_res, &h_res_bytes, res_bytes, "pinned results"))) return rc;
-    hipError_t e = hipMemcpyAsync(h_res, res_base, res_words * 8, hipMemcpyDeviceToHost, stream);
+    hipError_t e = hipMemcpyAsync(h_res, res_base, (res_words + (check_queue ? 1 : 0)) * 8, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess && check_aux)
         e = hipMemcpyAsync(h_res + res_words, check_aux, 16, hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess && check_queue)
-        e = hipMemcpyAsync(h_res + res_words + 2, check_queue, 4, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "execute");
-    if (check_queue) last_exact = (uint32_t)h_res[res_words + 2];
+    const uint32_t queued = check_queue ? (uint32_t)h_res[res_words] : 0;
+    if (check_queue) last_exact = queued;
     if (check_queue && re0 - rb0 >= 65536)  // remember the border share for the next queue
-        C->hyb_border = std::max(0.01, (double)(uint32_t)h_res[res_words + 2] / (double)(re0 - rb0));
-    if (check_queue && (uint32_t)h_res[res_words + 2] > pa.qcap) {
+        C->hyb_border = std::max(0.01, (double)queued / (double)(re0 - rb0));
+    if (check_queue && queued > pa.qcap) {
         // the queue overflowed: the whole pass again with the exact test inline
         last_exact = -1;
         PointArgs ia = pa;
