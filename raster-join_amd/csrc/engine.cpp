@@ -762,15 +762,12 @@ int rj_ctx::build_query(const rj_table* t, const QuerySpec& s, DevQuery* q)
             q->slot_hi[slot] = 0;
         }
     }
-    q->ncons = (int)s.cons.size();
     q->plain = 1;
     for (int k = 0; k < q->nslots; k++)
         if (q->slot_lo[k] != 0 || q->slot_hi[k] != UINT32_MAX) q->plain = 0;
     q->agg_type = s.agg_type;
-    q->agg_col = 0;
     if (s.agg_type != RJ_AGG_COUNT && s.fn != RJ_ERRORBOUNDS) {
         if (s.agg_col < 0 || s.agg_col >= t->ncols) return fail(RJ_ERR_ARG, "aggregation column out of range");
-        q->agg_col = s.agg_col;
         q->agg_slot = slot_of(s.agg_col);
         if (q->agg_slot < 0) return fail(RJ_ERR_LIMIT, "a query may use at most 5 attributes");
     } else {

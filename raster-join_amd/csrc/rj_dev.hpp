@@ -46,9 +46,7 @@ __host__ __device__ inline uint32_t col_key(int type, uint32_t bits)
 // (a conjunction) become one key range: a row passes when slot_lo[s] <=
 // col_key(type, value) <= slot_hi[s] for every slot (lo > hi: no row).
 struct DevQuery {
-    int32_t ncons;
     int32_t agg_type;  // RJ_AGG_*
-    int32_t agg_col;
     int32_t nslots;
     int32_t slot_col[kMaxSlots];
     uint32_t slot_lo[kMaxSlots];
