@@ -1,5 +1,6 @@
 """The --joinType driver (RasterJoin/main.cpp) end to end: polygons read
-from the reference's text format through a polygon-list file, the synthetic
+from the reference's text format through a polygon-list file (with and
+without the Web-Mercator projection of lat/lon vertices), the synthetic
 point table, the per-polygon CSVs (main.cpp:178-225) and the timing line
 (GLHandler.cpp:214-248) against the same joins through the library API."""
 import os
@@ -146,3 +147,53 @@ def test_cli_errorbounds_csv(tmp_path):
 def test_cli_rejects_unknown_join_type(tmp_path):
     r = subprocess.run([CLI, "--joinType", "bogus"], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0 and "Wrong join type" in r.stderr
+
+
+def mercator(lat, lon):
+    """Web Mercator world coordinates at zoom 22 with the textbook form of
+    the latitude term, ln(tan(pi/4 + phi/2)), minus those of the US bounding
+    box's center (the reader's transformPoint, UsefulFuncs.cpp:578-612)."""
+    def world(la, lo):
+        x = (lo + 180.0) / 360.0 * 256.0 * 2.0 ** 22
+        y = (1.0 - np.log(np.tan(np.pi / 4 + np.radians(la) / 2)) / np.pi) * 128.0 * 2.0 ** 22
+        return x, y
+    cx, cy = world((24.396308 + 49.384358) / 2, (-124.848974 + -66.885444) / 2)
+    x, y = world(np.float64(np.float32(lat)), np.float64(np.float32(lon)))
+    return np.float32(x - cx), np.float32(y - cy)
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="driver not built")
+def test_cli_web_mercator_polygons(tmp_path):
+    """--polyTransform reads `lat lon` vertices, projects them, and skips a
+    polygon that leaves the US bounding box: lat/lon rectangles become
+    axis-aligned rectangles whose counts follow from the points directly."""
+    d = str(tmp_path)
+    clat, clon = (24.396308 + 49.384358) / 2, (-124.848974 + -66.885444) / 2
+    rects = [(0.002, 0.006, 0.002, 0.007), (0.008, 0.0125, 0.004, 0.009), (0.001, 0.004, 0.009, 0.015)]
+    lines = [str(len(rects) + 1)]
+    boxes = []
+    for a, b, c, e in rects:  # lat below the center (y grows southwards), lon east of it
+        la0, la1, lo0, lo1 = clat - b, clat - a, clon + c, clon + e
+        lines += ["1", "4", f"{la0!r} {lo0!r}", f"{la0!r} {lo1!r}", f"{la1!r} {lo1!r}", f"{la1!r} {lo0!r}"]
+        x0, y1 = mercator(la0, lo0)
+        x1, y0 = mercator(la1, lo1)
+        boxes.append((x0, x1, y0, y1))
+        if len(boxes) == 2:  # a polygon with a vertex outside the US bbox: skipped
+            lines += ["1", "3", f"{clat!r} {clon!r}", "10.0 -90.0", f"{clat + 0.01!r} {clon!r}"]
+    with open(os.path.join(d, "polys.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    with open(os.path.join(d, "list.csv"), "w") as f:
+        f.write("test,polys.txt\n")
+    out, _ = run_cli(d, "--joinType", "index", "--indexRes", "64", "--polyTransform")
+    rows = read_csv(os.path.join(out, "raster_0_0.csv"))
+    assert [int(r[0]) for r in rows] == list(range(len(rects)))
+    pts = rj.gen_points(N, params=rj.point_params(seed=7))[0]
+    x, y = pts[0::2], pts[1::2]
+    exp = []
+    for x0, x1, y0, y1 in boxes:
+        assert 0 < x0 < x1 < 50000 and 0 < y0 < y1 < 50000
+        # no point within 0.01 m of an edge: the count does not depend on rounding
+        near = (np.minimum(abs(x - x0), abs(x - x1)) < 0.01) | (np.minimum(abs(y - y0), abs(y - y1)) < 0.01)
+        assert not near.any()
+        exp.append(int(((x > x0) & (x < x1) & (y > y0) & (y < y1)).sum()))
+    assert [int(r[1]) for r in rows] == exp and min(exp) > 0
