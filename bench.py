@@ -54,10 +54,14 @@ def log(*a):
 
 
 def cpu_baseline(xy, off, pts, fare, target_s=10.0, max_n=None):
-    """CPUJoin restatement (grid 1024, avg(fare)) on the host cores."""
+    """CPUJoin restatement (grid 1024, avg(fare)) on the host cores, its grid
+    build timed apart (CPUJoin.cpp:126-133); and the 1M-point index join
+    count at grid 256 (min of 6 runs, CPUJoin.cpp:131) on all cores and one."""
     from oracle import pyoracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    t = time.perf_counter()
     g = O.Grid(xy, off, 1024, nthreads=threads)
+    grid_s = time.perf_counter() - t
     n_total = len(fare)
     probe = min(n_total, 1_000_000)
     t = time.perf_counter()
@@ -78,11 +82,29 @@ def cpu_baseline(xy, off, pts, fare, target_s=10.0, max_n=None):
     t = time.perf_counter()
     O.join_exact(g, ps1, nthreads=1)
     one = time.perf_counter() - t
+    # the index workload: 1M points, count, grid 256
+    n2 = min(n_total, 1_000_000)
+    t = time.perf_counter()
+    g256 = O.Grid(xy, off, 256, nthreads=threads)
+    grid256_s = time.perf_counter() - t
+    ps2 = O.PointSet(pts[:2 * n2])
+    idx = {}
+    for name, th, reps in (("all_cores", threads, 6), ("single_core", 1, 2)):
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            O.join_exact(g256, ps2, nthreads=th)
+            ts.append(time.perf_counter() - t)
+        idx[name] = {"G_points_per_s": round(n2 / min(ts) / 1e9, 5), "ms_min": round(min(ts) * 1e3, 2),
+                     "cores": th, "runs": reps}
     return {"value": n / best / 1e9, "unit": "G points/s", "cores": threads, "kind": "port",
             "sample": f"first {n} of the {n_total} points, exact CPUJoin join (grid 1024) with "
                       f"avg(fare), min of 2 runs, {best:.2f} s",
+            "grid_build_s": round(grid_s, 4),
             "single_core": {"value": n1 / one / 1e9, "cores": 1,
-                            "sample": f"first {n1} points, one run, {one:.2f} s"}}
+                            "sample": f"first {n1} points, one run, {one:.2f} s"},
+            "index_1M": {"workload": f"index join count(), first {n2} points, 260 polygons, grid 256",
+                         "grid_build_s": round(grid256_s, 4), **idx}}
 
 
 def cell_edge_work(xy, off, res):
