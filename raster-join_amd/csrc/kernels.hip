@@ -455,9 +455,12 @@ template <bool AVG>
 __global__ __launch_bounds__(kBlock) void k_exact(PointArgs a)
 {
     extern __shared__ unsigned long long sacc[];
+    const uint32_t n = min(*a.qcount, a.qcap);
+    // a workgroup past the queued points leaves before its LDS sums (the
+    // whole workgroup: the test is uniform)
+    if ((uint64_t)blockIdx.x * blockDim.x >= n) return;
     Acc A;
     acc_init(A, sacc, a.P.np, a.lds_np, AVG, a.res);
-    const uint32_t n = min(*a.qcount, a.qcap);
     const DevQuery& q = a.q;
     const int agg_t = AVG ? a.cols.type[q.slot_col[q.agg_slot < 0 ? 0 : q.agg_slot]] : RJ_ATTR_UINT;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -674,7 +677,9 @@ void launch_exact(hipStream_t s, const PointArgs& a)
 {
     const bool avg = a.q.agg_type != RJ_AGG_COUNT;
     const size_t lds = acc_lds_bytes(a.P.np, a.lds_np, avg);
-    const int blocks = 2048;  // grid-stride over the queue; the count is read on the device
+    // grid-stride over the queue (the count is read on the device), about one
+    // workgroup per 512 entries of room, 64..2048
+    const int blocks = (int)std::min<uint32_t>(2048, std::max<uint32_t>(64, a.qcap / 512));
     if (avg) hipLaunchKernelGGL(k_exact<true>, dim3(blocks), dim3(kBlock), lds, s, a);
     else hipLaunchKernelGGL(k_exact<false>, dim3(blocks), dim3(kBlock), lds, s, a);
 }
