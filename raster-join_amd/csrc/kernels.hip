@@ -466,7 +466,7 @@ __global__ __launch_bounds__(kBlock) void k_exact(PointArgs a)
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float4 e = a.queue[i];
         const uint32_t v = __float_as_uint(e.z);
-        const int key = exact_first<AVG>(a.g, a.P, A, e.x, e.y, agg_t, v);
+        const int key = exact_first<AVG, true, 4>(a.g, a.P, A, e.x, e.y, agg_t, v);
         if (key >= 0) {
             unsigned long long h = 0, l = 0;
             if (AVG) agg_bits(agg_t, v, h, l);

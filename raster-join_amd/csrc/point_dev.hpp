@@ -179,6 +179,9 @@ __device__ __forceinline__ int grid_cell(const DevGrid& g, float x, float y, int
 // Crossing-number PIP (GridIndexF.cpp:383-402, the same operation order per
 // edge) over the edges of one row band of a polygon: the edges a point of that
 // row can cross, so the parity is that of the loop over all edges.
+// U > 1: edges are loaded U at a time (loads in flight together; the
+// parity does not depend on the order of the edges).
+template <int U = 1>
 __device__ __forceinline__ bool pip_band(const DevGrid& g, int p, int row, float tx, float ty)
 {
     const int lo = g.band_lo[p];
@@ -186,7 +189,19 @@ __device__ __forceinline__ bool pip_band(const DevGrid& g, int p, int row, float
     const int b = g.pband[p] + (row - lo);
     const int k1 = g.band_off[b + 1];
     bool c = false;
-    for (int k = g.band_off[b]; k < k1; k++) {
+    int k = g.band_off[b];
+    if (U > 1) {
+        for (; k + U <= k1; k += U) {
+            float4 e[U];
+#pragma unroll
+            for (int j = 0; j < U; j++) e[j] = g.edges[k + j];
+#pragma unroll
+            for (int j = 0; j < U; j++)
+                if (((e[j].y > ty) != (e[j].w > ty)) && (tx < (e[j].z - e[j].x) * (ty - e[j].y) / (e[j].w - e[j].y) + e[j].x))
+                    c = !c;
+        }
+    }
+    for (; k < k1; k++) {
         const float4 e = g.edges[k];  // vi = (x, y), vj = (z, w)
         if (((e.y > ty) != (e.w > ty)) && (tx < (e.z - e.x) * (ty - e.y) / (e.w - e.y) + e.x)) c = !c;
     }
@@ -330,7 +345,7 @@ __device__ __forceinline__ void acc_flush(const Acc& A, bool avg)
 
 // CPUJoin getRegion over the cell's candidates. Returns the first polygon that
 // contains the point; further containing polygons are added directly.
-template <bool AVG, bool LDS = true>
+template <bool AVG, bool LDS = true, int U = 1>
 __device__ __forceinline__ int exact_first(const DevGrid& g, const DevPolys& P, const Acc& A, float x,
                                            float y, int agg_t, uint32_t vraw)
 {
@@ -342,7 +357,7 @@ __device__ __forceinline__ int exact_first(const DevGrid& g, const DevPolys& P, 
     const int k1 = g.cell_off[cell + 1];
     for (int k = g.cell_off[cell]; k < k1; k++) {
         const int p = g.cand[k];
-        if (pip_band(g, p, row, x, y)) {
+        if (pip_band<U>(g, p, row, x, y)) {
             if (first < 0) {
                 first = p;
             } else {
