@@ -239,7 +239,7 @@ __device__ __forceinline__ void exact_tile(const PointArgs& a, uint32_t i0, uint
     for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         const float4 e = a.queue[i];
         const uint32_t v = __float_as_uint(e.z);
-        const int key = exact_first<AVG>(a.g, a.P, A, e.x, e.y, agg_t, v);
+        const int key = exact_first<AVG, true, 4>(a.g, a.P, A, e.x, e.y, agg_t, v);
         if (key >= 0) {
             unsigned long long h = 0, l = 0;
             if (AVG) agg_bits(agg_t, v, h, l);

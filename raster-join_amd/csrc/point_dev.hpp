@@ -608,7 +608,7 @@ __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, in
         if (MODE == PT_INDEX) {
 #pragma unroll
             for (int r = 0; r < R; r++)
-                key[r] = ok[r] ? exact_first<AVG, LDS>(a.g, a.P, A, px[r], py[r], agg_t, vraw[r]) : -1;
+                key[r] = ok[r] ? exact_first<AVG, LDS, 4>(a.g, a.P, A, px[r], py[r], agg_t, vraw[r]) : -1;
         } else {
             int pix[R], ixs[R], iys[R];
 #pragma unroll
