@@ -222,9 +222,13 @@ def test_hybrid_count_1M_voronoi260_reference(handler):
 # errorbounds (K1 count splat + K3 boundary bands)
 # ---------------------------------------------------------------------------
 
-@pytest.mark.parametrize("eps", [20.0, 100.0])
-def test_errorbounds_match_emulator_and_contain_exact(handler, eps):
-    xy, off = polys("voronoi260")
+@pytest.mark.parametrize("name,eps", [("voronoi260", 20.0), ("voronoi260", 100.0), ("tracts400", 10.0),
+                                      ("stars", 40.0)])
+def test_errorbounds_match_emulator_and_contain_exact(handler, name, eps):
+    """Bounds at several accuracies and collections, overlapping polygons
+    (stars) included: equal to the CPU emulation, and the exact count of
+    every polygon lies within them."""
+    xy, off = polys(name)
     pts, t, fare, pas = points(1_000_000, 15)
     cons = PASS_LE2
     ps = O.PointSet(pts, (t, fare, pas), COLS_TYPES, cons)
