@@ -106,11 +106,12 @@ def test_index_edge_cases(ih):
 # raster join (K1 splat + K2 scan, or label map + gather)
 # ---------------------------------------------------------------------------
 
-@pytest.mark.parametrize("eps", [10.0, 50.0])
-def test_raster_paths_match_emulator(handler, eps):
-    xy, off = polys("voronoi260")
+@pytest.mark.parametrize("name,eps,window", [("voronoi260", 10.0, False), ("voronoi260", 50.0, False),
+                                             ("tracts2200", 20.0, True)])
+def test_raster_paths_match_emulator(handler, name, eps, window):
+    xy, off = polys(name)
     pts, t, fare, pas = points(2_000_000, 10)
-    cons = [(2, rj.LTE, 4)]
+    cons = [(2, rj.LTE, 4)] + (WINDOW_30 if window else [])
     p = O.plan(O.poly_bbox(xy).astype(np.float64), eps)
     oc, ohi, olo = O.raster_join(p, xy, off, O.PointSet(pts, (t, fare, pas), COLS_TYPES, cons,
                                                           agg_type=rj.AVG, agg_col=1))
