@@ -173,6 +173,8 @@ def edge_tests(cell_work, pts):
 
 
 def valu_roofline(kernel, tests, ms):
+    """`tests` edge tests done in `ms` priced at FLOPS_PER_EDGE_TEST FP32
+    operations each, against the FP32 vector peak."""
     ach = tests * FLOPS_PER_EDGE_TEST / (ms * 1e-3) / 1e12
     return {"kernel": kernel, "bound": "valu", "edge_tests": int(tests),
             "G_edge_tests_per_s": round(tests / (ms * 1e-3) / 1e9, 2), "flops_per_edge_test": FLOPS_PER_EDGE_TEST,
