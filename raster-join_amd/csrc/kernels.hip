@@ -144,6 +144,24 @@ __device__ __forceinline__ int bit_at(const uint32_t* row, int col)
     return (row[col >> 5] >> (col & 31)) & 1;
 }
 
+// Bits col..col+7 of a row of wpr words (bit k of the result: column col+k),
+// 0 for columns outside [0, bw); col in [-7, bw).
+__device__ __forceinline__ uint32_t bits8(const uint32_t* row, int col, int bw, int wpr)
+{
+    uint32_t m;
+    if (col >= 0) {
+        const int w = col >> 5, sh = col & 31;
+        uint64_t v = row[w];
+        if (sh > 24 && w + 1 < wpr) v |= (uint64_t)row[w + 1] << 32;
+        m = (uint32_t)(v >> sh) & 0xFFu;
+    } else {
+        m = (row[0] << (-col)) & 0xFFu;
+    }
+    const int valid = bw - col;
+    if (valid < 8) m &= (1u << valid) - 1u;
+    return m;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_band(const Band* __restrict__ bands, DevPolys P, DevCanvas cv,
                                                  const uint32_t* __restrict__ cnt,
@@ -193,13 +211,7 @@ __global__ __launch_bounds__(kBlock) void k_band(const Band* __restrict__ bands,
         const uint32_t pp = (uint32_t)p | ((uint32_t)p << 16);
         for (int idx = threadIdx.x; idx < nr * ng; idx += blockDim.x) {
             const int row = idx / ng, g = g0 + idx % ng;
-            const uint32_t* trow = tog + row * wpr;
-            uint32_t m = 0;
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const int lc = 8 * g + k - c0;
-                if (lc >= 0 && lc < bw && bit_at(trow, lc)) m |= 1u << k;
-            }
+            const uint32_t m = bits8(tog + row * wpr, 8 * g - c0, bw, wpr);
             if (!m) continue;
             uint16_t* dst = static_cast<uint16_t*>(label) + (int64_t)(r0 + row) * cv.LW + 8 * g;
             if (m == 0xFFu) {
