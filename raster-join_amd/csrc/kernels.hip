@@ -70,10 +70,21 @@ __device__ __forceinline__ void edge_toggles(int64_t ax, int64_t ay, int64_t bx,
     if (jhi > R0 + NR - 1) jhi = R0 + NR - 1;
     if (jlo > jhi) return;
     const int64_t dy = by - ay, dx = bx - ax;
+    // k(row) = ceil((ax*dy + dx*(row*cs + h - ay) - h*dy) / (cs*dy)): the
+    // numerator grows by dx*cs per row, so k follows from one division for
+    // the first row and a quotient/remainder step per row (exact integers)
+    const int64_t den = cs * dy;
+    const int64_t num0 = ax * dy + dx * (jlo * cs + h - ay) - h * dy;
+    int64_t q = floor_div(num0, den), r = num0 - q * den;  // num = q*den + r, 0 <= r < den
+    const int64_t sq = floor_div(dx * cs, den), sr = dx * cs - sq * den;
     for (int64_t row = jlo; row <= jhi; row++) {
-        int64_t cy = row * cs + h;
-        int64_t N = ax * dy + dx * (cy - ay);
-        int64_t k = ceil_div(N - h * dy, cs * dy);
+        const int64_t k = q + (r != 0 ? 1 : 0);
+        q += sq;
+        r += sr;
+        if (r >= den) {
+            r -= den;
+            q++;
+        }
         int64_t local = k - C0;
         if (local < 0) local = 0;
         if (local >= BW) continue;
