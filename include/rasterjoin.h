@@ -56,10 +56,11 @@ typedef struct rj_table rj_table;
 typedef struct rj_polyset rj_polyset;
 typedef struct rj_server rj_server;
 
-/* GLHandler::getInstance(gpuMemMB, inMemory), GLHandler.cpp:144-149 (the
- * budget keeps the reference's 100 MB reserve, GLHandler.cpp:128-132). */
+/* GLHandler::getInstance(gpuMemMB, inMemory), GLHandler.cpp:64-69, and the
+ * constructor it calls (GLHandler.cpp:43-54; the budget keeps its 100 MB
+ * reserve, :51). */
 int rj_create(int64_t gpu_mem_mb, int32_t inmem, rj_ctx** out);
-/* GLHandler::~GLHandler, GLHandler.cpp:136-142 */
+/* GLHandler::~GLHandler, GLHandler.cpp:56-62 */
 int rj_destroy(rj_ctx* ctx);
 const char* rj_last_error(const rj_ctx* ctx);
 

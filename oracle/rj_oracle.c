@@ -280,7 +280,8 @@ static int pass_filters(const orc_points* P, int64_t i)
 
 /* Exact fixed-point value: v = hi + lo * 2^-32 with hi = floor(v), lo = the
  * fraction times 2^32 truncated (exact for |v| >= 2^-9). Integer columns give
- * hi = value, lo = 0. Non-finite or |v| >= 2^62 contributes 0. */
+ * hi = value, lo = 0. Non-finite or |v| >= 2^62 contributes 0. A float in
+ * [-2^-25, 0), whose float fraction v - floor(v) rounds to 1, counts as 0. */
 static void agg_value(const orc_points* P, int64_t i, int64_t* hi, uint64_t* lo, double* d)
 {
     int col = P->agg_col;
@@ -299,9 +300,11 @@ static void agg_value(const orc_points* P, int64_t i, int64_t* hi, uint64_t* lo,
         float f = ((const float*)P->cols[col])[i];
         if (!(fabsf(f) < 4.611686018427387904e18f)) { *hi = 0; *d = 0.0; return; }
         float fl = floorf(f);
+        float fr = f - fl;
         *hi = (int64_t)fl;
-        *lo = (uint64_t)((f - fl) * 4294967296.0f);
         *d = (double)f;
+        if (fr >= 1.0f) { *hi += 1; return; }
+        *lo = (uint64_t)(fr * 4294967296.0f);
     }
 }
 

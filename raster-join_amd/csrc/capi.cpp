@@ -275,7 +275,7 @@ int rj_create(int64_t gpu_mem_mb, int32_t inmem, rj_ctx** out)
         delete c;
         return RJ_ERR_DEVICE;
     }
-    // GLHandler keeps 100 MB of the budget for other data (GLHandler.cpp:128-132).
+    // GLHandler keeps 100 MB of the budget for other data (GLHandler.cpp:51).
     c->budget->limit = gpu_mem_mb > 0 ? (gpu_mem_mb - 100) * 1024LL * 1024LL : 0;
     if (gpu_mem_mb > 0 && c->budget->limit <= 0) c->budget->limit = 1;
     c->inmem = inmem;

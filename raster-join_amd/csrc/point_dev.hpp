@@ -48,8 +48,10 @@ __device__ __forceinline__ void slot_filter(int type, uint32_t lo, uint32_t hi, 
 }
 
 // Exact fixed-point aggregate value of a 32-bit pattern: hi = floor(v),
-// lo = frac(v) * 2^32 (integer columns: hi = value, lo = 0). The fraction
-// times 2^32 is below 2^32 - 255, so lo always fits 32 bits.
+// lo = frac(v) * 2^32 (integer columns: hi = value, lo = 0). For v in
+// [-2^-25, 0) the float difference v - floor(v) rounds to 1.0: such a v
+// counts as 0 (hi = floor(v) + 1, lo = 0). Otherwise the difference is at
+// most 1 - 2^-24, so lo <= 2^32 - 256 fits 32 bits.
 __device__ __forceinline__ void agg_bits(int type, uint32_t a, unsigned long long& hi, unsigned long long& lo)
 {
     lo = 0;
@@ -64,7 +66,12 @@ __device__ __forceinline__ void agg_bits(int type, uint32_t a, unsigned long lon
             hi = 0;  // non-finite or |v| >= 2^62 contributes nothing
             return;
         }
-        lo = (uint32_t)((f - fl) * 4294967296.0f);
+        const float fr = f - fl;
+        if (fr >= 1.0f) {
+            hi += 1ull;
+            return;
+        }
+        lo = (uint32_t)(fr * 4294967296.0f);
     } else if (type == RJ_ATTR_INT) {
         hi = (unsigned long long)(long long)(int32_t)a;
     } else {

@@ -103,3 +103,37 @@ def test_filters_fused_batch(handler, data):
         for cons, r in zip(sets[k:k + 48], res):
             assert r.status == 0
             assert np.array_equal(r.counts, oracle_counts(g, pts, cols, cons)), cons
+
+
+def test_float_sums_of_special_values(handler, data):
+    """avg over the float column that holds NaN, infinities, signed zeros,
+    denormals and tiny negatives: the exact fixed-point sums (hi, lo) equal
+    the oracle's on both index paths, the hybrid join and a fused batch.
+    A float in [-2^-25, 0) counts as 0; non-finite values count nothing."""
+    pts, cols = data
+    xy, off = polys("voronoi20")
+    g = O.Grid(xy, off, 64)
+    h = handler
+    h.set_polygons(xy, off)
+    h.set_points(pts, cols, TYPES)
+    h.set_aggregation(rj.AVG, 1)
+    h.set_poly_index_resolution(64)
+    h.set_accuracy_distance(20.0)
+    for cons in ([], [(1, rj.LTE, 0.0)], [(1, rj.GT, -1.0), (1, rj.LT, 1.0)]):
+        c, hi, lo, _ = O.join_exact(g, O.PointSet(pts, cols, TYPES, cons, agg_type=1, agg_col=1))
+        h.set_query_constraints(cons)
+        for fn, path in ((rj.INDEX, 0), (rj.INDEX, 1), (rj.HYBRID, 0)):
+            h.set_option(rj.OPT_INDEX_PATH, path)
+            got, _ = h.execute_function(fn)
+            ghi, glo = h.sums_fixed()
+            assert np.array_equal(got, c), (fn, path, cons)
+            assert np.array_equal(ghi, hi) and np.array_equal(glo, lo), (fn, path, cons)
+    h.set_option(rj.OPT_INDEX_PATH, 0)
+    table = rj.Table(h, pts, cols, TYPES)
+    ps = rj.PolySet(h, xy, off)
+    qs = [rj.make_query(ps, fn, rj.AVG, 1, [], eps=20.0, index_res=64) for fn in (rj.INDEX, rj.HYBRID)]
+    c, hi, lo, _ = O.join_exact(g, O.PointSet(pts, cols, TYPES, [], agg_type=1, agg_col=1))
+    for r in h.execute_batch(table, qs, [ps.np] * len(qs)):
+        assert r.status == 0
+        assert np.array_equal(r.counts, c)
+        assert np.array_equal(r.hi, hi) and np.array_equal(r.lo, lo)

@@ -80,6 +80,21 @@ def test_index_filters_and_avg_exact(ih):
     assert np.allclose(s, of64, rtol=1e-9, atol=0)
 
 
+def test_index_tracts2200_grid1024_filters_and_avg(ih):
+    """The 2200-tract collection (30-2000 jittered vertices per polygon) at
+    grid 1024 with a 30 % time window, passengers <= 2 and avg(fare): counts
+    and exact sums equal the oracle's CPUJoin grid (GridIndexF.cpp:60-155)."""
+    xy, off = polys("tracts2200")
+    pts, t, fare, pas = points(1_000_000, 31)
+    cons = WINDOW_30 + PASS_LE2
+    c, s, hi, lo = run(ih, rj.INDEX, xy, off, pts, (t, fare, pas), cons, rj.AVG, 1, eps=20.0, res=1024)
+    oc, ohi, olo, of64 = oracle_exact(xy, off, pts, (t, fare, pas), cons, rj.AVG, 1, res=1024)
+    assert np.array_equal(c, oc)
+    assert np.array_equal(hi, ohi) and np.array_equal(lo, olo)
+    assert np.allclose(s, of64, rtol=1e-9, atol=0)
+    assert c.sum() > 20_000
+
+
 def test_index_overlapping_polygons(ih):
     xy, off = polys("stars")
     pts = points(400_000, 9)[0]
@@ -171,7 +186,8 @@ def test_raster_pixel_aligned_squares_are_exact(handler):
 # ---------------------------------------------------------------------------
 
 @pytest.mark.parametrize("name,eps,res", [("voronoi260", 20.0, 256), ("voronoi260", 5.0, 1024),
-                                          ("tracts400", 20.0, 256), ("stars", 100.0, 64)])
+                                          ("tracts400", 20.0, 256), ("stars", 100.0, 64),
+                                          ("tracts2200", 20.0, 1024)])
 def test_hybrid_equals_cpujoin(handler, name, eps, res):
     xy, off = polys(name)
     pts, t, fare, pas = points(1_000_000, 14)
