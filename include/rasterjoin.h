@@ -191,6 +191,22 @@ enum { RJ_PATH_SINGLE = 0, RJ_PATH_FUSED = 1 };
 
 /* One query with the single-query kernels. */
 int rj_execute_query(rj_ctx* ctx, rj_table* t, const rj_query* q, rj_result* out);
+/* rj_execute_query without allocating: no device or pinned host memory is
+ * allocated or freed (no hipMalloc, hipHostMalloc, hipFree or allocating
+ * library routine). No counterpart in the reference; it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The per-call buffers (per-polygon results,
+ * the hybrid border queue and its counter) live in a static per-device scratch
+ * array of 32 MiB that the library holds until the process ends; every call on
+ * a device shares it, so calls that overlap on one device race (one context
+ * serializes its own calls). Limits: a device-resident table, RJ_RASTER
+ * (label-map path), RJ_HYBRID or RJ_INDEX, at most 65536 polygons
+ * (RJ_ERR_LIMIT beyond); RJ_ERRORBOUNDS and the canvas path return
+ * RJ_ERR_UNSUPPORTED. The query's polygon structures must be resident (run it
+ * once with rj_execute_query, caching on), else RJ_ERR_STATE. A border queue
+ * larger than the scratch's room reruns the pass with inline exact tests.
+ * Results equal rj_execute_query's exactly. */
+int rj_execute_query_noalloc(rj_ctx* ctx, rj_table* t, const rj_query* q, rj_result* out);
 /* A batch of queries over one table: queries that can share a scan run in
  * fused passes (up to 64 queries per pass, one read of the union of their
  * rows); errorbounds, the canvas path and queries over

@@ -279,6 +279,9 @@ int rj_create(int64_t gpu_mem_mb, int32_t inmem, rj_ctx** out)
     c->budget->limit = gpu_mem_mb > 0 ? (gpu_mem_mb - 100) * 1024LL * 1024LL : 0;
     if (gpu_mem_mb > 0 && c->budget->limit <= 0) c->budget->limit = 1;
     c->inmem = inmem;
+    // the no-alloc entry point's per-device scratch address (its lookup
+    // error, if any, is reported by rj_execute_query_noalloc)
+    (void)rj::noalloc_scratch_setup(c->device);
     *out = c;
     return RJ_OK;
 }

@@ -16,17 +16,7 @@ int organize_rows(hipStream_t s, int64_t n, float* xy, void** cols, const int32_
                   int64_t* off_dev, void* scratch);
 }  // namespace rj
 
-using namespace rj;
-
-namespace {
-
-double now_ms()
-{
-    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
-        .count();
-}
-
-size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+namespace rj {
 
 // Largest collection whose per-polygon sums a workgroup keeps in LDS indexed
 // by polygon (16 KiB); larger ones use the hashed slots.
@@ -45,6 +35,21 @@ double canvas_eps(const QuerySpec& s)
     const double ext = std::max(b[2] - b[0], b[3] - b[1]);
     return ext / 8191.999 * std::sqrt(2.0) * s.ground_res;
 }
+
+}  // namespace rj
+
+using namespace rj;
+
+namespace {
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+
 
 // label-map tiles: columns LW >> tshift, rows ceil(H / 2^tshift); the tile
 // buffer ends with two u64 overlap counters

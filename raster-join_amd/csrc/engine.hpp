@@ -86,6 +86,16 @@ struct QuerySpec {
     int64_t rb = 0, re = 0;  // rows of the table (before time narrowing)
 };
 
+// Largest collection whose per-polygon sums a point pass keeps in LDS indexed
+// by polygon; larger ones use hashed slots.
+int lds_direct_cap(bool avg);
+// Accuracy of the canvas a query runs on (hybrid passes: 8192 pixels on the
+// long side of the collection's bbox).
+double canvas_eps(const QuerySpec& s);
+// Looks up, once per device (current), the device address of the no-alloc
+// entry point's scratch array (noalloc.hip).
+hipError_t noalloc_scratch_setup(int device);
+
 // Results of one query: counts, exact sums hi + lo * 2^-32, error bounds.
 struct QueryOut {
     std::vector<int64_t> counts, hi, bounds;

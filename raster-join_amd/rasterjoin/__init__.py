@@ -32,6 +32,7 @@ OPT_TILE_SHIFT, OPT_PASS_WORKGROUPS, OPT_BATCH_ROWS, OPT_BATCH_WORKGROUPS, OPT_B
 OPT_INDEX_PATH, OPT_EXACT_QUEUE, OPT_STREAM_ROWS, OPT_BATCH_HYBRID_ROWS, OPT_BATCH_SHARE = 12, 13, 14, 15, 16
 # how a query ran
 PATH_SINGLE, PATH_FUSED = 0, 1
+ERR_ARG, ERR_STATE, ERR_NOMEM, ERR_DEVICE, ERR_LIMIT, ERR_UNSUPPORTED = -1, -2, -3, -4, -5, -6
 
 ERRORS = {0: "ok", -1: "bad argument", -2: "bad state", -3: "out of memory",
           -4: "device error", -5: "limit exceeded", -6: "unsupported"}
@@ -117,6 +118,7 @@ def lib():
             "rj_polyset_create": [vp, vp, vp, i32, vp, vp], "rj_polyset_count": [vp, vp],
             "rj_polyset_bytes": [vp, vp], "rj_polyset_destroy": [vp], "rj_bind_polyset": [vp, vp],
             "rj_execute_query": [vp, vp, vp, vp], "rj_execute_batch": [vp, vp, i32, vp, vp],
+            "rj_execute_query_noalloc": [vp, vp, vp, vp],
             "rj_autotune": [vp, vp, i32, vp, vp],
             "rj_server_create": [vp, vp, vp, vp], "rj_server_submit": [vp, vp, vp, vp],
             "rj_server_poll": [vp, C.c_uint64, i32], "rj_server_stats_get": [vp, vp],
@@ -502,6 +504,14 @@ class Handler:
     def execute_query(self, table, q, np_):
         res = QueryResult(np_)
         rc = lib().rj_execute_query(self.ctx, table.h, C.byref(q), C.byref(res.rec))
+        self._check(rc)
+        return res
+
+    def execute_query_noalloc(self, table, q, np_):
+        """rj_execute_query_noalloc: the query on resident structures, with its
+        per-call buffers in the library's static scratch array."""
+        res = QueryResult(np_)
+        rc = lib().rj_execute_query_noalloc(self.ctx, table.h, C.byref(q), C.byref(res.rec))
         self._check(rc)
         return res
 
