@@ -460,23 +460,13 @@ def stream_section(h, pts, cols, lib, args):
             "identical_results": bool(same)}
 
 
-POLY_LIB = {  # name: (sites, seed, spacing, jitter)
-    "v10": (10, 101, (400.0, 400.0), 30.0),
-    "v40": (40, 102, (150.0, 150.0), 30.0),
-    "v260": (260, 1, (60.0, 60.0), 30.0),
-    "v900": (900, 104, (40.0, 40.0), 30.0),
-    "t2200": (2200, 3, (2.2, 140.0), 30.0),
-    "v5000": (5000, 106, (25.0, 25.0), 20.0),
-}
+# the mixes' collections (name in the JSON line: POLY_SETS entry)
+POLY_LIB = {"v10": "voronoi10", "v40": "voronoi40", "v260": "voronoi260", "v900": "voronoi900",
+            "t2200": "tracts2200", "v5000": "voronoi5000"}
 
 
 def poly_library(h, names):
-    out = {}
-    for k in names:
-        s, seed, sp, jit = POLY_LIB[k]
-        xy, off = rj.gen_voronoi(s, seed=seed, spacing=sp, jitter=jit)
-        out[k] = rj.PolySet(h, xy, off)
-    return out
+    return {k: rj.PolySet(h, *rj.poly_set(POLY_LIB[k])) for k in names}
 
 
 def window_rows_frac(rng, lo_frac, hi_frac):

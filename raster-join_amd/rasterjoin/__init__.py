@@ -191,7 +191,18 @@ def gen_voronoi(nsites, seed=1, spacing=(60.0, 60.0), jitter=30.0, box=50000.0):
 POLY_SETS = {
     "voronoi260": dict(nsites=260, seed=1, spacing=(60.0, 60.0), jitter=30.0),
     "tracts2200": dict(nsites=2200, seed=3, spacing=(2.2, 140.0), jitter=30.0),
+    # the query mixes' collections of 10 to 5000 polygons
+    "voronoi10": dict(nsites=10, seed=101, spacing=(400.0, 400.0), jitter=30.0),
+    "voronoi40": dict(nsites=40, seed=102, spacing=(150.0, 150.0), jitter=30.0),
+    "voronoi900": dict(nsites=900, seed=104, spacing=(40.0, 40.0), jitter=30.0),
+    "voronoi5000": dict(nsites=5000, seed=106, spacing=(25.0, 25.0), jitter=20.0),
 }
+
+
+def poly_set(name):
+    """The (xy, off) of a named collection of POLY_SETS."""
+    cp = POLY_SETS[name]
+    return gen_voronoi(cp["nsites"], seed=cp["seed"], spacing=cp["spacing"], jitter=cp["jitter"])
 
 
 def make_constraints(cons):

@@ -9,13 +9,25 @@ other GPU tests):
   avg(fare): the hybrid join equals the index join's per-point grid test (two
   independent CPUJoin implementations); additive over row ranges; filtered
   counts never exceed unfiltered ones.
-* 1B points in day buckets: fused batches equal the queries run alone.
+* 1B points in day buckets (x, y, time, fare, passengers):
+  - 64 fused raster + hybrid queries after the autotuner (polygon sets of
+    10-2200, eps in {5, 10, 50, 100}, windows selecting 1M-1B rows) equal
+    the queries run alone;
+  - the serving loop under Poisson arrivals (60/30/10 raster/hybrid/index,
+    10-5000 polygons, windows 1 hour - 1 year, passengers filters) returns
+    every query's alone result;
+  - in both, queries restricted to a few million rows of the 1B table (rows
+    straddling a window's start, deep in the table) run in the same fused
+    passes or served mix and equal the oracle on those rows.
 """
 import gc
+import threading
+import time
 
 import numpy as np
 import pytest
 
+from oracle import pyoracle as O
 import rasterjoin as rj
 from common import COLS_TYPES, PASS_LE2, T0, WINDOW_30, polys
 
@@ -124,33 +136,166 @@ def test_hybrid_500M_filters_equals_grid_index():
     gc.collect()
 
 
-def test_batch_1B_equals_single():
+YEAR = 31536000
+
+
+@pytest.fixture(scope="module")
+def big():
+    """The resident 1B-row table in (day bucket, Morton) order and the mixes'
+    collections (10 to 5000 polygons)."""
     n = 1_000_000_000
     pts, t, fare, pas = gen(n, 7)
     h = rj.Handler()
     table = rj.Table(h, pts, (t, fare, pas), COLS_TYPES, time_col=0, time0=T0, bucket_seconds=86400)
     del pts, t, fare, pas
     gc.collect()
-    sets = [rj.PolySet(h, *polys(k)) for k in ("voronoi260", "tracts400", "voronoi20", "tracts2200")]
-    rng = np.random.default_rng(77)
-    qs, nps = [], []
-    for i in range(16):
-        ps = sets[i % len(sets)]
-        length = int(np.exp(rng.uniform(np.log(3600), np.log(31536000))))
-        lo = T0 + int(rng.uniform(0, 31536000 - length))
-        cons = [(0, rj.GTE, lo), (0, rj.LTE, lo + length)]
-        fn = (rj.RASTER, rj.HYBRID, rj.INDEX)[i % 3]
-        agg = rj.AVG if i % 2 else rj.COUNT
-        qs.append(rj.make_query(ps, fn, agg, 1 if agg == rj.AVG else -1, cons,
-                                eps=float((5, 10, 50, 100)[i % 4]), index_res=256))
-        nps.append(ps.np)
-    batch = h.execute_batch(table, qs, nps)
-    for q, k, b in zip(qs, nps, batch):
-        a = h.execute_query(table, q, k)
-        assert b.status == 0
-        assert np.array_equal(a.counts, b.counts) and np.array_equal(a.hi, b.hi)
-        assert np.array_equal(a.lo, b.lo)
-    assert sum(b.path == rj.PATH_FUSED for b in batch) >= 12
+    names = ("voronoi10", "voronoi40", "voronoi260", "voronoi900", "tracts2200", "voronoi5000")
+    lib = {k: rj.PolySet(h, *rj.poly_set(k)) for k in names}
+    yield h, table, lib
+    for ps in lib.values():
+        ps.close()
+    table.close()
     h.close()
-    del table
     gc.collect()
+
+
+def same(a, b):
+    return (a.status == 0 and b.status == 0 and np.array_equal(a.counts, b.counts)
+            and np.array_equal(a.hi, b.hi) and np.array_equal(a.lo, b.lo))
+
+
+def window_frac(rng, lo_frac, hi_frac):
+    f = float(np.exp(rng.uniform(np.log(lo_frac), np.log(hi_frac))))
+    length = max(1, int(f * YEAR))
+    start = T0 + int(rng.uniform(0, max(0.0, 1.0 - f)) * YEAR)
+    return [(0, rj.GTE, start), (0, rj.LTE, start + length - 1)]
+
+
+def oracle_checked(table, lib, specs):
+    """Queries over 3M rows of the table that straddle the start of a 30 %
+    time window (the rows before it fail the time filter), with their
+    oracle results on the rows read back: raster against the pixel-center
+    emulation, hybrid and index against the CPUJoin restatement."""
+    lo, hi = T0 + 200 * 86400, T0 + 200 * 86400 + int(0.3 * YEAR)
+    b0, _ = table.time_rows(lo, hi)
+    rb, re = b0 - 1_000_000, b0 + 2_000_000
+    xy, cols = table.read(rb, re - rb)
+    out = []
+    for name, fn, eps, agg, extra in specs:
+        ps = lib[name]
+        cons = [(0, rj.GTE, lo), (0, rj.LTE, hi)] + extra
+        q = rj.make_query(ps, fn, agg, 1 if agg == rj.AVG else -1, cons, eps=eps, index_res=256, rows=(rb, re))
+        pxy, poff = rj.poly_set(name)
+        pset = O.PointSet(xy, cols, COLS_TYPES, cons, agg_type=agg, agg_col=1)
+        if fn == rj.RASTER:
+            want = O.raster_join(O.plan(O.poly_bbox(pxy).astype(np.float64), eps), pxy, poff, pset)
+        else:
+            want = O.join_exact(O.Grid(pxy, poff, 256), pset)[:3]
+        out.append((q, ps.np, want, agg))
+    return out
+
+
+def check_oracle(r, want, agg):
+    assert r.status == 0
+    assert np.array_equal(r.counts, want[0])
+    if agg == rj.AVG:
+        assert np.array_equal(r.hi, want[1]) and np.array_equal(r.lo, want[2])
+    assert r.counts.sum() > 0
+
+
+def test_batch_1B_64_queries_autotuned(big):
+    """64 raster (70 %) and hybrid (30 %) queries fused over the 1B-row
+    table after rj_autotune, each equal to its single-query result, plus six
+    row-range queries in the same passes equal to the oracle."""
+    h, table, lib = big
+    rng = np.random.default_rng(41)
+    names = ("voronoi10", "voronoi40", "voronoi260", "voronoi900", "tracts2200")
+    qs, nps = [], []
+    for _ in range(64):
+        ps = lib[names[int(rng.integers(len(names)))]]
+        fn = rj.RASTER if rng.random() < 0.7 else rj.HYBRID
+        eps = float((5.0, 10.0, 50.0, 100.0)[int(rng.integers(4))])
+        cons = window_frac(rng, 1e6 / table.n, 1.0)
+        agg = rj.AVG if rng.random() < 0.5 else rj.COUNT
+        qs.append(rj.make_query(ps, fn, agg, 1 if agg == rj.AVG else -1, cons, eps=eps, index_res=256))
+        nps.append(ps.np)
+    rows = [table.time_rows(q.cons[0].val, q.cons[1].val) for q in qs]
+    sel = [e - b for b, e in rows]
+    assert min(sel) < 20_000_000 and max(sel) > 500_000_000
+    chosen = h.autotune(table, qs)
+    assert chosen["tile_shift"] in (3, 4, 5)
+    checked = oracle_checked(table, lib, [("voronoi260", rj.RASTER, 10.0, rj.AVG, []),
+                                          ("tracts2200", rj.RASTER, 50.0, rj.COUNT, []),
+                                          ("voronoi40", rj.RASTER, 100.0, rj.AVG, [(2, rj.LTE, 2)]),
+                                          ("tracts2200", rj.HYBRID, 20.0, rj.AVG, []),
+                                          ("voronoi900", rj.HYBRID, 10.0, rj.COUNT, [(2, rj.GTE, 3)]),
+                                          ("voronoi10", rj.HYBRID, 5.0, rj.AVG, [])])
+    allq = qs + [c[0] for c in checked]
+    alln = nps + [c[1] for c in checked]
+    batch = h.execute_batch(table, allq, alln)
+    for i, (q, k, b) in enumerate(zip(allq, alln, batch)):
+        assert b.status == 0 and b.path == rj.PATH_FUSED, i
+        assert same(h.execute_query(table, q, k), b), i
+    for (q, k, want, agg), b in zip(checked, batch[64:]):
+        check_oracle(b, want, agg)
+
+
+def test_serving_1B_mix_with_5000_polygons(big):
+    """The serving loop on the 1B-row table: Poisson arrivals at 250 qps of
+    the serving mix (60/30/10 raster/hybrid/index, 10-5000 polygons, windows
+    log-uniform from 1 hour to 1 year, 30 % with a passengers filter), with
+    six row-range queries interleaved: every served result equals the query
+    run alone, the row-range ones equal the oracle."""
+    h, table, lib = big
+    rng = np.random.default_rng(43)
+    names = list(lib)
+    qs, nps = [], []
+    for _ in range(300):
+        ps = lib[names[int(rng.integers(len(names)))]]
+        u = rng.random()
+        fn = rj.RASTER if u < 0.6 else (rj.HYBRID if u < 0.9 else rj.INDEX)
+        eps = float((10.0, 20.0, 50.0, 100.0)[int(rng.integers(4))])
+        cons = window_frac(rng, 3600.0 / YEAR, 1.0)
+        if rng.random() < 0.3:
+            cons.append((2, rj.LTE, int(rng.integers(1, 7))))
+        agg = rj.AVG if rng.random() < 0.5 else rj.COUNT
+        qs.append(rj.make_query(ps, fn, agg, 1 if agg == rj.AVG else -1, cons, eps=eps, index_res=256))
+        nps.append(ps.np)
+    assert any(k == 5000 for k in nps) and any(k == 10 for k in nps)
+    checked = oracle_checked(table, lib, [("voronoi5000", rj.RASTER, 20.0, rj.AVG, []),
+                                          ("voronoi5000", rj.HYBRID, 10.0, rj.COUNT, [(2, rj.LTE, 3)]),
+                                          ("tracts2200", rj.INDEX, 10.0, rj.AVG, []),
+                                          ("voronoi260", rj.INDEX, 10.0, rj.COUNT, [(2, rj.GTE, 2)]),
+                                          ("voronoi40", rj.RASTER, 50.0, rj.COUNT, []),
+                                          ("voronoi900", rj.HYBRID, 20.0, rj.AVG, [])])
+    for j, c in enumerate(checked):
+        qs.insert(40 * j + 7, c[0])
+        nps.insert(40 * j + 7, c[1])
+    alone = [h.execute_query(table, q, k) for q, k in zip(qs, nps)]
+    srv = rj.Server(h, table)
+    tickets = []
+
+    def client():
+        t_next = time.perf_counter()
+        for q, k in zip(qs, nps):
+            t_next += rng.exponential(1.0 / 250.0)
+            d = t_next - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets.append(srv.submit(q, k))
+
+    th = threading.Thread(target=client)
+    th.start()
+    th.join()
+    outs = [srv.poll(tk, wait_ms=120000) for tk in tickets]
+    st = srv.stats()
+    srv.close()
+    for i, (a, b) in enumerate(zip(alone, outs)):
+        assert b is not None, i
+        assert same(a, b), i
+    assert st["completed"] == len(qs) and st["failed"] == 0
+    assert 0 < st["p50_ms"] <= st["p99_ms"]
+    for j, (q, k, want, agg) in enumerate(checked):
+        check_oracle(outs[40 * j + 7], want, agg)
+    print(f"served {len(qs)} queries at 250 qps offered: p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms, "
+          f"mean batch {st['mean_batch']:.2f}")
