@@ -250,6 +250,17 @@ int rj_server_stats_reset(rj_server* s);
 /* Finishes every admitted query, then stops the thread. */
 int rj_server_destroy(rj_server* s);
 
+/* The CPUJoin grid index of a collection at resolution res as the engine
+ * builds it on the device (GridIndexF::buildGrid, CPUJoin/GridIndexF.cpp:60-155;
+ * the reference's own GPU index is GLFunction::createPolyIndex,
+ * GLFunction.cpp:65-156): cell_off (res*res + 1 entries, cell = y*res + x) and
+ * the candidate lists cand (*ncand entries, polygons ascending within a cell),
+ * and the *ndef cells whose list misses a polygon that meets the cell
+ * (deficient, ascending). Builds the grid when it is not resident. Arrays may
+ * be NULL (then only the sizes are returned). A test and measurement aid. */
+int rj_polyset_grid(rj_ctx* ctx, rj_polyset* ps, int32_t res, int32_t* cell_off, int32_t* cand, int64_t* ncand,
+                    int32_t* deficient, int64_t* ndef);
+
 /* Canvas geometry of the current polygons + accuracy (RasterJoin.cpp:51-78). */
 int rj_canvas_info(rj_ctx* ctx, int32_t* W, int32_t* H);
 

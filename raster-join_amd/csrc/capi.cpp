@@ -664,6 +664,27 @@ int rj_time_stats(rj_ctx* ctx, int32_t fn, char* buf, size_t len)
     return RJ_OK;
 }
 
+int rj_polyset_grid(rj_ctx* ctx, rj_polyset* ps, int32_t res, int32_t* cell_off, int32_t* cand, int64_t* ncand,
+                    int32_t* deficient, int64_t* ndef)
+{
+    RJ_GUARD(ctx);
+    if (!ps || ps->ctx != ctx || res < 1 || res > 16384) return ctx->fail(RJ_ERR_ARG, "bad collection or resolution");
+    Grid* G = nullptr;
+    int rc = ctx->grid_for(ps, res, &G);
+    if (rc) return rc;
+    if (ncand) *ncand = G->ncand;
+    if (ndef) *ndef = (int64_t)G->g.deficient.size();
+    hipError_t e = hipSuccess;
+    if (cell_off)
+        e = hipMemcpyAsync(cell_off, G->d_cell_off.p, ((size_t)res * res + 1) * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (cand && G->ncand && e == hipSuccess)
+        e = hipMemcpyAsync(cand, G->d_cand.p, (size_t)G->ncand * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "grid export");
+    if (deficient) std::copy(G->g.deficient.begin(), G->g.deficient.end(), deficient);
+    return RJ_OK;
+}
+
 int rj_canvas_info(rj_ctx* ctx, int32_t* W, int32_t* H)
 {
     RJ_GUARD(ctx);

@@ -449,28 +449,12 @@ int rj_ctx::grid_for(rj_polyset* P, int res, Grid** out)
     }
     G->last_use = ++clock;
     double t0 = now_ms();
-    build_grid(P->ps, res, &G->g);
+    int rc = grid_device(P, res, G);
     cur_polyindex += now_ms() - t0;
-    // row bands in one buffer: band_lo | band_hi | pband | band_off
-    const GridIndex& gi = G->g;
-    const size_t np = gi.band_lo.size();
-    std::vector<int32_t> bands;
-    bands.reserve(3 * np + 1 + gi.band_off.size());
-    bands.insert(bands.end(), gi.band_lo.begin(), gi.band_lo.end());
-    bands.insert(bands.end(), gi.band_hi.begin(), gi.band_hi.end());
-    bands.insert(bands.end(), gi.pband.begin(), gi.pband.end());
-    bands.insert(bands.end(), gi.band_off.begin(), gi.band_off.end());
-    int rc;
-    if ((rc = upload(G->d_cell_off, gi.cell_off.data(), gi.cell_off.size() * 4, "grid cells")) ||
-        (rc = upload(G->d_cand, gi.cand.data(), gi.cand.size() * 4, "grid candidates")) ||
-        (rc = upload(G->d_bands, bands.data(), bands.size() * 4, "grid row bands")) ||
-        (rc = upload(G->d_edges, gi.edges.data(), gi.edges.size() * 4, "grid band edges"))) {
+    if (rc) {
         P->grids.erase(res);
         return rc;
     }
-    // the host vectors above are read by the copies: wait before they go away
-    hipError_t e = hipStreamSynchronize(stream);
-    if (e != hipSuccess) return hip_fail(e, "grid upload");
     for (auto& c : P->canvases)
         if (c.second->hyb_res == res) c.second->hyb_res = -1;
     *out = G;
@@ -698,7 +682,7 @@ DevGrid rj_ctx::dev_grid(const Grid* G) const
     g.band_h = G->g.band_h;
     g.band_rows = G->g.band_rows;
     const int32_t* b = G->d_bands.as<int32_t>();
-    const size_t np = G->g.band_lo.size();
+    const size_t np = (size_t)G->np;
     g.band_lo = b;
     g.band_hi = b + np;
     g.pband = b + 2 * np;

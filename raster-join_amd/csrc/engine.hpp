@@ -68,6 +68,8 @@ struct Canvas {
 
 struct Grid {
     GridIndex g;
+    int np = 0;         // polygons of the collection
+    int64_t ncand = 0;  // entries of the candidate lists
     DevBuf d_cell_off, d_cand, d_bands, d_edges;  // d_bands: band_lo | band_hi | pband | band_off
     uint64_t last_use = 0;
     size_t device_bytes() const { return d_cell_off.bytes + d_cand.bytes + d_bands.bytes + d_edges.bytes; }
@@ -305,6 +307,8 @@ struct rj_ctx {
                      std::unique_ptr<rj_polyset>* out);
     int canvas_for(rj_polyset* P, double eps_m, double gres, rj::Canvas** out);
     int grid_for(rj_polyset* P, int res, rj::Grid** out);
+    // builds grid G of P at resolution res on the device (gridbuild.hip)
+    int grid_device(rj_polyset* P, int res, rj::Grid* G);
     int raster_labels(rj_polyset* P, rj::Canvas* C, bool* fresh);
     int hybrid_labels(rj_polyset* P, rj::Canvas* C, rj::Grid* G);
     int build_labels(rj_polyset* P, rj::Canvas* C, void* lab, int label16, rj::DevBuf& tiles, int tshift,

@@ -1,0 +1,528 @@
+// gridbuild.hip -- the CPUJoin grid index of a polygon collection, built on
+// the device: the candidate polygons of every grid cell
+// (GridIndexF::buildGrid, CPUJoin/GridIndexF.cpp:60-155, with the cell test
+// polygonRectIntersectionCPU, :461-496), the cells whose candidate list
+// misses a polygon that meets them, and the edge bands of the exact test.
+// The reference builds its GPU index the same way in two dispatches, sizes
+// then lists (GLFunction::createPolyIndex, GLFunction.cpp:65-156;
+// index-size.glsl, create-index.glsl), with the GL shader's exclusive cell
+// loop; here the cell range and the test follow CPUJoin, whose lists decide
+// the index and hybrid joins' results.
+//
+// Compiled with -ffp-contract=off: the cell bounds and the cell test must
+// round as CPUJoin's float code does, and the candidate lists come out in
+// CPUJoin's order (cell-major, polygons ascending).
+//
+//   1. k_grid_cells: one workgroup per block of a polygon's bounding-box
+//      cells (its vertices in LDS); each thread tests one cell and appends
+//      (cell, polygon) for a candidate, the cell for a deficient one.
+//   2. radix sort of the (cell << 32 | polygon) keys; k_grid_offsets writes
+//      the candidate list and every cell's first entry.
+//   3. edge bands: k_edge_rows gives every edge its band range (and each
+//      polygon its first and last band), then one thread per (polygon, band)
+//      counts and fills the band's edges in edge order.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace rj {
+namespace {
+
+constexpr int kGridLdsVerts = 4096;  // vertices a workgroup stages in LDS
+constexpr int kGridItemCells = 2048; // bounding-box cells of one work item
+
+struct GridItem {
+    int32_t poly;
+    int32_t first;  // first cell of the polygon's box, numbered (x - stx) * hy + (y - sty)
+    int32_t count;
+    int32_t pad;
+};
+
+// Crossing number, GridIndexF::isInsidePoly (GridIndexF.cpp:383-402).
+__device__ __forceinline__ bool pip(const float* v, int n, float tx, float ty)
+{
+    bool c = false;
+    for (int i = 0, j = n - 1; i < n; j = i++) {
+        const float vix = v[2 * i], viy = v[2 * i + 1], vjx = v[2 * j], vjy = v[2 * j + 1];
+        if (((viy > ty) != (vjy > ty)) && (tx < (vjx - vix) * (ty - viy) / (vjy - viy) + vix)) c = !c;
+    }
+    return c;
+}
+
+// Cohen-Sutherland region code, GridIndexF::ComputeOutCode (:499-511).
+__device__ __forceinline__ int region_code(float x, float y, float lbx, float lby, float rtx, float rty)
+{
+    int code = 0;
+    if (x < lbx) code |= 1;
+    else if (x > rtx) code |= 2;
+    if (y < lby) code |= 4;
+    else if (y > rty) code |= 8;
+    return code;
+}
+
+// Ray from o along d against segment p0-p1, GridIndexF::rayToLineSegmentIntersection
+// (:513-549): parallel when the two slopes compare equal, hit when r >= 0 and
+// 0 <= s <= 1; the hit point is o + d*r.
+__device__ __forceinline__ bool ray_hits(float ox, float oy, float dx, float dy, float p0x, float p0y, float p1x,
+                                         float p1y, float* hx, float* hy)
+{
+    const float x1 = ox, y1 = oy, x2 = ox + dx, y2 = oy + dy;
+    if ((y2 - y1) / (x2 - x1) == (p1y - p0y) / (p1x - p0x)) return false;
+    const float d = ((x2 - x1) * (p1y - p0y)) - (y2 - y1) * (p1x - p0x);
+    if (d == 0) return false;
+    const float r = (((y1 - p0y) * (p1x - p0x)) - (x1 - p0x) * (p1y - p0y)) / d;
+    const float s = (((y1 - p0y) * (x2 - x1)) - (x1 - p0x) * (y2 - y1)) / d;
+    if (!(r >= 0) || !(s >= 0 && s <= 1)) return false;
+    *hx = ox + dx * r;
+    *hy = oy + dy * r;
+    return true;
+}
+
+// GridIndexF::lineRectIntersection (:404-424); the second test compares the
+// hit's y with the rectangle's right x, as the reference does (:412).
+__device__ __forceinline__ bool segment_meets_cell(float ax, float ay, float bx, float by, float lbx, float lby,
+                                                   float rtx, float rty)
+{
+    float hx, hy;
+    if (ray_hits(lbx, lby, 0.0f, 1.0f, ax, ay, bx, by, &hx, &hy) && hy <= rty) return true;
+    if (ray_hits(lbx, lby, 1.0f, 0.0f, ax, ay, bx, by, &hx, &hy) && hy <= rtx) return true;
+    if (ray_hits(rtx, rty, 0.0f, -1.0f, ax, ay, bx, by, &hx, &hy) && hy >= lby) return true;
+    if (ray_hits(rtx, rty, -1.0f, 0.0f, ax, ay, bx, by, &hx, &hy) && hx >= lbx) return true;
+    return false;
+}
+
+// GridIndexF::polygonRectIntersectionCPU (:461-496): a vertex in the cell, a
+// cell corner in the polygon, or one of the edges (i, i+1), i < n-1, meeting
+// the cell (the closing edge is not tested).
+__device__ bool cell_candidate(const float* v, int n, float lbx, float lby, float rtx, float rty)
+{
+    for (int i = 0; i < n; i++) {
+        const float x = v[2 * i], y = v[2 * i + 1];
+        if (x >= lbx && x <= rtx && y >= lby && y <= rty) return true;
+    }
+    if (pip(v, n, lbx, lby) || pip(v, n, rtx, rty) || pip(v, n, lbx, rty) || pip(v, n, rtx, lby)) return true;
+    int code_b = region_code(v[0], v[1], lbx, lby, rtx, rty);
+    for (int i = 0; i + 1 < n; i++) {
+        const int code_a = code_b;
+        code_b = region_code(v[2 * i + 2], v[2 * i + 3], lbx, lby, rtx, rty);
+        if (!(code_a & code_b) &&
+            segment_meets_cell(v[2 * i], v[2 * i + 1], v[2 * i + 2], v[2 * i + 3], lbx, lby, rtx, rty))
+            return true;
+    }
+    return false;
+}
+
+// Does the closed polygon meet the closed cell, grown by a relative margin of
+// 1e-6 (double precision)? A cell that meets a polygon that is not among its
+// candidates is deficient: the hybrid join sends its points to the exact test.
+__device__ bool polygon_meets_rect(const float* v, int n, double lbx, double lby, double rtx, double rty)
+{
+    const double mx = 1e-6 * (fabs(rtx) + fabs(lbx) + (rtx - lbx));
+    const double my = 1e-6 * (fabs(rty) + fabs(lby) + (rty - lby));
+    lbx -= mx; rtx += mx; lby -= my; rty += my;
+    for (int i = 0; i < n; i++) {
+        const double x = v[2 * i], y = v[2 * i + 1];
+        if (x >= lbx && x <= rtx && y >= lby && y <= rty) return true;
+    }
+    {  // the cell's center inside (the cell may lie in the polygon's interior)
+        const double cx = 0.5 * (lbx + rtx), cy = 0.5 * (lby + rty);
+        bool c = false;
+        for (int i = 0, j = n - 1; i < n; j = i++) {
+            const double xi = v[2 * i], yi = v[2 * i + 1], xj = v[2 * j], yj = v[2 * j + 1];
+            if (((yi > cy) != (yj > cy)) && (cx < (xj - xi) * (cy - yi) / (yj - yi) + xi)) c = !c;
+        }
+        if (c) return true;
+    }
+    // any edge, the closing one included, crossing the cell (Liang-Barsky clip)
+    for (int i = 0, j = n - 1; i < n; j = i++) {
+        const double x0 = v[2 * j], y0 = v[2 * j + 1], dx = v[2 * i] - x0, dy = v[2 * i + 1] - y0;
+        double t0 = 0, t1 = 1;
+        const double pp[4] = {-dx, dx, -dy, dy};
+        const double qq[4] = {x0 - lbx, rtx - x0, y0 - lby, rty - y0};
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (!ok) break;
+            if (pp[k] == 0) {
+                if (qq[k] < 0) ok = false;
+            } else {
+                const double t = qq[k] / pp[k];
+                if (pp[k] < 0) t0 = (t0 < t) ? t : t0;
+                else t1 = (t < t1) ? t : t1;
+                if (t0 > t1) ok = false;
+            }
+        }
+        if (ok) return true;
+    }
+    return false;
+}
+
+struct GridArgs {
+    const float* xy;         // collection vertices (CSR)
+    const int32_t* off;
+    const int32_t* mbr;      // per polygon: stx, enx, sty, eny
+    int32_t xs;
+    float minx, miny, width, height;
+    unsigned long long* keys;  // candidates: cell << 32 | polygon
+    int32_t* defi;             // deficient cells
+    unsigned long long* count; // [0] candidates, [1] deficient cells
+    int64_t cap;               // room of keys and defi
+};
+
+// Append the lanes' flagged values: one atomic per wave.
+template <class T>
+__device__ __forceinline__ void wave_append(bool flag, T val, T* out, unsigned long long* counter, int64_t cap)
+{
+    const unsigned long long m = __ballot(flag);
+    if (!m) return;
+    const int lane = (int)__lane_id();
+    const int lead = __ffsll((long long)m) - 1;
+    unsigned long long base = 0;
+    if (lane == lead) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+    base = __shfl(base, lead, 64);
+    const unsigned long long below = lane ? (m & (~0ull >> (64 - lane))) : 0ull;
+    const unsigned long long pos = base + (unsigned long long)__popcll(below);
+    if (flag && (int64_t)pos < cap) out[pos] = val;
+}
+
+__global__ __launch_bounds__(256) void k_grid_cells(GridArgs a, const GridItem* __restrict__ items)
+{
+    __shared__ float sv[2 * kGridLdsVerts];
+    const GridItem it = items[blockIdx.x];
+    const int p = it.poly;
+    const int v0 = a.off[p], n = a.off[p + 1] - v0;
+    const float* v = a.xy + 2 * (int64_t)v0;
+    if (n <= kGridLdsVerts) {
+        for (int k = threadIdx.x; k < 2 * n; k += blockDim.x) sv[k] = v[k];
+        __syncthreads();
+        v = sv;
+    }
+    const int stx = a.mbr[4 * p], sty = a.mbr[4 * p + 2], eny = a.mbr[4 * p + 3];
+    const int hy = eny - sty + 1;
+    // every lane takes part in the appends: the loop runs to a multiple of the block
+    const int rounds = (it.count + (int)blockDim.x - 1) / (int)blockDim.x;
+    for (int r = 0; r < rounds; r++) {
+        const int i = r * (int)blockDim.x + (int)threadIdx.x;
+        bool cand = false, def = false;
+        int cell = 0;
+        if (i < it.count) {
+            const int local = it.first + i;
+            const int x = stx + local / hy, y = sty + local % hy;
+            // cell bounds as GridIndexF::buildGrid computes them (:96-99)
+            const float lbx = (float)x * a.width + a.minx, lby = (float)y * a.height + a.miny;
+            const float rtx = (float)(x + 1) * a.width + a.minx, rty = (float)(y + 1) * a.height + a.miny;
+            cell = y * a.xs + x;
+            cand = cell_candidate(v, n, lbx, lby, rtx, rty);
+            if (!cand) def = polygon_meets_rect(v, n, lbx, lby, rtx, rty);
+        }
+        wave_append<unsigned long long>(cand, ((unsigned long long)(uint32_t)cell << 32) | (uint32_t)p, a.keys,
+                                        &a.count[0], a.cap);
+        wave_append<int32_t>(def, cell, a.defi, &a.count[1], a.cap);
+    }
+}
+
+// From the sorted keys: the candidate list and, for every cell c, the index
+// of its first candidate (keys of cells >= c start there). Thread k writes the
+// cells between the previous key's cell and its own; thread n the cells after
+// the last key.
+__global__ void k_grid_offsets(const unsigned long long* __restrict__ keys, int64_t n, int64_t ncells,
+                               int32_t* __restrict__ cell_off, int32_t* __restrict__ cand)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > n) return;
+    const int64_t prev = k == 0 ? -1 : (int64_t)(keys[k - 1] >> 32);
+    const int64_t cur = k < n ? (int64_t)(keys[k] >> 32) : ncells;
+    if (k < n) cand[k] = (int32_t)(uint32_t)keys[k];
+    for (int64_t c = prev + 1; c <= cur; c++) cell_off[c] = (int32_t)k;
+}
+
+// Band range of every edge (vertex k and its predecessor), as the host's
+// band_row: the edge toggles the crossing test of a point at ty iff
+// min(y) <= ty < max(y), whose bands run from band_row(min) to band_row(the
+// largest float below max). Each polygon gets its first and last band.
+__global__ void k_edge_rows(const float* __restrict__ xy, const int32_t* __restrict__ prev,
+                            const int32_t* __restrict__ off, int np, int64_t nv, float miny, float band_h, int rows,
+                            int2* __restrict__ er, int32_t* __restrict__ lo, int32_t* __restrict__ hi,
+                            unsigned long long* __restrict__ total)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long span = 0;
+    if (k < nv) {
+        const float yi = xy[2 * k + 1], yj = xy[2 * (int64_t)prev[k] + 1];
+        const float y0 = fminf(yi, yj), y1 = fmaxf(yi, yj);
+        int r0 = 0, r1 = -1;
+        if (y0 < y1) {
+            r0 = band_row(y0, miny, band_h, rows);
+            r1 = band_row(nextafterf(y1, -INFINITY), miny, band_h, rows);
+        }
+        er[k] = make_int2(r0, r1);
+        if (r0 <= r1) {
+            int a = 0, b = np - 1;  // the polygon of vertex k: the last with off[p] <= k
+            while (a < b) {
+                const int m = (a + b + 1) >> 1;
+                if (off[m] <= k) a = m;
+                else b = m - 1;
+            }
+            atomicMin(&lo[a], r0);
+            atomicMax(&hi[a], r1);
+            span = (unsigned long long)(r1 - r0 + 1);
+        }
+    }
+    for (int d = 32; d > 0; d >>= 1) span += __shfl_xor(span, d, 64);
+    if (__lane_id() == 0 && span) atomicAdd(total, span);
+}
+
+// One thread per band of one polygon: the number of its edges (FILL = false)
+// or the edges themselves in edge order, (vi, vj) as float4 (FILL = true).
+template <bool FILL>
+__global__ void k_band_edges(const float* __restrict__ xy, const int32_t* __restrict__ off,
+                             const int32_t* __restrict__ prev, const int2* __restrict__ er,
+                             const int32_t* __restrict__ bpoly, const int32_t* __restrict__ blo,
+                             const int32_t* __restrict__ pband, int64_t nbands, int32_t* __restrict__ band_off,
+                             float4* __restrict__ edges)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbands) return;
+    const int p = bpoly[b];
+    const int y = blo[p] + (int)(b - pband[p]);
+    int64_t w = FILL ? band_off[b] : 0;
+    int cnt = 0;
+    for (int k = off[p]; k < off[p + 1]; k++) {
+        const int2 r = er[k];
+        if (y < r.x || y > r.y) continue;
+        if (FILL) {
+            const int j = prev[k];
+            edges[w++] = make_float4(xy[2 * (int64_t)k], xy[2 * (int64_t)k + 1], xy[2 * (int64_t)j], xy[2 * (int64_t)j + 1]);
+        } else {
+            cnt++;
+        }
+    }
+    if (!FILL) band_off[b + 1] = cnt;
+}
+
+}  // namespace
+}  // namespace rj
+
+using namespace rj;
+
+// CPUJoin grid of collection P at resolution res, built on the device into G
+// (its device buffers, scalars and deficient cells).
+int rj_ctx::grid_device(rj_polyset* P, int res, Grid* G)
+{
+    const PolySet& ps = P->ps;
+    const int np = ps.np;
+    GridIndex& g = G->g;
+    g.xs = g.ys = res;
+    g.minx = ps.bboxf[0];
+    g.miny = ps.bboxf[1];
+    g.width = (ps.bboxf[2] - ps.bboxf[0]) / (float)res;  // GridIndexF.cpp:66-67
+    g.height = (ps.bboxf[3] - ps.bboxf[1]) / (float)res;
+    g.deficient.clear();
+    const int64_t ncells = (int64_t)res * res;
+    int rc;
+
+    // each polygon's box of cells (GridIndexF.cpp:82-91) and the work items
+    std::vector<int32_t> mbr(4 * (size_t)np);
+    std::vector<GridItem> items;
+    int64_t total_cells = 0;
+    for (int p = 0; p < np; p++) {
+        const float* b = &ps.pbox[4 * (size_t)p];
+        int stx = (int)((b[0] - g.minx) / g.width);
+        int enx = (int)((b[2] - g.minx) / g.width) + 1;
+        int sty = (int)((b[1] - g.miny) / g.height);
+        int eny = (int)((b[3] - g.miny) / g.height) + 1;
+        if (enx >= res) enx = res - 1;
+        if (eny >= res) eny = res - 1;
+        if (stx < 0) stx = 0;
+        if (sty < 0) sty = 0;
+        mbr[4 * (size_t)p] = stx; mbr[4 * (size_t)p + 1] = enx;
+        mbr[4 * (size_t)p + 2] = sty; mbr[4 * (size_t)p + 3] = eny;
+        const int64_t cells = (enx >= stx && eny >= sty) ? (int64_t)(enx - stx + 1) * (eny - sty + 1) : 0;
+        for (int64_t c = 0; c < cells; c += kGridItemCells)
+            items.push_back({p, (int32_t)c, (int32_t)std::min<int64_t>(kGridItemCells, cells - c), 0});
+        total_cells += cells;
+    }
+    if (total_cells >= ((int64_t)1 << 31) || ncells >= ((int64_t)1 << 31))
+        return fail(RJ_ERR_LIMIT, "grid too fine for this collection");
+
+    DevBuf d_mbr, d_items, d_keys, d_keys2, d_defi, d_count, d_tmp;
+    const int64_t cap = std::max<int64_t>(1, total_cells);
+    if ((rc = upload(d_mbr, mbr.data(), mbr.size() * 4, "grid boxes")) ||
+        (rc = upload(d_items, items.data(), std::max<size_t>(1, items.size()) * sizeof(GridItem), "grid items")) ||
+        (rc = alloc(d_keys, (size_t)cap * 8, "grid keys")) || (rc = alloc(d_defi, (size_t)cap * 4, "grid cells")) ||
+        (rc = alloc(d_count, 16, "grid counters")))
+        return rc;
+    unsigned long long* cnt = d_count.as<unsigned long long>();
+    if ((rc = launch("grid_clear", [&] { launch_fill_u32(stream, reinterpret_cast<uint32_t*>(cnt), 4, 0); })))
+        return rc;
+    GridArgs a{P->d_xy.as<float>(), P->d_off.as<int32_t>(), d_mbr.as<int32_t>(), res, g.minx, g.miny, g.width,
+               g.height, d_keys.as<unsigned long long>(), d_defi.as<int32_t>(), cnt, cap};
+    if (!items.empty() &&
+        (rc = launch("grid_cells", [&] {
+             hipLaunchKernelGGL(k_grid_cells, dim3((unsigned)items.size()), dim3(256), 0, stream, a, d_items.as<GridItem>());
+         })))
+        return rc;
+    unsigned long long hc[2] = {0, 0};
+    hipError_t e = hipMemcpyAsync(hc, cnt, 16, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "grid cells");
+    const int64_t nk = (int64_t)hc[0], nd = (int64_t)hc[1];
+    if (nk > cap || nd > cap) return fail(RJ_ERR_STATE, "grid build overflow");
+    G->ncand = nk;
+    if (nd) {
+        g.deficient.resize(nd);
+        e = hipMemcpyAsync(g.deficient.data(), d_defi.p, nd * 4, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return hip_fail(e, "grid deficient cells");
+        std::sort(g.deficient.begin(), g.deficient.end());
+        g.deficient.erase(std::unique(g.deficient.begin(), g.deficient.end()), g.deficient.end());
+    }
+    d_defi.release();  // temporaries go as soon as they are read: a tight budget holds the peak
+
+    // candidates sorted by (cell, polygon): CPUJoin's lists
+    if (nk == 0) {
+        if ((rc = alloc(G->d_cell_off, (size_t)(ncells + 1) * 4, "grid cells")) ||
+            (rc = alloc(G->d_cand, 4, "grid candidates")))
+            return rc;
+        int32_t* co = G->d_cell_off.as<int32_t>();
+        if ((rc = launch("grid_offsets", [&] { launch_fill_u32(stream, (uint32_t*)co, ncells + 1, 0); }))) return rc;
+    } else {
+        int cell_bits = 1;
+        while (((int64_t)1 << cell_bits) < ncells) cell_bits++;
+        size_t tmp = 0;
+        unsigned long long* k0 = d_keys.as<unsigned long long>();
+        if ((rc = alloc(d_keys2, (size_t)nk * 8, "grid keys"))) return rc;
+        unsigned long long* k1 = d_keys2.as<unsigned long long>();
+        (void)hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, k0, k1, (int)nk, 0, 32 + cell_bits, stream);
+        if ((rc = alloc(d_tmp, tmp + 16, "grid sort"))) return rc;
+        e = hipcub::DeviceRadixSort::SortKeys(d_tmp.p, tmp, k0, k1, (int)nk, 0, 32 + cell_bits, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return hip_fail(e, "grid sort");
+        d_tmp.release();
+        d_keys.release();
+        if ((rc = alloc(G->d_cell_off, (size_t)(ncells + 1) * 4, "grid cells")) ||
+            (rc = alloc(G->d_cand, (size_t)nk * 4, "grid candidates")))
+            return rc;
+        int32_t* co = G->d_cell_off.as<int32_t>();
+        int32_t* cd = G->d_cand.as<int32_t>();
+        if ((rc = launch("grid_offsets", [&] {
+                 hipLaunchKernelGGL(k_grid_offsets, dim3((unsigned)((nk + 1 + 255) / 256)), dim3(256), 0, stream, k1,
+                                    nk, ncells, co, cd);
+             })))
+            return rc;
+    }
+    e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "grid offsets");
+    d_keys.release();
+    d_keys2.release();
+
+    // edge bands: the grid rows, split k ways for about 4 edges per band
+    const int64_t nv = ps.off[np];
+    DevBuf d_er, d_lohi, d_bpoly, d_scan, d_bcnt;
+    if ((rc = alloc(d_er, (size_t)std::max<int64_t>(1, nv) * 8, "edge rows")) ||
+        (rc = alloc(d_lohi, (size_t)np * 8 + 16, "band ranges")))
+        return rc;
+    int2* er = d_er.as<int2>();
+    int32_t* dlo = d_lohi.as<int32_t>();
+    int32_t* dhi = dlo + np;
+    std::vector<int32_t> lo(np), hi(np);
+    auto edge_rows = [&](float band_h, int rows, unsigned long long* total) -> int {
+        int r2;
+        if ((r2 = launch("band_init", [&] {
+                 launch_fill_u32(stream, (uint32_t*)dlo, np, (uint32_t)INT_MAX);
+                 launch_fill_u32(stream, (uint32_t*)dhi, np, 0xFFFFFFFFu);
+                 launch_fill_u32(stream, (uint32_t*)cnt, 4, 0);
+             })))
+            return r2;
+        const int32_t* prev = P->d_prev.as<int32_t>();
+        if ((r2 = launch("edge_rows", [&] {
+                 hipLaunchKernelGGL(k_edge_rows, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, stream,
+                                    P->d_xy.as<float>(), prev, P->d_off.as<int32_t>(), np, nv, g.miny, band_h, rows,
+                                    er, dlo, dhi, cnt);
+             })))
+            return r2;
+        hipError_t e2 = hipMemcpyAsync(lo.data(), dlo, (size_t)np * 4, hipMemcpyDeviceToHost, stream);
+        if (e2 == hipSuccess) e2 = hipMemcpyAsync(hi.data(), dhi, (size_t)np * 4, hipMemcpyDeviceToHost, stream);
+        if (e2 == hipSuccess) e2 = hipMemcpyAsync(total, cnt, 8, hipMemcpyDeviceToHost, stream);
+        if (e2 == hipSuccess) e2 = hipStreamSynchronize(stream);
+        if (e2 != hipSuccess) return hip_fail(e2, "edge rows");
+        // polygons without a band-crossing edge: no bands (lo 0, hi -1)
+        for (int p = 0; p < np; p++)
+            if (hi[p] < lo[p]) {
+                lo[p] = 0;
+                hi[p] = -1;
+            }
+        return RJ_OK;
+    };
+    unsigned long long entries = 0;
+    if ((rc = edge_rows(g.height, res, &entries))) return rc;
+    int64_t nb = 0;
+    for (int p = 0; p < np; p++) nb += hi[p] - lo[p] + 1;
+    const double per = (double)entries / (double)std::max<int64_t>(1, nb);
+    int ksplit = 1;
+    while (ksplit < 64 && per / ksplit > 4.0) ksplit *= 2;
+    g.band_h = g.height;
+    g.band_rows = res;
+    if (ksplit > 1) {
+        g.band_h = g.height / (float)ksplit;
+        g.band_rows = res * ksplit;
+        if ((rc = edge_rows(g.band_h, g.band_rows, &entries))) return rc;
+    }
+    // band_lo | band_hi | pband (np + 1) | band_off (nbands + 1)
+    std::vector<int32_t> pband(np + 1, 0);
+    for (int p = 0; p < np; p++) pband[p + 1] = pband[p] + (hi[p] - lo[p] + 1);
+    const int64_t nbands = pband[np];
+    std::vector<int32_t> bpoly((size_t)std::max<int64_t>(1, nbands));
+    for (int p = 0; p < np; p++)
+        for (int b = pband[p]; b < pband[p + 1]; b++) bpoly[b] = p;
+    G->np = np;
+    if ((rc = alloc(G->d_bands, ((size_t)3 * np + 1 + nbands + 1) * 4, "grid row bands"))) return rc;
+    int32_t* bands = G->d_bands.as<int32_t>();
+    e = hipMemcpyAsync(bands, lo.data(), (size_t)np * 4, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(bands + np, hi.data(), (size_t)np * 4, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(bands + 2 * np, pband.data(), (size_t)(np + 1) * 4, hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) return hip_fail(e, "grid row bands");
+    if ((rc = upload(d_bpoly, bpoly.data(), bpoly.size() * 4, "band polygons"))) return rc;
+    int32_t* band_off = bands + 3 * np + 1;
+    const int32_t* blo = bands;
+    const int32_t* pb = bands + 2 * np;
+    const int32_t* bp = d_bpoly.as<int32_t>();
+    const unsigned nbk = (unsigned)((std::max<int64_t>(1, nbands) + 255) / 256);
+    if ((rc = alloc(d_bcnt, (size_t)(std::max<int64_t>(1, nbands) + 1) * 4, "band counts"))) return rc;
+    int32_t* bcnt = d_bcnt.as<int32_t>();  // bcnt[b + 1]: edges of band b
+    if ((rc = launch("band_count", [&] {
+             launch_fill_u32(stream, (uint32_t*)band_off, 1, 0);
+             if (nbands)
+                 hipLaunchKernelGGL(k_band_edges<false>, dim3(nbk), dim3(256), 0, stream, P->d_xy.as<float>(),
+                                    P->d_off.as<int32_t>(), P->d_prev.as<int32_t>(), er, bp, blo, pb, nbands, bcnt,
+                                    nullptr);
+         })))
+        return rc;
+    int32_t nedges = 0;
+    if (nbands) {
+        size_t tmp = 0;
+        (void)hipcub::DeviceScan::InclusiveSum(nullptr, tmp, bcnt + 1, band_off + 1, (int)nbands, stream);
+        if ((rc = alloc(d_scan, tmp + 16, "band scan"))) return rc;
+        e = hipcub::DeviceScan::InclusiveSum(d_scan.p, tmp, bcnt + 1, band_off + 1, (int)nbands, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&nedges, band_off + nbands, 4, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return hip_fail(e, "band scan");
+    }
+    if ((rc = alloc(G->d_edges, (size_t)std::max<int32_t>(1, nedges) * 16, "grid band edges"))) return rc;
+    float4* ed = G->d_edges.as<float4>();
+    if (nbands && (rc = launch("band_fill", [&] {
+                       hipLaunchKernelGGL(k_band_edges<true>, dim3(nbk), dim3(256), 0, stream, P->d_xy.as<float>(),
+                                          P->d_off.as<int32_t>(), P->d_prev.as<int32_t>(), er, bp, blo, pb, nbands,
+                                          band_off, ed);
+                   })))
+        return rc;
+    // the temporaries above are freed when this returns: wait for the kernels
+    e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "grid build");
+    return RJ_OK;
+}

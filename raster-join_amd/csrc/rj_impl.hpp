@@ -52,32 +52,27 @@ void rasterize_polys(const PolySet& ps, const CanvasPlan& plan, PolyRaster* out)
 int64_t snap_x(const CanvasPlan& plan, float x);
 int64_t snap_y(const CanvasPlan& plan, float y);
 
-// CPUJoin grid index (CPUJoin/GridIndexF.cpp:60-155): cell = y*xs + x.
-// `deficient` lists cells whose candidate list misses a polygon that meets
-// the cell (possible through the reference's cell test); the hybrid join
-// treats their pixels as border pixels so its result stays CPUJoin's.
+// CPUJoin grid index (CPUJoin/GridIndexF.cpp:60-155): cell = y*xs + x, built
+// on the device (rj_ctx::grid_device); the host keeps its geometry and the
+// cells whose candidate list misses a polygon that meets the cell (possible
+// through the reference's cell test), which the hybrid join treats as border
+// pixels so that its result stays CPUJoin's.
 //
-// Edge bands: horizontal bands of height band_h from miny (band_rows of
-// them, band_row() below, finer than the grid's rows for polygons with many
-// edges per row). For polygon p and band y in [band_lo[p], band_hi[p]], the
-// edges (vi, vj) whose y-span [min, max) holds the y of some point that
-// band_row maps to y, as float4 (vi.x, vi.y, vj.x, vj.y) in
+// Edge bands (device arrays): horizontal bands of height band_h from miny
+// (band_rows of them, band_row(), finer than the grid's rows for polygons
+// with many edges per row). For polygon p and band y in [band_lo[p],
+// band_hi[p]], the edges (vi, vj) whose y-span [min, max) holds the y of some
+// point that band_row maps to y, as float4 (vi.x, vi.y, vj.x, vj.y) in
 // edges[band_off[pband[p] + y - band_lo[p]] ..). The crossing test of a point
 // in band y toggles only on those edges (every other edge fails its straddle
 // test), so testing them gives the full loop's result with far fewer edges.
 struct GridIndex {
     int32_t xs = 0, ys = 0;
     float minx = 0, miny = 0, width = 0, height = 0;
-    std::vector<int32_t> cell_off;
-    std::vector<int32_t> cand;
     std::vector<int32_t> deficient;
     float band_h = 0;
     int32_t band_rows = 0;
-    std::vector<int32_t> band_lo, band_hi, pband, band_off;
-    std::vector<float> edges;  // 4 floats per entry
 };
-// band_row (rj_dev.hpp) maps a y to its band.
-void build_grid(const PolySet& ps, int res, GridIndex* g);
 // Pixel rectangles (i0, i1, j0, j1) covering the deficient cells, one pixel margin.
 void deficient_rects(const GridIndex& g, const CanvasPlan& plan, std::vector<int32_t>* rects);
 

@@ -119,6 +119,7 @@ def lib():
             "rj_polyset_bytes": [vp, vp], "rj_polyset_destroy": [vp], "rj_bind_polyset": [vp, vp],
             "rj_execute_query": [vp, vp, vp, vp], "rj_execute_batch": [vp, vp, i32, vp, vp],
             "rj_execute_query_noalloc": [vp, vp, vp, vp],
+            "rj_polyset_grid": [vp, vp, i32, vp, vp, vp, vp, vp],
             "rj_autotune": [vp, vp, i32, vp, vp],
             "rj_server_create": [vp, vp, vp, vp], "rj_server_submit": [vp, vp, vp, vp],
             "rj_server_poll": [vp, C.c_uint64, i32], "rj_server_stats_get": [vp, vp],
@@ -293,6 +294,19 @@ class PolySet:
                                                C.byref(self.h)))
         self.np = len(off) - 1
         self.handler = handler
+
+    def grid(self, res):
+        """The device-built CPUJoin grid at resolution res: (cell_off,
+        cand, deficient cells)."""
+        nc, nd = C.c_int64(), C.c_int64()
+        h = self.handler
+        h._check(lib().rj_polyset_grid(h.ctx, self.h, res, None, None, C.byref(nc), None, C.byref(nd)))
+        cell_off = np.zeros(res * res + 1, np.int32)
+        cand = np.zeros(max(1, nc.value), np.int32)
+        defi = np.zeros(max(1, nd.value), np.int32)
+        h._check(lib().rj_polyset_grid(h.ctx, self.h, res, _p(cell_off), _p(cand), C.byref(nc), _p(defi),
+                                       C.byref(nd)))
+        return cell_off, cand[:nc.value], defi[:nd.value]
 
     def device_bytes(self):
         b = C.c_int64()

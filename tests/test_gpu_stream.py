@@ -143,7 +143,7 @@ def test_oversized_points_stream_in_passes():
     pts, t, fare, pas = points(2_000_000, 2)
     cols = (t, fare, pas)
     res = {}
-    for mb in (0, 110):  # unlimited, then 10 MB after the 100 MB reserve
+    for mb in (0, 116):  # unlimited, then 16 MB after the 100 MB reserve (26 MB of points)
         h = rj.Handler(gpu_mem_mb=mb)
         h.set_polygons(xy, off)
         h.set_points(pts, cols, COLS_TYPES)
