@@ -78,6 +78,29 @@ int rj_set_polygons(rj_ctx* ctx, const float* xy, const int32_t* off, int32_t np
 int rj_set_points(rj_ctx* ctx, int64_t n, const float* xy, int32_t ncols, const void* const* cols,
                   const int32_t* types);
 
+/* Point records of the reference's binary datasets, read as
+ * Dataset::getNextRecord reads them (db/Dataset.cpp:13-36): 88-byte
+ * TaxiRecordData records (db/TaxiRecord.hpp:15-46) or 32-byte Tweet records
+ * (db/TwitterRecord.hpp:19-28), native byte order. A record is kept when its
+ * index location (taxi: pick_y, pick_x; twitter: loc_x, loc_y; as lat, lon
+ * floats) passes transformPoint's US bounding-box filter
+ * (common/UsefulFuncs.cpp:578-597); the location is then the projected one
+ * (Web Mercator at zoom 22, minus the projection of the bbox center).
+ * loc_attrib is the location attribute (taxi: 1 pickup, 2 dropoff, which the
+ * reference does not project; twitter: 2); attribs[k] are 4-byte attributes,
+ * written to cols[k] as getAttributeAsBinary gives them, types[k] receiving
+ * their RJ_ATTR_* type: taxi 3 pickup time, 4 dropoff time, 6 duration, 8 fare
+ * (cents), 18 passengers (Uint), 7 miles (Float); twitter 1 time (Uint),
+ * 3 favorites, 4 retweets (Int). Other attributes: RJ_ERR_UNSUPPORTED.
+ * At most max_records records are read (0: all). With xy NULL only
+ * *n_records (the records that would be read) is set; otherwise xy (2 floats
+ * per row) and cols[k] need room for *n_records rows and *n_kept receives the
+ * rows kept. Host code only (no device work). */
+enum { RJ_DATASET_TAXI = 0, RJ_DATASET_TWITTER = 1 };
+int rj_read_points(const char* path, int32_t dataset, int32_t loc_attrib, int32_t nattr, const int32_t* attribs,
+                   int64_t max_records, int64_t* n_records, int64_t* n_kept, float* xy, void* const* cols,
+                   int32_t* types);
+
 /* Resident point tables (replacing the disk backend, db/HashGridIndex.cpp:78-137).
  * organize = 1 orders the rows by a spatial (Morton) key so that splats and
  * lookups stay local; join results do not depend on row order. */

@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <set>
 #include <sstream>
@@ -41,6 +42,48 @@ bool transform_point(float lat, float lon, float* tx, float* ty)
     *tx = (float)(px - cx);
     *ty = (float)(py - cy);
     return true;
+}
+
+// One record of the reference's binary datasets, the fields the join reads.
+// TaxiRecordData (db/TaxiRecord.hpp:15-46), 88 bytes: db_idx @0, pick_x @8,
+// pick_y @16, drop_x @24, drop_y @32 (doubles), pickup_time @40,
+// dropoff_time @48 (u64), vendor @56, duration @60 (u32), miles @64 (float),
+// fare @68 ... license_id @82 (u16), store_and_forward @84, payment_type @85,
+// passengers @86, rate_code @87 (u8). Tweet (db/TwitterRecord.hpp:19-28),
+// 32 bytes: user_id @0, time @8 (i64), loc_x @16, loc_y @20 (floats),
+// favCt @24, rtCt @28 (i32).
+constexpr size_t kTaxiRecord = 88, kTweetRecord = 32;
+
+template <class T>
+T field(const unsigned char* r, size_t at)
+{
+    T v;
+    std::memcpy(&v, r + at, sizeof(T));
+    return v;
+}
+
+// The 4-byte attribute `id` of a record as getAttributeAsBinary returns it
+// (TaxiRecord.hpp:107-262, TwitterRecord.hpp:57-100); type receives RJ_ATTR_*.
+// Returns false for an attribute that is not 4 bytes wide.
+bool attribute(int dataset, int id, const unsigned char* r, uint32_t* v, int32_t* type)
+{
+    if (dataset == RJ_DATASET_TAXI) {
+        switch (id) {
+        case 3: *v = (uint32_t)field<uint64_t>(r, 40); *type = RJ_ATTR_UINT; return true;
+        case 4: *v = (uint32_t)field<uint64_t>(r, 48); *type = RJ_ATTR_UINT; return true;
+        case 6: *v = field<uint32_t>(r, 60); *type = RJ_ATTR_UINT; return true;
+        case 7: *v = field<uint32_t>(r, 64); *type = RJ_ATTR_FLOAT; return true;
+        case 8: *v = (uint32_t)field<uint16_t>(r, 68); *type = RJ_ATTR_UINT; return true;
+        case 18: *v = (uint32_t)field<uint8_t>(r, 86); *type = RJ_ATTR_UINT; return true;
+        default: return false;
+        }
+    }
+    switch (id) {
+    case 1: *v = (uint32_t)field<int64_t>(r, 8); *type = RJ_ATTR_UINT; return true;
+    case 3: *v = field<uint32_t>(r, 24); *type = RJ_ATTR_INT; return true;
+    case 4: *v = field<uint32_t>(r, 28); *type = RJ_ATTR_INT; return true;
+    default: return false;
+    }
 }
 
 }  // namespace
@@ -260,3 +303,76 @@ std::string GLHandler::printTimeStats(FunctionType fn)
 }
 
 }  // namespace rjcpp
+
+// ---------------------------------------------------------------------------
+// point records (Dataset::getNextRecord, db/Dataset.cpp:13-36)
+// ---------------------------------------------------------------------------
+
+extern "C" int rj_read_points(const char* path, int32_t dataset, int32_t loc_attrib, int32_t nattr,
+                              const int32_t* attribs, int64_t max_records, int64_t* n_records, int64_t* n_kept,
+                              float* xy, void* const* cols, int32_t* types)
+{
+    using namespace rjcpp;
+    if (!path || !n_records || (dataset != RJ_DATASET_TAXI && dataset != RJ_DATASET_TWITTER) || nattr < 0 ||
+        nattr > RJ_MAX_COLUMNS || (nattr > 0 && !attribs) || max_records < 0)
+        return RJ_ERR_ARG;
+    if (dataset == RJ_DATASET_TAXI ? (loc_attrib != 1 && loc_attrib != 2) : loc_attrib != 2) return RJ_ERR_ARG;
+    const size_t rs = dataset == RJ_DATASET_TAXI ? kTaxiRecord : kTweetRecord;
+    unsigned char probe[kTaxiRecord] = {0};
+    for (int k = 0; k < nattr; k++) {
+        uint32_t v;
+        int32_t t;
+        if (!attribute(dataset, attribs[k], probe, &v, &t)) return RJ_ERR_UNSUPPORTED;  // not a 4-byte attribute
+    }
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return RJ_ERR_ARG;
+    std::fseek(f, 0, SEEK_END);
+    const long bytes = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    int64_t total = bytes > 0 ? (int64_t)bytes / (int64_t)rs : 0;
+    if (max_records > 0) total = std::min(total, max_records);
+    *n_records = total;
+    if (!xy) {
+        std::fclose(f);
+        return RJ_OK;
+    }
+    if (!n_kept || (nattr > 0 && (!cols || !types))) {
+        std::fclose(f);
+        return RJ_ERR_ARG;
+    }
+    int64_t kept = 0;
+    std::vector<unsigned char> buf(rs * 65536);
+    for (int64_t done = 0; done < total;) {
+        const int64_t n = std::min<int64_t>(65536, total - done);
+        if (std::fread(buf.data(), rs, (size_t)n, f) != (size_t)n) {
+            std::fclose(f);
+            return RJ_ERR_ARG;
+        }
+        for (int64_t i = 0; i < n; i++) {
+            const unsigned char* r = buf.data() + (size_t)i * rs;
+            // index location: (lat, lon) as floats (TaxiRecord.hpp:277-285, TwitterRecord.hpp:103-110);
+            // a kept record's location becomes the projected one (TaxiRecord.hpp:299-302)
+            const float lat = dataset == RJ_DATASET_TAXI ? (float)field<double>(r, 16) : field<float>(r, 16);
+            const float lon = dataset == RJ_DATASET_TAXI ? (float)field<double>(r, 8) : field<float>(r, 20);
+            float tx, ty;
+            if (!transform_point(lat, lon, &tx, &ty)) continue;  // outside the US bbox: skipped
+            if (dataset == RJ_DATASET_TAXI && loc_attrib == 2) {
+                // the dropoff location is not projected (TaxiRecord.hpp:134-145)
+                tx = (float)field<double>(r, 32);
+                ty = (float)field<double>(r, 24);
+            }
+            xy[2 * kept] = tx;
+            xy[2 * kept + 1] = ty;
+            for (int k = 0; k < nattr; k++) {
+                uint32_t v;
+                attribute(dataset, attribs[k], r, &v, &types[k]);
+                static_cast<uint32_t*>(cols[k])[kept] = v;
+            }
+            kept++;
+        }
+        done += n;
+    }
+    std::fclose(f);
+    *n_kept = kept;
+    return RJ_OK;
+}

@@ -120,6 +120,7 @@ def lib():
             "rj_execute_query": [vp, vp, vp, vp], "rj_execute_batch": [vp, vp, i32, vp, vp],
             "rj_execute_query_noalloc": [vp, vp, vp, vp],
             "rj_polyset_grid": [vp, vp, i32, vp, vp, vp, vp, vp],
+            "rj_read_points": [C.c_char_p, i32, i32, i32, vp, i64, vp, vp, vp, vp, vp],
             "rj_autotune": [vp, vp, i32, vp, vp],
             "rj_server_create": [vp, vp, vp, vp], "rj_server_submit": [vp, vp, vp, vp],
             "rj_server_poll": [vp, C.c_uint64, i32], "rj_server_stats_get": [vp, vp],
@@ -204,6 +205,43 @@ def poly_set(name):
     """The (xy, off) of a named collection of POLY_SETS."""
     cp = POLY_SETS[name]
     return gen_voronoi(cp["nsites"], seed=cp["seed"], spacing=cp["spacing"], jitter=cp["jitter"])
+
+
+TAXI_RECORD = np.dtype([("db_idx", "<u8"), ("pick_x", "<f8"), ("pick_y", "<f8"), ("drop_x", "<f8"),
+                        ("drop_y", "<f8"), ("pickup_time", "<u8"), ("dropoff_time", "<u8"), ("vendor", "S4"),
+                        ("duration", "<u4"), ("miles", "<f4"), ("fare", "<u2"), ("surcharge", "<u2"),
+                        ("mta_tax", "<u2"), ("tip", "<u2"), ("toll", "<u2"), ("total", "<u2"),
+                        ("medallion_id", "<u2"), ("license_id", "<u2"), ("store_and_forward", "u1"),
+                        ("payment_type", "u1"), ("passengers", "u1"), ("rate_code", "u1")])
+TWEET_RECORD = np.dtype([("user_id", "<i8"), ("time", "<i8"), ("loc_x", "<f4"), ("loc_y", "<f4"),
+                         ("favCt", "<i4"), ("rtCt", "<i4")])
+DATASET_TAXI, DATASET_TWITTER = 0, 1
+
+
+def read_points(path, dataset, loc_attrib, attribs=(), max_records=0):
+    """rj_read_points: the kept rows' locations (2 floats per row) and the
+    attribute columns (uint32/int32/float32 arrays) of a file of the
+    reference's point records; returns (xy, cols, records read)."""
+    na = len(attribs)
+    attribs = np.ascontiguousarray(list(attribs) or [0], np.int32)
+    nrec, kept = C.c_int64(), C.c_int64()
+    p = os.fsencode(path)
+    rc = lib().rj_read_points(p, dataset, loc_attrib, na, _p(attribs), max_records, C.byref(nrec), None, None,
+                              None, None)
+    if rc:
+        raise RJError(rc, "rj_read_points")
+    xy = np.zeros(2 * max(1, nrec.value), np.float32)
+    raw = [np.zeros(max(1, nrec.value), np.uint32) for _ in range(na)]
+    colptr = (C.c_void_p * max(1, na))(*[_p(c) for c in raw])
+    types = np.zeros(max(1, na), np.int32)
+    rc = lib().rj_read_points(p, dataset, loc_attrib, na, _p(attribs), max_records, C.byref(nrec), C.byref(kept),
+                              _p(xy), C.cast(colptr, C.c_void_p), _p(types))
+    if rc:
+        raise RJError(rc, "rj_read_points")
+    k = kept.value
+    view = {UINT: np.uint32, INT: np.int32, FLOAT: np.float32}
+    cols = [c[:k].view(view[int(t)]) for c, t in zip(raw, types[:na])]
+    return xy[:2 * k], cols, nrec.value
 
 
 def make_constraints(cons):

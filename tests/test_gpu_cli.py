@@ -197,3 +197,70 @@ def test_cli_web_mercator_polygons(tmp_path):
         assert not near.any()
         exp.append(int(((x > x0) & (x < x1) & (y > y0) & (y < y1)).sum()))
     assert [int(r[1]) for r in rows] == exp and min(exp) > 0
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="driver not built")
+def test_cli_taxi_record_file(tmp_path):
+    """--pointFile with taxi records (lat/lon pickups, as the reference's
+    datasets hold them): records outside the US bounding box are skipped, the
+    pickups projected like the polygons; an index join with a pickup-time
+    window, the passengers == 1 filter (--nAttrib 1, main.cpp:85-91) and
+    avg(fare) gives the counts and means of the records inside each lat/lon
+    rectangle directly."""
+    d = str(tmp_path)
+    clat, clon = (24.396308 + 49.384358) / 2, (-124.848974 + -66.885444) / 2
+    rects = [(0.002, 0.006, 0.002, 0.007), (0.008, 0.0125, 0.004, 0.009), (0.001, 0.004, 0.009, 0.015)]
+    lines = [str(len(rects))]
+    for a, b, c, e in rects:
+        la0, la1, lo0, lo1 = clat - b, clat - a, clon + c, clon + e
+        lines += ["1", "4", f"{la0!r} {lo0!r}", f"{la0!r} {lo1!r}", f"{la1!r} {lo1!r}", f"{la1!r} {lo0!r}"]
+    with open(os.path.join(d, "polys.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    with open(os.path.join(d, "list.csv"), "w") as f:
+        f.write("test,polys.txt\n")
+    rng = np.random.default_rng(9)
+    n = N  # run_cli reads at most --inputSize N records
+    lat = clat - rng.uniform(0.0, 0.014, n)
+    lon = clon + rng.uniform(0.0, 0.017, n)
+    # no record within 1e-5 degrees of an edge line: rounding cannot move one across
+    m = 1e-5
+    la, lo = lat.astype(np.float32).astype(np.float64), lon.astype(np.float32).astype(np.float64)
+    near = np.zeros(n, bool)
+    for a, b, c, e in rects:
+        for v in (clat - b, clat - a):
+            near |= abs(la - v) < m
+        for v in (clon + c, clon + e):
+            near |= abs(lo - v) < m
+    lat, lon, la, lo = lat[~near], lon[~near], la[~near], lo[~near]
+    n = len(lat)
+    r = np.zeros(n, rj.TAXI_RECORD)
+    lat[:1000], lon[:1000] = 10.0, -100.0   # outside the US bbox: skipped
+    la[:1000], lo[:1000] = 10.0, -100.0
+    r["pick_y"], r["pick_x"] = lat, lon
+    t0 = 1230768000
+    r["pickup_time"] = rng.integers(t0, t0 + 31536000, n)
+    r["fare"] = rng.integers(250, 9000, n)
+    r["passengers"] = rng.integers(1, 7, n)
+    path = os.path.join(d, "taxi.bin")
+    r.tofile(path)
+    lo_t, hi_t = t0 + 1_000_000, t0 + 20_000_000
+    t = r["pickup_time"]
+    sel = (t >= lo_t) & (t <= hi_t) & (r["passengers"] == 1)
+    exp_c, exp_avg = [], []
+    for a, b, c, e in rects:
+        la0, la1, lo0, lo1 = clat - b, clat - a, clon + c, clon + e
+        k = (la > la0) & (la < la1) & (lo > lo0) & (lo < lo1) & sel
+        exp_c.append(int(k.sum()))
+        exp_avg.append(r["fare"][k].astype(np.float64).sum() / max(1, k.sum()))
+    assert min(exp_c) > 100
+    flags = ["--joinType", "index", "--indexRes", "128", "--polyTransform", "--pointFile", path,
+             "--pointDataset", "taxi", "--locAttrib", "1", "--startTime", str(lo_t), "--endTime", str(hi_t),
+             "--nAttrib", "1"]
+    out, _ = run_cli(d, *flags)
+    rows = read_csv(os.path.join(out, f"raster_{hi_t}_0.csv"))
+    assert [int(x[1]) for x in rows] == exp_c
+    out, _ = run_cli(d, *flags, "--avg", "8")
+    rows = read_csv(os.path.join(out, f"raster_{hi_t}_0.csv"))
+    assert np.allclose([float(x[1]) for x in rows], exp_avg, rtol=1e-9, atol=0)
+    xy, _, nrec = rj.read_points(path, rj.DATASET_TAXI, 1)
+    assert nrec == n and len(xy) == 2 * (n - 1000)
