@@ -251,8 +251,10 @@ def headline(h, pts, cols, xy, off, args):
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
     h.set_option(rj.OPT_CACHE, 1)
     h.set_option(rj.OPT_TIMING, 0)
+    canvas = canvas_section(h, n, W, H, args)
     table.close()
     return {
+        "canvas_path": canvas,
         "value": n * args.steps / dt / 1e9, "ms_per_step": dt / args.steps * 1e3,
         "config": {"workload": "raster join eps=10 m, count()+avg(fare), 100M points, 260 Voronoi "
                                "polygons, resident organized table",
@@ -266,6 +268,36 @@ def headline(h, pts, cols, xy, off, args):
         "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in ktot.items()},
         "step_ms": per,
     }
+
+
+def canvas_section(h, n, W, H, args):
+    """The canvas path on the headline table (RasterJoinBounds and the
+    splat + polygon pass of RasterJoin.cpp:91-194): errorbounds count() and
+    the raster avg(fare) forced onto the canvas, eps = 10 m; polygon
+    structures cached. The splat's algorithmic bytes per launch: x,y of every
+    point (8 B; with sums the fare too, 4 B), plus each canvas pixel written
+    once (4 B count; with sums also 8 + 8 B)."""
+    out = {}
+    steps = max(6, args.steps // 2)
+    for name, fn, agg, path in (("errorbounds_count", rj.ERRORBOUNDS, rj.COUNT, 0),
+                                ("raster_avg_canvas", rj.RASTER, rj.AVG, 1)):
+        h.set_aggregation(agg, 1 if agg == rj.AVG else -1)
+        h.set_option(rj.OPT_RASTER_PATH, path)
+        h.set_option(rj.OPT_TIMING, 1)
+        dt, ktot, per = timed_steps(h, fn, steps, 2)
+        k_ms = ktot["splat"][0] / ktot["splat"][1]
+        alg = n * (8 if agg == rj.COUNT else 12) + W * H * (4 if agg == rj.COUNT else 20)
+        ach = alg / (k_ms * 1e-3) / 1e9
+        out[name] = {"ms_per_join": round(dt / steps * 1e3, 3), **per,
+                     "G_points_per_s": round(n * steps / dt / 1e9, 2),
+                     "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in ktot.items()},
+                     "roofline": {"kernel": "splat", "bound": "hbm", "achieved": round(ach, 1),
+                                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                                  "kernel_ms": round(k_ms, 4), "alg_bytes": int(alg)}}
+        log(f"canvas path {name}: {out[name]}")
+    h.set_option(rj.OPT_RASTER_PATH, 0)
+    h.set_option(rj.OPT_TIMING, 0)
+    return {"workload": f"eps={args.eps} m, {n} points, 260 Voronoi polygons, canvas {W}x{H}", **out}
 
 
 def profiled_traffic():

@@ -272,7 +272,83 @@ struct Acc {
     DevResult g;
     int lds_np;               // direct mode: polygons in LDS (0: none)
     int hash;                 // 1: hashed mode
+    // canvas splat (PT_SPLAT): per-pixel sums of the workgroup in LDS slots
+    uint32_t* stag;           // pixel of each slot
+    uint32_t* scnt;
+    unsigned long long* shi;
+    unsigned long long* slo;
+    int sslots;
 };
+
+// Canvas splat in LDS: kSplatSlots tagged slots keyed by pixel (half as many
+// with sums), linear probing over 8 slots; a pixel that finds none is added
+// to the canvas directly. A workgroup's rows are spatially close (a spatially
+// ordered table), so its points repeat pixels and each pixel reaches the
+// canvas once per workgroup instead of once per run of a lane.
+constexpr int kSplatSlots = 2048;
+
+__device__ __forceinline__ void splat_init(Acc& A, unsigned long long* sacc, bool avg)
+{
+    const int slots = avg ? kSplatSlots / 2 : kSplatSlots;
+    A.sslots = slots;
+    A.stag = reinterpret_cast<uint32_t*>(sacc);
+    A.scnt = A.stag + slots;
+    A.shi = reinterpret_cast<unsigned long long*>(A.scnt + slots);
+    A.slo = A.shi + slots;
+    for (int k = threadIdx.x; k < slots; k += blockDim.x) {
+        A.stag[k] = kHashEmpty;
+        A.scnt[k] = 0;
+        if (avg) {
+            A.shi[k] = 0;
+            A.slo[k] = 0;
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void splat_add(const Acc& A, uint32_t* cnt, unsigned long long* chi,
+                                          unsigned long long* clo, int key, uint32_t c, unsigned long long hi,
+                                          unsigned long long lo, bool avg)
+{
+    const uint32_t h = (uint32_t)key * 2654435761u;
+    for (int p = 0; p < 8; p++) {
+        const int sl = (int)((h + (uint32_t)p) & (uint32_t)(A.sslots - 1));
+        uint32_t t = A.stag[sl];
+        if (t == kHashEmpty) {
+            const uint32_t old = atomicCAS(&A.stag[sl], kHashEmpty, (uint32_t)key);
+            t = old == kHashEmpty ? (uint32_t)key : old;
+        }
+        if (t == (uint32_t)key) {
+            atomicAdd(&A.scnt[sl], c);
+            if (avg) {
+                atomicAdd(&A.shi[sl], hi);
+                atomicAdd(&A.slo[sl], lo);
+            }
+            return;
+        }
+    }
+    atomicAdd(&cnt[key], c);
+    if (avg) {
+        atomicAdd(&chi[key], hi);
+        atomicAdd(&clo[key], lo);
+    }
+}
+
+// The workgroup's LDS pixels to the canvas (after a barrier inside).
+__device__ __forceinline__ void splat_flush(const Acc& A, uint32_t* cnt, unsigned long long* chi,
+                                            unsigned long long* clo, bool avg)
+{
+    __syncthreads();
+    for (int k = threadIdx.x; k < A.sslots; k += blockDim.x) {
+        const uint32_t key = A.stag[k];
+        if (key == kHashEmpty) continue;
+        atomicAdd(&cnt[key], A.scnt[k]);
+        if (avg) {
+            if (A.shi[k]) atomicAdd(&chi[key], A.shi[k]);
+            if (A.slo[k]) atomicAdd(&clo[key], A.slo[k]);
+        }
+    }
+}
 
 __device__ __forceinline__ int hash_slot(int key)
 {
@@ -389,11 +465,7 @@ __device__ __forceinline__ void run_flush(const PointArgs& a, const Acc& A, int 
                                           unsigned long long ch, unsigned long long cl)
 {
     if (MODE == PT_SPLAT) {
-        atomicAdd(&a.cnt[ck], cc);
-        if (AVG) {
-            atomicAdd(&a.chi[ck], ch);
-            atomicAdd(&a.clo[ck], cl);
-        }
+        splat_add(A, a.cnt, a.chi, a.clo, ck, cc, ch, cl, AVG);
     } else {
         acc_add<LDS>(A, ck, cc, ch, cl, AVG);
     }

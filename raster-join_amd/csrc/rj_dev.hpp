@@ -126,6 +126,7 @@ enum PointMode { PT_INDEX = 0, PT_LABEL = 1, PT_LABEL_HYBRID = 2, PT_SPLAT = 3, 
 // the per-workgroup flush stays small), other tables the minimum (each
 // workgroup flushes about every polygon).
 constexpr int kPassWorkgroupsMin = 4096, kPassWorkgroupsMax = 49152, kPassRowsPerWorkgroup = 12288;
+constexpr int kSplatRowsPerWorkgroup = 4096;  // canvas splats (PT_SPLAT)
 
 // Arguments of one point pass.
 struct PointArgs {
