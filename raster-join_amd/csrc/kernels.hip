@@ -16,6 +16,7 @@
 #include <algorithm>
 
 #include "point_dev.hpp"
+#include "scan_ops.hpp"
 
 namespace rj {
 namespace {
@@ -54,42 +55,18 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v)
 // ---------------------------------------------------------------------------
 
 // Edge-flag toggles of the centers of cells of size cs (subunits) for rows
-// [R0, R0+NR) and columns from C0 (BW columns, WPR words per row).
+// [R0, R0+NR) and columns from C0 (BW columns, WPR words per row): each
+// crossing toggles the first column at or right of it.
 __device__ __forceinline__ void edge_toggles(int64_t ax, int64_t ay, int64_t bx, int64_t by,
                                              int64_t cs, int R0, int NR, int C0, int BW, int WPR,
                                              uint32_t* buf)
 {
-    if (ay == by) return;
-    if (ay > by) {
-        int64_t t = ax; ax = bx; bx = t;
-        t = ay; ay = by; by = t;
-    }
-    const int64_t h = cs / 2;
-    int64_t jlo = ceil_div(ay - h, cs), jhi = ceil_div(by - h, cs) - 1;
-    if (jlo < R0) jlo = R0;
-    if (jhi > R0 + NR - 1) jhi = R0 + NR - 1;
-    if (jlo > jhi) return;
-    const int64_t dy = by - ay, dx = bx - ax;
-    // k(row) = ceil((ax*dy + dx*(row*cs + h - ay) - h*dy) / (cs*dy)): the
-    // numerator grows by dx*cs per row, so k follows from one division for
-    // the first row and a quotient/remainder step per row (exact integers)
-    const int64_t den = cs * dy;
-    const int64_t num0 = ax * dy + dx * (jlo * cs + h - ay) - h * dy;
-    int64_t q = floor_div(num0, den), r = num0 - q * den;  // num = q*den + r, 0 <= r < den
-    const int64_t sq = floor_div(dx * cs, den), sr = dx * cs - sq * den;
-    for (int64_t row = jlo; row <= jhi; row++) {
-        const int64_t k = q + (r != 0 ? 1 : 0);
-        q += sq;
-        r += sr;
-        if (r >= den) {
-            r -= den;
-            q++;
-        }
+    edge_crossings(ax, ay, bx, by, cs, R0, NR, [&](int64_t row, int64_t k) {
         int64_t local = k - C0;
         if (local < 0) local = 0;
-        if (local >= BW) continue;
+        if (local >= BW) return;
         atomicXor(&buf[(row - R0) * WPR + (local >> 5)], 1u << (local & 31));
-    }
+    });
 }
 
 // Pixels whose closed square meets segment ab, rows [TR0, TR0+TNR) of the
@@ -153,24 +130,6 @@ __device__ __forceinline__ void prefix_xor_row(uint32_t* w, int nw)
 __device__ __forceinline__ int bit_at(const uint32_t* row, int col)
 {
     return (row[col >> 5] >> (col & 31)) & 1;
-}
-
-// Bits col..col+7 of a row of wpr words (bit k of the result: column col+k),
-// 0 for columns outside [0, bw); col in [-7, bw).
-__device__ __forceinline__ uint32_t bits8(const uint32_t* row, int col, int bw, int wpr)
-{
-    uint32_t m;
-    if (col >= 0) {
-        const int w = col >> 5, sh = col & 31;
-        uint64_t v = row[w];
-        if (sh > 24 && w + 1 < wpr) v |= (uint64_t)row[w + 1] << 32;
-        m = (uint32_t)(v >> sh) & 0xFFu;
-    } else {
-        m = (row[0] << (-col)) & 0xFFu;
-    }
-    const int valid = bw - col;
-    if (valid < 8) m &= (1u << valid) - 1u;
-    return m;
 }
 
 template <int MODE>
