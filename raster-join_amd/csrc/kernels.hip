@@ -408,9 +408,16 @@ __global__ void k_tile_labels(const void* __restrict__ label, int label16, int L
 // kRows (fewer iterations of dependent loads per workgroup).
 constexpr int64_t kQueuePassRows4 = 1 << 24;
 
+// Rows per thread of the two-stage passes (rows_compact).
+#ifndef RJ_COMPACT_ROWS
+#define RJ_COMPACT_ROWS 8
+#endif
+constexpr int kCompactRows = RJ_COMPACT_ROWS;
+
 // One pass over rows [begin, end): each workgroup takes a contiguous block of
 // rows (point_rows) and sums per polygon in LDS (acc_init / acc_flush).
-template <int MODE, bool AVG, int R>
+// CMP: the two-stage rows of a query with constraints (rows_compact).
+template <int MODE, bool AVG, int R, bool CMP = false>
 __global__ __launch_bounds__(kBlock) void k_points(PointArgs a)
 {
     extern __shared__ unsigned long long sacc[];
@@ -427,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void k_points(PointArgs a)
     const int64_t b0 = a.start + (int64_t)blockIdx.x * a.per_block;
     const int64_t b1 = min(a.end, b0 + a.per_block);
     Run run{-1, 0, 0, 0};
-    point_rows<MODE, AVG, R>(a, b0, max(a.begin, b0), b1, A, run);
+    point_rows<MODE, AVG, R, true, CMP>(a, b0, max(a.begin, b0), b1, A, run);
     if (run.ck >= 0) run_flush<MODE, AVG>(a, A, run.ck, run.cc, run.ch, run.cl);
     if (MODE == PT_SPLAT) splat_flush(A, a.cnt, a.chi, a.clo, AVG);
     else acc_flush(A, AVG);
@@ -643,6 +650,21 @@ void launch_points(hipStream_t s, int mode, PointArgs a)
                                           : acc_lds_bytes(a.P.np, a.lds_np, avg);
 #define RJ_LAUNCH_PTS(M, A) \
     hipLaunchKernelGGL((k_points<M, A, kRows>), dim3(blocks), dim3(kBlock), lds, s, a)
+    // a label or hybrid-queue pass with constraints: two-stage rows
+    if (!a.q.plain && (mode == PT_LABEL || mode == PT_HYBRID_QUEUE)) {
+        const int64_t per_c = (n + cap - 1) / cap;
+        a.per_block = (per_c + kBlock * kCompactRows - 1) / (kBlock * kCompactRows) * (kBlock * kCompactRows);
+        const int blocks_c = (int)((n + a.per_block - 1) / a.per_block);
+#define RJ_LAUNCH_CMP(M, A) \
+    hipLaunchKernelGGL((k_points<M, A, kCompactRows, true>), dim3(blocks_c), dim3(kBlock), lds, s, a)
+        if (mode == PT_LABEL) {
+            if (avg) RJ_LAUNCH_CMP(PT_LABEL, true); else RJ_LAUNCH_CMP(PT_LABEL, false);
+        } else {
+            if (avg) RJ_LAUNCH_CMP(PT_HYBRID_QUEUE, true); else RJ_LAUNCH_CMP(PT_HYBRID_QUEUE, false);
+        }
+#undef RJ_LAUNCH_CMP
+        return;
+    }
     switch (mode) {
     case PT_INDEX: if (avg) RJ_LAUNCH_PTS(PT_INDEX, true); else RJ_LAUNCH_PTS(PT_INDEX, false); break;
     case PT_LABEL: if (avg) RJ_LAUNCH_PTS(PT_LABEL, true); else RJ_LAUNCH_PTS(PT_LABEL, false); break;

@@ -8,6 +8,8 @@
 // the grid cell and the pixel formula must round exactly like the host
 // restatement of CPUJoin.
 #pragma once
+#include <type_traits>
+
 #include "rj_dev.hpp"
 
 namespace rj {
@@ -629,6 +631,228 @@ __device__ __forceinline__ void row_labels(const PointArgs& a, const float (&px)
     }
 }
 
+// The slots of one query decoded once per pass into wave-uniform values:
+// the column of each slot that is read (constrained, or the aggregate's),
+// and for a constrained slot its key range as lo and span = hi - lo, with
+// col_key computed as k = v ^ (xr | (sign(v) & fl)) (Float: xr = 2^31,
+// fl = ~0; Int: xr = 2^31; Uint and u8: neither). A row passes a slot when
+// k - lo <= span (unsigned); no row passes a slot in none_mask (lo > hi).
+struct SlotPlan {
+    const void* col[kMaxSlots];
+    int32_t type[kMaxSlots];
+    uint32_t xr[kMaxSlots], fl[kMaxSlots], lo[kMaxSlots], span[kMaxSlots];
+    uint32_t read_mask;  // bit s: slot s is read
+    uint32_t filt_mask;  // bit s: slot s has constraints
+    uint32_t none_mask;  // bit s: no value passes slot s
+    int agg;             // slot of the aggregate column (-1: count)
+};
+
+__device__ __forceinline__ SlotPlan make_plan(const DevQuery& q, const DevCols& C, bool avg)
+{
+    SlotPlan P;
+    P.read_mask = P.filt_mask = P.none_mask = 0;
+    P.agg = avg ? q.agg_slot : -1;
+#pragma unroll
+    for (int s = 0; s < kMaxSlots; s++) {
+        const bool in = s < q.nslots;
+        const int col = in ? q.slot_col[s] : 0;
+        const int t = C.type[col];
+        const uint32_t lo = q.slot_lo[s], hi = q.slot_hi[s];
+        const bool filt = in && !(lo == 0u && hi == 0xFFFFFFFFu);
+        P.col[s] = C.col[col];
+        P.type[s] = t;
+        P.xr[s] = (t == RJ_ATTR_FLOAT || t == RJ_ATTR_INT) ? 0x80000000u : 0u;
+        P.fl[s] = t == RJ_ATTR_FLOAT ? 0xFFFFFFFFu : 0u;
+        P.lo[s] = lo;
+        P.span[s] = hi - lo;
+        if (filt) P.filt_mask |= 1u << s;
+        if (filt && lo > hi) P.none_mask |= 1u << s;
+        if (filt || (in && s == P.agg)) P.read_mask |= 1u << s;
+    }
+    return P;
+}
+
+// The constraints of slot s on R rows with values v (the test of
+// slot_filter): ok[r] stays true when the row passes.
+template <int R>
+__device__ __forceinline__ void plan_filter(const SlotPlan& P, int s, const uint32_t (&v)[R], bool (&ok)[R])
+{
+    if ((P.none_mask >> s) & 1u) {
+#pragma unroll
+        for (int r = 0; r < R; r++) ok[r] = false;
+        return;
+    }
+    if (P.xr[s] | P.fl[s]) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const uint32_t k = v[r] ^ (P.xr[s] | ((uint32_t)((int32_t)v[r] >> 31) & P.fl[s]));
+            ok[r] = ok[r] & (k - P.lo[s] <= P.span[s]);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) ok[r] = ok[r] & (v[r] - P.lo[s] <= P.span[s]);
+    }
+}
+
+// One row into its polygon's running (key, count, sum) of the lane: a new
+// key flushes the run first.
+template <int MODE, bool AVG, bool LDS>
+__device__ __forceinline__ void run_add(const PointArgs& a, const Acc& A, Run& run, int k, int agg_t, uint32_t v)
+{
+    if (k != run.ck) {
+        if (run.ck >= 0) run_flush<MODE, AVG, LDS>(a, A, run.ck, run.cc, run.ch, run.cl);
+        run.ck = k;
+        run.cc = 0;
+        run.ch = run.cl = 0;
+    }
+    run.cc++;
+    if (AVG) {
+        unsigned long long h, l;
+        agg_bits(agg_t, v, h, l);
+        run.ch += h;
+        run.cl += l;
+    }
+}
+
+// Rows [lo, hi) of a label or hybrid pass whose query has constraints, in
+// two stages. Each lane reads the columns of its R rows and tests the
+// constraints; the rows that pass go to the wave's queue in LDS (x, y and
+// the aggregate's value). Whenever the queue holds K waves' worth, every
+// lane takes K of them (rows lane, lane + 64, ...) through the pixel, the
+// label maps (and, in a hybrid pass, the halo test and the border queue)
+// into its polygon's run, the K rows' gathers in flight together. The label
+// work then scales with the rows that pass, not the rows read (500M rows,
+// 10% passing, hybrid: 2.50 -> 1.67 ms; 33% passing, label: 1.54 -> 1.25 ms).
+// Only the wave touches its queue (no workgroup barrier); rows enter it two
+// per lane at a time, so it never holds more than 64 K - 1 + 128 rows.
+//
+// K: the hybrid pass's longer chain of gathers (tile, pixel, halo) gains
+// from three; the label pass keeps two and the smaller queue (with three:
+// 1.75 -> 1.67 ms hybrid, 1.25 -> 1.40 ms label).
+template <int MODE>
+constexpr int drain_rows() { return MODE == PT_HYBRID_QUEUE ? 3 : 2; }
+// entries per wave: a power of two above 64 * drain rows + 127
+constexpr int row_queue(int k) { return k <= 2 ? 256 : 512; }
+
+template <int MODE, bool AVG, int R, bool LDS>
+__device__ __forceinline__ void rows_compact(const PointArgs& a, int64_t start, int64_t lo, int64_t hi,
+                                             const Acc& A, Run& run)
+{
+    static_assert(MODE == PT_LABEL || MODE == PT_HYBRID_QUEUE, "label and hybrid-queue passes");
+    static_assert(R % 2 == 0, "rows enter the queue two at a time");
+    constexpr int kDrainRows = drain_rows<MODE>();
+    constexpr int kRowQueue = row_queue(kDrainRows);
+    __shared__ float s_x[kBlock / kWave][kRowQueue];
+    __shared__ float s_y[kBlock / kWave][kRowQueue];
+    __shared__ uint32_t s_v[AVG ? kBlock / kWave : 1][AVG ? kRowQueue : 1];
+    constexpr int kMask = kRowQueue - 1;
+    const int w = threadIdx.x / kWave;
+    const int lane = (int)__lane_id();
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const DevQuery& q = a.q;
+    const int agg_t = AVG ? a.cols.type[q.slot_col[q.agg_slot < 0 ? 0 : q.agg_slot]] : RJ_ATTR_UINT;
+    const SlotPlan SP = make_plan(q, a.cols, AVG);
+    const int64_t stride = (int64_t)blockDim.x * R;
+    int head = 0, cnt = 0;  // the wave's queue: entries head .. head + cnt - 1 (mod kRowQueue)
+    auto wave_sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    // the first n (<= K * 64) rows of the queue, K per lane (rows lane,
+    // lane + 64, ...): their label-map gathers are in flight together
+    auto drain = [&](auto kk, int n) {
+        constexpr int K = decltype(kk)::value;
+        wave_sync();
+        float px[K], py[K];
+        uint32_t v[K];
+        int pix[K], ixs[K], iys[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const bool in = lane + k * kWave < n;
+            const int e = (head + lane + k * kWave) & kMask;
+            px[k] = in ? s_x[w][e] : 0.0f;
+            py[k] = in ? s_y[w][e] : 0.0f;
+            v[k] = (AVG && in) ? s_v[w][e] : 0u;
+            ixs[k] = iys[k] = 0;
+            pix[k] = in ? pixel_xy(a.cv, px[k], py[k], ixs[k], iys[k]) : -1;
+        }
+        head = (head + n) & kMask;
+        cnt -= n;
+        uint32_t lab[K];
+        row_labels<MODE == PT_HYBRID_QUEUE, K>(a, px, py, pix, ixs, iys, lab, [] {});
+        if (MODE == PT_HYBRID_QUEUE) {
+            bool need[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) need[k] = lab[k] == kLabelBorder;
+            queue_border<K>(a, px, py, v, need);
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (lab[k] < kLabelMaxId) run_add<MODE, AVG, LDS>(a, A, run, (int)lab[k], agg_t, v[k]);
+    };
+    using one = std::integral_constant<int, 1>;
+    using many = std::integral_constant<int, kDrainRows>;
+    float nx[R], ny[R];
+    load_xy<R>(a.xy, start + (int64_t)threadIdx.x * R, hi, a.end, nx, ny);
+    for (int64_t base = start; base < hi; base += stride) {
+        const int64_t i0 = base + (int64_t)threadIdx.x * R;
+        float px[R], py[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            px[r] = nx[r];
+            py[r] = ny[r];
+        }
+        load_xy<R>(a.xy, i0 + stride, hi, a.end, nx, ny);
+        // interior: the workgroup's whole row group lies in [lo, hi) (and so
+        // in the table): no per-row bounds
+        const bool interior = base >= lo && base + stride <= hi;
+        bool ok[R];
+        if (interior) {
+#pragma unroll
+            for (int r = 0; r < R; r++) ok[r] = true;
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) ok[r] = (i0 + r >= lo) && (i0 + r < hi);
+        }
+        uint32_t vraw[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) vraw[r] = 0;
+        if (interior || i0 < hi) {
+            const bool full = interior || (i0 + R <= a.end);
+#pragma unroll
+            for (int s = 0; s < kMaxSlots; s++) {
+                if (!((SP.read_mask >> s) & 1u)) continue;
+                uint32_t v[R];
+                if (interior) load_col<R>(SP.col[s], SP.type[s], i0, true, a.end, v);
+                else load_col<R>(SP.col[s], SP.type[s], i0, full, a.end, v);
+                if ((SP.filt_mask >> s) & 1u) plan_filter<R>(SP, s, v, ok);
+                if (AVG && s == SP.agg) {
+#pragma unroll
+                    for (int r = 0; r < R; r++) vraw[r] = v[r];
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r += 2) {
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const unsigned long long m = __ballot(ok[r + u]);
+                if (ok[r + u]) {
+                    const int e = (head + cnt + __popcll(m & below)) & kMask;
+                    s_x[w][e] = px[r + u];
+                    s_y[w][e] = py[r + u];
+                    if (AVG) s_v[w][e] = vraw[r + u];
+                }
+                cnt += __popcll(m);
+            }
+            if (cnt >= kDrainRows * kWave) drain(many{}, kDrainRows * kWave);
+        }
+    }
+    if (cnt > kWave) drain(many{}, cnt);
+    else if (cnt > 0) drain(one{}, cnt);
+}
+
 // Rows [lo, hi) of one point pass, read in groups of R consecutive rows per
 // thread from `start` (a multiple of R, <= lo). Every row gets one key
 // (polygon id, or pixel for PT_SPLAT) and the lane keeps a running
@@ -638,10 +862,16 @@ __device__ __forceinline__ void row_labels(const PointArgs& a, const float (&px)
 // only x,y and then overlaps the column loads of its group. a.end is the last
 // row the loads may touch. LDS = false: every sum goes to the result arrays
 // (A.lds_np is ignored).
-template <int MODE, bool AVG, int R, bool LDS = true>
+// CMP: the rows go through rows_compact (label and hybrid-queue passes;
+// any query, though only one with constraints gains).
+template <int MODE, bool AVG, int R, bool LDS = true, bool CMP = false>
 __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, int64_t lo, int64_t hi, const Acc& A,
                                            Run& run)
 {
+    if constexpr (CMP) {
+        rows_compact<MODE, AVG, R, LDS>(a, start, lo, hi, A, run);
+        return;
+    }
     const DevQuery& q = a.q;
     const int agg_t = AVG ? a.cols.type[q.slot_col[q.agg_slot < 0 ? 0 : q.agg_slot]] : RJ_ATTR_UINT;
     const int64_t stride = (int64_t)blockDim.x * R;
