@@ -36,7 +36,8 @@ __device__ __forceinline__ void query_rows(const PointArgs& a, int64_t r0, int64
     __syncthreads();  // the previous query's sums have been flushed
     acc_init(A, sacc, a.P.np, lds_cap, AVG, a.res);
     Run run{-1, 0, 0, 0};
-    point_rows<MODE, AVG, R>(a, r0 - r0 % R, r0, r1, A, run);
+    // a chunk inside the rows that pass every constraint skips the filters
+    point_rows<MODE, AVG, R>(a, r0 - r0 % R, r0, r1, A, run, r0 >= a.in_b && r1 <= a.in_e);
     if (run.ck >= 0) run_flush<MODE, AVG>(a, A, run.ck, run.cc, run.ch, run.cl);
     acc_flush(A, AVG);
 }
@@ -86,6 +87,8 @@ __device__ __forceinline__ void member_rows(const PointArgs& a, int64_t r0, int6
     const DevQuery& q = a.q;
     const int agg_t = AVG ? a.cols.type[q.slot_col[q.agg_slot < 0 ? 0 : q.agg_slot]] : RJ_ATTR_UINT;
     const int64_t stride = (int64_t)blockDim.x * R;
+    // a chunk inside the rows that pass every constraint skips the filters
+    const bool plain = q.plain || (r0 >= a.in_b && r1 <= a.in_e);
     for (int64_t base = r0 - r0 % R; base < r1; base += stride) {
         const int64_t i0 = base + (int64_t)threadIdx.x * R;
         if (i0 >= r1) continue;
@@ -96,16 +99,20 @@ __device__ __forceinline__ void member_rows(const PointArgs& a, int64_t r0, int6
         uint32_t vraw[R];
 #pragma unroll
         for (int r = 0; r < R; r++) vraw[r] = 0;
+        if (plain) {
+            if (AVG) load_rows<R>(a.cols, q.slot_col[q.agg_slot], i0, full, a.end, vraw);
+        } else {
 #pragma unroll
-        for (int s = 0; s < kMaxSlots; s++) {
-            if (s >= q.nslots) break;
-            uint32_t v[R];
-            const int col = q.slot_col[s];
-            load_rows<R>(a.cols, col, i0, full, a.end, v);
-            slot_filter<R>(a.cols.type[col], q.slot_lo[s], q.slot_hi[s], v, ok);
-            if (AVG && s == q.agg_slot) {
+            for (int s = 0; s < kMaxSlots; s++) {
+                if (s >= q.nslots) break;
+                uint32_t v[R];
+                const int col = q.slot_col[s];
+                load_rows<R>(a.cols, col, i0, full, a.end, v);
+                slot_filter<R>(a.cols.type[col], q.slot_lo[s], q.slot_hi[s], v, ok);
+                if (AVG && s == q.agg_slot) {
 #pragma unroll
-                for (int r = 0; r < R; r++) vraw[r] = v[r];
+                    for (int r = 0; r < R; r++) vraw[r] = v[r];
+                }
             }
         }
         int key[R];

@@ -796,6 +796,47 @@ void rj_ctx::narrow_rows(const rj_table* t, const QuerySpec& s, int64_t* b, int6
     if (*e < *b) *e = *b;
 }
 
+// Rows where every constraint of the query holds by the table's time
+// buckets alone: the whole buckets inside its time window, when every
+// constraint is on the (Uint) time column. *b == *e when there are none.
+// Bucket k holds the times [first(k), last(k)]: bucket 0 also every time
+// below t0, the last bucket every time from its start on (time_rows).
+void rj_ctx::interior_rows(const rj_table* t, const QuerySpec& s, int64_t* b, int64_t* e) const
+{
+    *b = *e = 0;
+    if (t->time_col < 0 || t->bucket_off.size() < 2 || s.cons.empty() || t->span == 0) return;
+    if (t->types[t->time_col] != RJ_ATTR_UINT) return;
+    uint64_t lo = 0, hi = UINT32_MAX;
+    for (const rj_constraint& c : s.cons) {
+        if (c.attrib_id != t->time_col) return;
+        const uint64_t v = c.val.uval;
+        switch (c.type) {
+        case RJ_EQ: lo = std::max(lo, v); hi = std::min(hi, v); break;
+        case RJ_LT: if (v == 0) return; hi = std::min(hi, v - 1); break;
+        case RJ_LTE: hi = std::min(hi, v); break;
+        case RJ_GT: if (v == UINT32_MAX) return; lo = std::max(lo, v + 1); break;
+        case RJ_GTE: lo = std::max(lo, v); break;
+        default: return;
+        }
+    }
+    if (lo > hi) return;
+    const int64_t nb = (int64_t)t->bucket_off.size() - 1;
+    const uint64_t t0 = t->t0, span = t->span;
+    // the first bucket whose times start at or after lo
+    int64_t k0;
+    if (lo == 0) k0 = 0;
+    else if (lo <= t0) k0 = 1;
+    else k0 = std::max<int64_t>(1, (int64_t)((lo - t0 + span - 1) / span));
+    // the last bucket whose times end at or before hi
+    int64_t k1;
+    if (hi == UINT32_MAX) k1 = nb - 1;
+    else if (hi + 1 < t0 + span) k1 = -1;
+    else k1 = std::min<int64_t>(nb - 2, (int64_t)((hi + 1 - t0) / span) - 1);
+    if (k0 > k1 || k0 >= nb) return;
+    *b = t->bucket_off[k0];
+    *e = t->bucket_off[k1 + 1];
+}
+
 int rj_ctx::ensure_canvas_buffers(const CanvasPlan& plan, bool sums)
 {
     const size_t npx = (size_t)plan.W * plan.H;
@@ -997,6 +1038,7 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
     pa.lds_np = lds_direct_cap(avg);
     pa.cap = opt_cap;
     pa.wg_rows = t->organized ? kPassRowsPerWorkgroup : 0;
+    interior_rows(t, s, &pa.in_b, &pa.in_e);
     Canvas* C = nullptr;
     Grid* G = nullptr;
     const unsigned long long* check_aux = nullptr;  // overlap counters of a fresh label map
@@ -1475,6 +1517,7 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
         qb[j] = s.rb;
         qe[j] = s.re;
         narrow_rows(t, s, &qb[j], &qe[j]);
+        interior_rows(t, s, &a.in_b, &a.in_e);
     }
     // border queues of the hybrid members: one region each, 1/8 of its rows
     // (at least 65536); members beyond kQueueEntriesMax queued points in all test inline.

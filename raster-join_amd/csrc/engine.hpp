@@ -323,6 +323,7 @@ struct rj_ctx {
     // queries
     int build_query(const rj_table* t, const rj::QuerySpec& s, rj::DevQuery* q);
     void narrow_rows(const rj_table* t, const rj::QuerySpec& s, int64_t* b, int64_t* e) const;
+    void interior_rows(const rj_table* t, const rj::QuerySpec& s, int64_t* b, int64_t* e) const;
     rj::DevCols dev_cols(const rj_table* t) const;
     rj::DevPolys dev_polys(const rj_polyset* P, const rj::Canvas* C) const;
     rj::DevGrid dev_grid(const rj::Grid* G) const;

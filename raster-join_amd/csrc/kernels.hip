@@ -628,6 +628,26 @@ void launch_halo_flags(hipStream_t s, const uint32_t* mask, const DevCanvas& c, 
 void launch_points(hipStream_t s, int mode, PointArgs a)
 {
     if (a.end <= a.begin) return;
+    // rows that pass every constraint run as a query without constraints;
+    // the rows before and after them keep the filters (the sums, the canvas
+    // and the border queue accumulate over the launches)
+    if (!a.q.plain && a.in_b < a.in_e) {
+        const int64_t ib = std::max(a.begin, a.in_b), ie = std::min(a.end, a.in_e);
+        a.in_b = a.in_e = 0;
+        if (ib < ie) {
+            PointArgs part = a;
+            part.end = ib;
+            launch_points(s, mode, part);
+            part.begin = ib;
+            part.end = ie;
+            part.q.plain = 1;
+            launch_points(s, mode, part);
+            part = a;
+            part.begin = ie;
+            launch_points(s, mode, part);
+            return;
+        }
+    }
     // Row groups start at a multiple of kRows so that vector loads are aligned.
     a.start = a.begin - a.begin % kRows;
     const int64_t n = a.end - a.start;

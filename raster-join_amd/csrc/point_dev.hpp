@@ -863,10 +863,12 @@ __device__ __forceinline__ void rows_compact(const PointArgs& a, int64_t start, 
 // row the loads may touch. LDS = false: every sum goes to the result arrays
 // (A.lds_np is ignored).
 // CMP: the rows go through rows_compact (label and hybrid-queue passes;
-// any query, though only one with constraints gains).
+// any query, though only one with constraints gains). plain_rows: rows
+// [lo, hi) pass every constraint of the query (they are read as for a query
+// without constraints).
 template <int MODE, bool AVG, int R, bool LDS = true, bool CMP = false>
 __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, int64_t lo, int64_t hi, const Acc& A,
-                                           Run& run)
+                                           Run& run, bool plain_rows = false)
 {
     if constexpr (CMP) {
         rows_compact<MODE, AVG, R, LDS>(a, start, lo, hi, A, run);
@@ -896,7 +898,7 @@ __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, in
         uint32_t vraw[R];
 #pragma unroll
         for (int r = 0; r < R; r++) vraw[r] = 0;
-        if (q.plain) {
+        if (q.plain || plain_rows) {
             // no constraints: at most the aggregate column
             if (AVG) load_rows<R>(a.cols, q.slot_col[q.agg_slot], i0, full, a.end, vraw);
         } else

@@ -164,6 +164,10 @@ struct PointArgs {
     int cap;              // workgroups of the pass (0: from wg_rows)
     int wg_rows;          // cap 0: one workgroup per wg_rows rows, kPassWorkgroupsMin..Max
                           // (wg_rows 0: kPassWorkgroupsMin)
+    // rows [in_b, in_e) pass every constraint of the query (whole time
+    // buckets inside its time window, no other constraint): they run as a
+    // query without constraints
+    int64_t in_b, in_e;
 };
 
 // ---- fused passes over many queries (batch.hip) ----

@@ -6,7 +6,8 @@ infinities, integer extremes, empty and contradictory ranges, no row or
 every row passing) through the hybrid join (exact: CPUJoin's counts and
 fixed-point sums) and the label-map raster join (the oracle's raster
 emulator); row ranges that start and end inside a workgroup's rows, on a
-table whose size is not a multiple of a row group."""
+table whose size is not a multiple of a row group; time windows on a table
+ordered by time buckets, whose whole buckets run without the filter."""
 import numpy as np
 import pytest
 
@@ -104,4 +105,68 @@ def test_row_ranges_inside_row_groups(handler, rows):
         h.set_option(rj.OPT_RASTER_PATH, 0)
     assert np.array_equal(r.counts, oc)
     assert np.array_equal(r.hi, ohi) and np.array_equal(r.lo, olo)
+    table.close()
+
+
+DAY = 86400
+T0B = 1230768000
+
+
+def bucket_windows():
+    d = DAY
+    return [
+        [(0, rj.GTE, T0B + 5 * d), (0, rj.LTE, T0B + 9 * d - 1)],       # whole buckets
+        [(0, rj.GTE, T0B + 5 * d + 1), (0, rj.LTE, T0B + 9 * d)],       # one second off each end
+        [(0, rj.GT, T0B + 5 * d - 1), (0, rj.LT, T0B + 9 * d)],
+        [(0, rj.LTE, T0B + 2 * d - 1)],                                  # from time 0 (bucket 0 holds t < t0)
+        [(0, rj.GTE, T0B - 100)],                                        # to the end (the last bucket)
+        [(0, rj.GTE, 0)],
+        [(0, rj.EQ, T0B + 7 * d)],
+        [(0, rj.LT, T0B)],
+        [(0, rj.GTE, T0B + 30 * d), (0, rj.LTE, 0xFFFFFFFF)],
+        [(0, rj.GTE, T0B + 3 * d), (0, rj.LTE, T0B + 20 * d - 1), (2, rj.LTE, 3)],  # and another column
+        [(0, rj.GT, 0xFFFFFFFF)],
+        [],
+    ]
+
+
+def test_time_bucket_windows(handler):
+    """A table ordered by day buckets (times below the first bucket's start
+    and on bucket edges included): rows in whole buckets inside a window run
+    without the filter; single queries of three joins and one fused batch
+    equal the oracle on every window."""
+    n = 300_001
+    pts, _, fare, pas = points(n, 41)
+    rng = np.random.default_rng(9)
+    t = (T0B + rng.integers(-3 * DAY, 40 * DAY, n)).astype(np.uint32)
+    edge = rng.random(n) < 0.05
+    t[edge] = (T0B + rng.integers(-2, 40, edge.sum()) * DAY - rng.integers(0, 2, edge.sum())).astype(np.uint32)
+    cols = (t, fare, pas)
+    xy, off = polys("voronoi260")
+    h = handler
+    table = rj.Table(h, pts, cols, COLS_TYPES, time_col=0, time0=T0B, bucket_seconds=DAY)
+    ps = rj.PolySet(h, xy, off)
+    g = O.Grid(xy, off, 256)
+    p = O.plan(O.poly_bbox(xy).astype(np.float64), 10.0)
+    h.set_option(rj.OPT_RASTER_PATH, 2)
+    try:
+        qs, want = [], []
+        for k, cons in enumerate(bucket_windows()):
+            agg, col = (rj.AVG, 1) if k % 2 == 0 else (rj.COUNT, -1)
+            ops = O.PointSet(pts, cols, COLS_TYPES, cons, agg_type=agg, agg_col=max(col, 0))
+            exact = O.join_exact(g, ops)[:3]
+            raster = O.raster_join(p, xy, off, ops)
+            for fn, ref in ((rj.RASTER, raster), (rj.HYBRID, exact), (rj.INDEX, exact)):
+                q = rj.make_query(ps, fn, agg, col, cons, eps=10.0, index_res=256)
+                r = h.execute_query(table, q, ps.np)
+                assert np.array_equal(r.counts, ref[0]), (fn, cons)
+                assert np.array_equal(r.hi, ref[1]) and np.array_equal(r.lo, ref[2]), (fn, cons)
+                qs.append(q)
+                want.append(ref)
+        for r, ref in zip(h.execute_batch(table, qs, [ps.np] * len(qs)), want):
+            assert r.status == 0
+            assert np.array_equal(r.counts, ref[0])
+            assert np.array_equal(r.hi, ref[1]) and np.array_equal(r.lo, ref[2])
+    finally:
+        h.set_option(rj.OPT_RASTER_PATH, 0)
     table.close()
