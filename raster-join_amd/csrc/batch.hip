@@ -38,6 +38,7 @@ __device__ __forceinline__ void query_rows(const PointArgs& a, int64_t r0, int64
     Run run{-1, 0, 0, 0};
     // a chunk inside the rows that pass every constraint skips the filters
     point_rows<MODE, AVG, R>(a, r0 - r0 % R, r0, r1, A, run, r0 >= a.in_b && r1 <= a.in_e);
+    if (MODE == PT_HYBRID_QUEUE) border_finish(a, run);
     if (run.ck >= 0) run_flush<MODE, AVG>(a, A, run.ck, run.cc, run.ch, run.cl);
     acc_flush(A, AVG);
 }

@@ -435,6 +435,7 @@ __global__ __launch_bounds__(kBlock) void k_points(PointArgs a)
     const int64_t b1 = min(a.end, b0 + a.per_block);
     Run run{-1, 0, 0, 0};
     point_rows<MODE, AVG, R, true, CMP>(a, b0, max(a.begin, b0), b1, A, run);
+    if (MODE == PT_HYBRID_QUEUE) border_finish(a, run);
     if (run.ck >= 0) run_flush<MODE, AVG>(a, A, run.ck, run.cc, run.ch, run.cl);
     if (MODE == PT_SPLAT) splat_flush(A, a.cnt, a.chi, a.clo, AVG);
     else acc_flush(A, AVG);

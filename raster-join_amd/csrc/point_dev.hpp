@@ -455,11 +455,14 @@ __device__ __forceinline__ int exact_first(const DevGrid& g, const DevPolys& P, 
     return first;
 }
 
-// A lane's running (key, count, sum) of a point pass.
+// A lane's running (key, count, sum) of a point pass, and the wave's staged
+// border rows (queue_border): ring position qh and count qn, the same in
+// every lane that ran the wave's last row group.
 struct Run {
     int ck;
     uint32_t cc;
     unsigned long long ch, cl;
+    int qh, qn;
 };
 
 template <int MODE, bool AVG, bool LDS = true>
@@ -551,35 +554,79 @@ __device__ __forceinline__ void exact_compact(const PointArgs& a, const Acc& A, 
     }
 }
 
-// Append a wave's border rows to the pass's queue: one atomic per wave
-// reserves the space, the lanes write their rows in ballot order. Rows past
-// the capacity are dropped; the count still grows, so the host sees the
-// overflow.
-template <int R>
-__device__ __forceinline__ void queue_border(const PointArgs& a, const float (&px)[R], const float (&py)[R],
-                                             const uint32_t (&vraw)[R], const bool (&need)[R])
+// LDS writes of a wave's lanes visible to its other lanes (and its reads
+// done before later writes).
+__device__ __forceinline__ void wave_sync()
 {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The border queue of a pass is filled a wave's worth at a time: a wave
+// stages its border rows in an LDS ring and reserves queue room with one
+// atomic per 64 rows (a returning atomic on the pass's one counter for every
+// row group of every wave serialized the pass: hybrid counts over 900M and
+// 580M rows of a day-ordered table, 260 and 900 polygons, 12.8 -> 2.8 ms
+// and 13.3 -> 2.5 ms). Rows past the capacity are dropped; the count still
+// grows, so the host sees the overflow.
+constexpr int kBorderStage = 128;  // staged rows per wave (a power of two, >= 128)
+
+__device__ __forceinline__ float4* border_stage()
+{
+    __shared__ float4 s_bq[kBlock / kWave][kBorderStage];
+    return s_bq[threadIdx.x / kWave];
+}
+
+// The first n (<= 64) staged rows into the queue (the wave's active lanes).
+__device__ __forceinline__ void border_flush(const PointArgs& a, Run& run, int n)
+{
+    float4* stage = border_stage();
+    wave_sync();
     const int lane = (int)__lane_id();
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    int cnt = 0;
-    int pos[R];
+    const unsigned long long act = __ballot(1);
+    const int lead = __ffsll((long long)act) - 1;
+    const int rank = __popcll(act & below), nact = __popcll(act);
+    uint32_t base = 0;
+    if (lane == lead) base = atomicAdd(a.qcount, (uint32_t)n);
+    base = __shfl(base, lead, kWave);
+    for (int i = rank; i < n; i += nact) {
+        const uint32_t k = base + (uint32_t)i;
+        if (k < a.qcap) a.queue[k] = stage[(run.qh + i) & (kBorderStage - 1)];
+    }
+    run.qh = (run.qh + n) & (kBorderStage - 1);
+    run.qn -= n;
+    wave_sync();
+}
+
+// Stage a wave's border rows (need[r]) in ballot order.
+template <int R>
+__device__ __forceinline__ void queue_border(const PointArgs& a, const float (&px)[R], const float (&py)[R],
+                                             const uint32_t (&vraw)[R], const bool (&need)[R], Run& run)
+{
+    float4* stage = border_stage();
+    const int lane = (int)__lane_id();
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const unsigned long long m = __ballot(need[r]);
-        pos[r] = cnt + __popcll(m & below);
-        cnt += __popcll(m);
+        if (m == 0) continue;
+        if (need[r])
+            stage[(run.qh + run.qn + __popcll(m & below)) & (kBorderStage - 1)] =
+                make_float4(px[r], py[r], __uint_as_float(vraw[r]), 0.0f);
+        run.qn += __popcll(m);
+        if (run.qn >= kWave) border_flush(a, run, kWave);
     }
-    if (cnt == 0) return;
-    const unsigned long long act = __ballot(1);
-    const int lead = __ffsll((long long)act) - 1;
-    uint32_t base = 0;
-    if (lane == lead) base = atomicAdd(a.qcount, (uint32_t)cnt);
-    base = __shfl(base, lead, kWave);
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        const uint32_t i = base + (uint32_t)pos[r];
-        if (need[r] && i < a.qcap) a.queue[i] = make_float4(px[r], py[r], __uint_as_float(vraw[r]), 0.0f);
-    }
+}
+
+// The staged rows left at the end of a pass (every lane of the wave; lane
+// 0, whose rows come first, ran every row group the wave ran).
+__device__ __forceinline__ void border_finish(const PointArgs& a, Run& run)
+{
+    run.qh = __shfl(run.qh, 0, kWave);
+    run.qn = __shfl(run.qn, 0, kWave);
+    if (run.qn > 0) border_flush(a, run, run.qn);
 }
 
 // Labels of R rows from their pixels (pix < 0: no label), as point_rows
@@ -717,22 +764,18 @@ __device__ __forceinline__ void run_add(const PointArgs& a, const Acc& A, Run& r
 // Rows [lo, hi) of a label or hybrid pass whose query has constraints, in
 // two stages. Each lane reads the columns of its R rows and tests the
 // constraints; the rows that pass go to the wave's queue in LDS (x, y and
-// the aggregate's value). Whenever the queue holds K waves' worth, every
-// lane takes K of them (rows lane, lane + 64, ...) through the pixel, the
+// the aggregate's value). Whenever the queue holds two waves' worth, every
+// lane takes two of them (rows lane and lane + 64) through the pixel, the
 // label maps (and, in a hybrid pass, the halo test and the border queue)
-// into its polygon's run, the K rows' gathers in flight together. The label
-// work then scales with the rows that pass, not the rows read (500M rows,
-// 10% passing, hybrid: 2.50 -> 1.67 ms; 33% passing, label: 1.54 -> 1.25 ms).
-// Only the wave touches its queue (no workgroup barrier); rows enter it two
-// per lane at a time, so it never holds more than 64 K - 1 + 128 rows.
-//
-// K: the hybrid pass's longer chain of gathers (tile, pixel, halo) gains
-// from three; the label pass keeps two and the smaller queue (with three:
-// 1.75 -> 1.67 ms hybrid, 1.25 -> 1.40 ms label).
-template <int MODE>
-constexpr int drain_rows() { return MODE == PT_HYBRID_QUEUE ? 3 : 2; }
-// entries per wave: a power of two above 64 * drain rows + 127
-constexpr int row_queue(int k) { return k <= 2 ? 256 : 512; }
+// into its polygon's run, the two rows' gathers in flight together. The
+// label work then scales with the rows that pass, not the rows read (500M
+// rows, 10% passing, hybrid: 2.50 -> 1.62 ms; 33% passing, label: 1.54 ->
+// 1.25 ms). Only the wave touches its queue (no workgroup barrier); rows
+// enter it two per lane at a time, so it never holds more than 127 + 128.
+// (Three rows a lane per drain, with a queue of 512: hybrid 1.67 ms, label
+// 1.40 ms; one: hybrid 2.12 ms, label 1.35 ms.)
+constexpr int kDrainRows = 2;
+constexpr int kRowQueue = 256;  // entries per wave (a power of two above 64 * kDrainRows + 127)
 
 template <int MODE, bool AVG, int R, bool LDS>
 __device__ __forceinline__ void rows_compact(const PointArgs& a, int64_t start, int64_t lo, int64_t hi,
@@ -740,8 +783,6 @@ __device__ __forceinline__ void rows_compact(const PointArgs& a, int64_t start, 
 {
     static_assert(MODE == PT_LABEL || MODE == PT_HYBRID_QUEUE, "label and hybrid-queue passes");
     static_assert(R % 2 == 0, "rows enter the queue two at a time");
-    constexpr int kDrainRows = drain_rows<MODE>();
-    constexpr int kRowQueue = row_queue(kDrainRows);
     __shared__ float s_x[kBlock / kWave][kRowQueue];
     __shared__ float s_y[kBlock / kWave][kRowQueue];
     __shared__ uint32_t s_v[AVG ? kBlock / kWave : 1][AVG ? kRowQueue : 1];
@@ -754,11 +795,6 @@ __device__ __forceinline__ void rows_compact(const PointArgs& a, int64_t start, 
     const SlotPlan SP = make_plan(q, a.cols, AVG);
     const int64_t stride = (int64_t)blockDim.x * R;
     int head = 0, cnt = 0;  // the wave's queue: entries head .. head + cnt - 1 (mod kRowQueue)
-    auto wave_sync = [] {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    };
     // the first n (<= K * 64) rows of the queue, K per lane (rows lane,
     // lane + 64, ...): their label-map gathers are in flight together
     auto drain = [&](auto kk, int n) {
@@ -785,7 +821,7 @@ __device__ __forceinline__ void rows_compact(const PointArgs& a, int64_t start, 
             bool need[K];
 #pragma unroll
             for (int k = 0; k < K; k++) need[k] = lab[k] == kLabelBorder;
-            queue_border<K>(a, px, py, v, need);
+            queue_border<K>(a, px, py, v, need, run);
         }
 #pragma unroll
         for (int k = 0; k < K; k++)
@@ -977,7 +1013,7 @@ __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, in
 #pragma unroll
                     for (int r = 0; r < R; r++) need[r] = lab[r] == kLabelBorder;
                     if (MODE == PT_LABEL_HYBRID) exact_compact<AVG, LDS, R>(a, A, agg_t, px, py, vraw, need, key);
-                    else queue_border<R>(a, px, py, vraw, need);
+                    else queue_border<R>(a, px, py, vraw, need, run);
                 }
             }
         }
