@@ -300,10 +300,11 @@ def canvas_section(h, n, W, H, args):
     return {"workload": f"eps={args.eps} m, {n} points, 260 Voronoi polygons, canvas {W}x{H}", **out}
 
 
-def profiled_traffic():
-    """HBM bytes per launch of the headline point pass as measured by the
-    rocprofv3 FETCH_SIZE/WRITE_SIZE runs summarized under profiles/."""
-    p = os.path.join(ROOT, "profiles", "points_label_100M_eps10_traffic.json")
+def profiled_traffic(name="points_label_100M_eps10"):
+    """HBM bytes per launch of a point pass (the headline's by default) as
+    measured by the rocprofv3 FETCH_SIZE/WRITE_SIZE runs summarized under
+    profiles/."""
+    p = os.path.join(ROOT, "profiles", name + "_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
@@ -444,7 +445,7 @@ def workloads_section(h, pts, cols, args):
         "kernels_ms": kms,
         "roofline": {"kernel": kname, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "kernel_ms": round(k_ms, 4),
-                     "alg_bytes": int(alg)},
+                     "alg_bytes": int(alg), "traffic": profiled_traffic("points_hybrid_500M_filters")},
         "border_effective_ref_edge_rate": exact}
     t3.close()
     return out
