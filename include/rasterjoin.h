@@ -303,7 +303,7 @@ int rj_canvas_info(rj_ctx* ctx, int32_t* W, int32_t* H);
  *   RJ_OPT_PASS_WORKGROUPS: workgroups of a single-query point pass; 0 (default): one per
  *                           12288 rows of a spatially ordered table (4096..49152), else 4096
  *   RJ_OPT_BATCH_ROWS:      rows per thread of the fused label-map and index kernels, 4 (default) or 8
- *   RJ_OPT_BATCH_HYBRID_ROWS: rows per thread of the fused hybrid kernel, 4 or 8 (default)
+ *   RJ_OPT_BATCH_HYBRID_ROWS: rows per thread of the fused hybrid kernel, 4 (default) or 8
  *   RJ_OPT_BATCH_SHARE:     1 (default) fused raster queries on the same label map (same
  *                           collection and accuracy) compute each row's label once per chunk;
  *                           0 every query computes its own

@@ -80,8 +80,9 @@ __device__ __forceinline__ void group_keys(const PointArgs& a, int64_t r0, int64
 }
 
 // One member of a group over the chunk: its filters and aggregate column,
-// the keys from LDS.
-template <bool AVG, int R>
+// the keys from LDS. A hybrid member queues its border rows (their x,y are
+// read only for the waves that have one).
+template <int MODE, bool AVG, int R>
 __device__ __forceinline__ void member_rows(const PointArgs& a, int64_t r0, int64_t r1, const uint16_t* skey,
                                             const Acc& A, Run& run)
 {
@@ -117,17 +118,29 @@ __device__ __forceinline__ void member_rows(const PointArgs& a, int64_t r0, int6
             }
         }
         int key[R];
+        bool need[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const uint32_t k16 = ok[r] ? skey[i0 + r - r0] : kKeyNone;
             key[r] = k16 < 0xFFF0u ? (int)k16 : -1;
+            need[r] = MODE == PT_HYBRID_QUEUE && k16 == kKeyBorder;
+        }
+        if (MODE == PT_HYBRID_QUEUE) {
+            bool any = false;
+#pragma unroll
+            for (int r = 0; r < R; r++) any = any || need[r];
+            if (__ballot(any)) {
+                float px[R], py[R];
+                load_xy<R>(a.xy, i0, r1, a.end, px, py);
+                queue_border<R>(a, px, py, vraw, need, run);
+            }
         }
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int k = key[r];
             if (k < 0) continue;
             if (k != run.ck) {
-                if (run.ck >= 0) run_flush<PT_LABEL, AVG>(a, A, run.ck, run.cc, run.ch, run.cl);
+                if (run.ck >= 0) run_flush<MODE, AVG>(a, A, run.ck, run.cc, run.ch, run.cl);
                 run.ck = k;
                 run.cc = 0;
                 run.ch = run.cl = 0;
@@ -143,7 +156,7 @@ __device__ __forceinline__ void member_rows(const PointArgs& a, int64_t r0, int6
     }
 }
 
-template <bool AVG, int R>
+template <int MODE, bool AVG, int R>
 __device__ __forceinline__ void member_query(const PointArgs& a, int64_t r0, int64_t r1, const uint16_t* skey,
                                              unsigned long long* sacc, int lds_cap)
 {
@@ -151,8 +164,9 @@ __device__ __forceinline__ void member_query(const PointArgs& a, int64_t r0, int
     __syncthreads();  // the previous query's sums have been flushed
     acc_init(A, sacc, a.P.np, lds_cap, AVG, a.res);
     Run run{-1, 0, 0, 0};
-    member_rows<AVG, R>(a, r0, r1, skey, A, run);
-    if (run.ck >= 0) run_flush<PT_LABEL, AVG>(a, A, run.ck, run.cc, run.ch, run.cl);
+    member_rows<MODE, AVG, R>(a, r0, r1, skey, A, run);
+    if (MODE == PT_HYBRID_QUEUE) border_finish(a, run);
+    if (run.ck >= 0) run_flush<MODE, AVG>(a, A, run.ck, run.cc, run.ch, run.cl);
     acc_flush(A, AVG);
 }
 
@@ -192,23 +206,24 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchBlock* __restrict__
                                (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)sm);
         while (m) {
             const int q = __ffsll((long long)m) - 1;
-            if (CLS == BC_LABEL) {
+            if (CLS == BC_LABEL || CLS == BC_HYBRID) {
                 // the members of q's group in this chunk share its labels
+                constexpr int GM = CLS == BC_HYBRID ? PT_HYBRID_QUEUE : PT_LABEL;
                 const int g = CB->grp[q];
                 const unsigned long long gm = g >= 0 ? (m & CB->gmask[g]) : 0ull;
                 if (__popcll(gm) >= 2) {
                     m &= ~gm;
                     uint16_t* skey = reinterpret_cast<uint16_t*>(reinterpret_cast<char*>(sacc) + B->key_off);
                     __syncthreads();  // the previous group's members have read their keys
-                    group_keys<false, R>(*(const PointArgs*)(&CB->q[q]), r0, r1, skey);
+                    group_keys<CLS == BC_HYBRID, R>(*(const PointArgs*)(&CB->q[q]), r0, r1, skey);
                     unsigned long long mm = gm;
                     while (mm) {
                         const int j = __ffsll((long long)mm) - 1;
                         mm &= mm - 1;
                         const PointArgs& aj = *(const PointArgs*)(&CB->q[j]);
                         const bool avg = aj.q.agg_type != RJ_AGG_COUNT;
-                        if (avg) member_query<true, R>(aj, r0, r1, skey, sacc, lds_cap);
-                        else member_query<false, R>(aj, r0, r1, skey, sacc, lds_cap);
+                        if (avg) member_query<GM, true, R>(aj, r0, r1, skey, sacc, lds_cap);
+                        else member_query<GM, false, R>(aj, r0, r1, skey, sacc, lds_cap);
                     }
                     continue;
                 }

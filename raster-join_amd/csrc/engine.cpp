@@ -1567,16 +1567,16 @@ int rj_ctx::batch_launch(int slot, rj_table* t, const QuerySpec* qs, const std::
         B.cls[c] |= 1ull << j;
         classes |= 1u << c;
     }
-    // groups: raster members that read the same 16-bit label map share each
-    // chunk's labels (computed once into LDS)
+    // groups: raster members that read the same 16-bit label map, and
+    // hybrid members (queue mode) that read the same 16-bit hybrid map, share
+    // each chunk's labels (computed once into LDS)
     const int64_t chunk_max = (opt_batch_chunk_rows + 2047) / 2048 * 2048;
     B.ngroups = 0;
     for (size_t j = 0; j < members.size(); j++) B.grp[j] = -1;
     if (opt_batch_share && chunk_max <= kGroupChunkMax) {
         for (size_t j = 0; j < members.size(); j++) {
-            // raster members only: hybrid members spend their time on their own
-            // filters, border queues and exact tests, and measured no gain
-            if (B.mode[j] != PT_LABEL || !B.q[j].label16 || B.grp[j] >= 0) continue;
+            if ((B.mode[j] != PT_LABEL && B.mode[j] != PT_HYBRID_QUEUE) || !B.q[j].label16 || B.grp[j] >= 0)
+                continue;
             unsigned long long m = 1ull << j;
             for (size_t i = j + 1; i < members.size(); i++)
                 if (B.mode[i] == B.mode[j] && B.q[i].label == B.q[j].label) m |= 1ull << i;

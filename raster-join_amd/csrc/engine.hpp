@@ -187,7 +187,7 @@ struct rj_ctx {
     int raster_tshift() const { return opt_tshift ? opt_tshift : 3; }
     int hybrid_tshift() const { return opt_tshift ? opt_tshift : rj::kTileShiftDefault; }
     int opt_batch_rows = 4;                // rows per thread of the fused label-map and index kernels
-    int opt_batch_hybrid_rows = 8;         // rows per thread of the fused hybrid kernel
+    int opt_batch_hybrid_rows = 4;         // rows per thread of the fused hybrid kernel
     bool opt_batch_share = true;           // fused members on one label map share its lookups
     int opt_batch_wg = 2048;               // persistent workgroups of a fused pass
     int opt_batch_chunk_rows = 8192;       // rows per chunk of a fused pass
