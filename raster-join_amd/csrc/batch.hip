@@ -93,11 +93,18 @@ __device__ __forceinline__ void member_rows(const PointArgs& a, int64_t r0, int6
     const bool plain = q.plain || (r0 >= a.in_b && r1 <= a.in_e);
     for (int64_t base = r0 - r0 % R; base < r1; base += stride) {
         const int64_t i0 = base + (int64_t)threadIdx.x * R;
-        if (i0 >= r1) continue;
-        const bool full = (i0 + R <= a.end);
+        // interior: the workgroup's whole row group lies in the chunk
+        const bool interior = base >= r0 && base + stride <= r1;
+        if (!interior && i0 >= r1) continue;
+        const bool full = interior || (i0 + R <= a.end);
         bool ok[R];
+        if (interior) {
 #pragma unroll
-        for (int r = 0; r < R; r++) ok[r] = (i0 + r >= r0) && (i0 + r < r1);
+            for (int r = 0; r < R; r++) ok[r] = true;
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) ok[r] = (i0 + r >= r0) && (i0 + r < r1);
+        }
         uint32_t vraw[R];
 #pragma unroll
         for (int r = 0; r < R; r++) vraw[r] = 0;

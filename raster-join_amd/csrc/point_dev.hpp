@@ -925,11 +925,19 @@ __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, in
         }
         constexpr bool late = MODE == PT_LABEL || MODE == PT_LABEL_HYBRID || MODE == PT_HYBRID_QUEUE;
         if (!late) load_xy<R>(a.xy, i0 + stride, hi, a.end, nx, ny);
-        if (i0 >= hi) continue;
-        const bool full = (i0 + R <= a.end);
+        // interior: the workgroup's whole row group lies in [lo, hi) (and so
+        // in the table): no per-row bounds
+        const bool interior = base >= lo && base + stride <= hi;
+        if (!interior && i0 >= hi) continue;
+        const bool full = interior || (i0 + R <= a.end);
         bool ok[R];
+        if (interior) {
 #pragma unroll
-        for (int r = 0; r < R; r++) ok[r] = (i0 + r >= lo) && (i0 + r < hi);
+            for (int r = 0; r < R; r++) ok[r] = true;
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) ok[r] = (i0 + r >= lo) && (i0 + r < hi);
+        }
         // raw 32-bit aggregate values; decoded to fixed point only when added
         uint32_t vraw[R];
 #pragma unroll
