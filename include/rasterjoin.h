@@ -176,7 +176,11 @@ int rj_execute(rj_ctx* ctx, int32_t fn, int64_t* counts, double* sums, int32_t* 
 /* Exact sums of the last execute: sum = hi + lo * 2^-32 per polygon. */
 int rj_get_sums_fixed(rj_ctx* ctx, int64_t* hi, uint64_t* lo);
 /* GLHandler::getErrorBounds (GLHandler.cpp:206-212): 4*np values
- * [fn | fp | fn1 | fp1] of the last RJ_ERRORBOUNDS execute. */
+ * [fn | fp | fn1 | fp1] of the last RJ_ERRORBOUNDS execute, summed over each
+ * polygon's boundary pixels (pixels an edge touches, dilated by one) whose
+ * center is outside (fn) or inside (fp): fn1/fp1 add the pixel's count,
+ * fn/fp add ceil(min(a, 1 - a) * count) with a the polygon's exact area share
+ * of the pixel (line-bound.frag:178-183, 247-257). */
 int rj_error_bounds(rj_ctx* ctx, int64_t* out4np);
 /* GLHandler::printTimeStats (GLHandler.cpp:214-248): the tab-separated timing
  * line (minimum over executions, ms). */

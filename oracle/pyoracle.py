@@ -63,6 +63,7 @@ def lib():
                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p]
         _lib.orc_boundary.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
+        _lib.orc_area.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
         _lib.orc_errorbounds.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
                                          C.c_void_p, C.c_void_p]
     return _lib
@@ -191,6 +192,18 @@ def coverage(p, xy, method):
     sub = 4 if method >= 2 else 1
     out = np.zeros((p.H * sub, p.W * sub), np.uint8)
     lib().orc_coverage(C.byref(p), _p(xy), len(xy) // 2, method, _p(out))
+    return out
+
+
+AREA_ONE = 1 << 24
+
+
+def area(p, xy):
+    """Signed area of the snapped ring inside each pixel, H x W int32 in
+    units of 2^-24 of a pixel (orc_area)."""
+    xy = np.ascontiguousarray(xy, np.float32)
+    out = np.zeros((p.H, p.W), np.int32)
+    lib().orc_area(C.byref(p), _p(xy), len(xy) // 2, _p(out))
     return out
 
 

@@ -685,9 +685,114 @@ int orc_boundary(const orc_plan* p, const float* xy, int32_t nv, int32_t r, uint
 }
 
 /* ------------------------------------------------------------------------- */
+/* Area of a ring inside each pixel (the area fraction of line-bound.frag's   */
+/* computeFraction, :48-187, taken over the whole polygon): for every row a  */
+/* snapped edge crosses, the area of the row's piece between the edge and    */
+/* each column's right side, G(X) = int max(X - x(y), 0) dy floored to 1/256 */
+/* square subunit, is deposited as column differences; the row prefix sum of */
+/* the deposits of all edges, signed by edge direction, is the pixel's area  */
+/* (2^24 = the whole pixel).                                                 */
+/* ------------------------------------------------------------------------- */
+
+#define ORC_AREA_ONE 16777216LL
+
+static int64_t area_g(double X, double lo, double hi, double mid, double hd, double k)
+{
+    double g;
+    if (X <= lo) return 0;
+    if (X >= hi) g = hd * (X - mid);
+    else g = (X - lo) * (X - lo) * k;
+    return (int64_t)floor(g * 256.0);
+}
+
+/* Deposits of edge (ax,ay)-(bx,by) into acc (bh rows of bw, first pixel     */
+/* (c0, r0)); columns left of c0 land on c0, columns past the region drop.   */
+static void area_edge(int64_t ax, int64_t ay, int64_t bx, int64_t by, int64_t c0, int64_t r0,
+                      int64_t bw, int64_t bh, uint32_t* acc)
+{
+    if (ay == by) return;
+    uint32_t sg = 0xFFFFFFFFu;  /* upward edge: -1, so a counter-clockwise ring sums positive */
+    if (by < ay) {
+        int64_t t = ax; ax = bx; bx = t;
+        t = ay; ay = by; by = t;
+        sg = 1u;
+    }
+    int64_t den = by - ay;
+    int64_t j0 = floor_div(ay, 256), j1 = ceil_div(by, 256) - 1;
+    if (j0 < r0) j0 = r0;
+    if (j1 > r0 + bh - 1) j1 = r0 + bh - 1;
+    for (int64_t j = j0; j <= j1; j++) {
+        int64_t ya = ay > 256 * j ? ay : 256 * j;
+        int64_t yb = by < 256 * j + 256 ? by : 256 * j + 256;
+        int64_t na = ax * den + (bx - ax) * (ya - ay);
+        int64_t nb = ax * den + (bx - ax) * (yb - ay);
+        double ua = (double)na / (double)den, ub = (double)nb / (double)den;
+        double lo = ua < ub ? ua : ub, hi = ua < ub ? ub : ua;
+        double mid = 0.5 * (ua + ub), hd = (double)(yb - ya);
+        double k = hi > lo ? hd / (2.0 * (hi - lo)) : 0.0;
+        int64_t cl = floor_div(na < nb ? na : nb, 256 * den);
+        int64_t ch = floor_div(na < nb ? nb : na, 256 * den);
+        uint32_t* row = acc + (j - r0) * bw;
+        int64_t gl = 0, fl = 0;
+        for (int64_t c = cl; c <= ch + 1; c++) {
+            int64_t f;
+            if (c <= ch) {
+                int64_t g = area_g((double)(256 * (c + 1)), lo, hi, mid, hd, k);
+                f = g - gl;
+                gl = g;
+            } else {
+                f = 65536 * (yb - ya);
+            }
+            uint32_t d = (uint32_t)(f - fl) * sg;
+            fl = f;
+            int64_t i = c - c0 < 0 ? 0 : c - c0;
+            if (i < bw) row[i] += d;
+        }
+    }
+}
+
+/* Area shares of one ring over the rectangle [c0, c0+bw) x [r0, r0+bh):    */
+/* acc receives the signed per-pixel areas (int32 in 2^-24 of a pixel,      */
+/* positive for a counter-clockwise ring).                                   */
+static void area_region(const int64_t* X, const int64_t* Y, int n, int64_t c0, int64_t r0,
+                        int64_t bw, int64_t bh, uint32_t* acc)
+{
+    memset(acc, 0, sizeof(uint32_t) * (size_t)(bw * bh));
+    for (int i = 0, j = n - 1; i < n; j = i++)
+        area_edge(X[j], Y[j], X[i], Y[i], c0, r0, bw, bh, acc);
+    for (int64_t r = 0; r < bh; r++) {
+        uint32_t run = 0;
+        for (int64_t c = 0; c < bw; c++) {
+            run += acc[r * bw + c];
+            acc[r * bw + c] = run;
+        }
+    }
+}
+
+/* min(a, 1 - a) of a pixel's signed area sum, |area| capped at one pixel. */
+static uint32_t area_min_share(uint32_t acc)
+{
+    int64_t v = (int32_t)acc;
+    if (v < 0) v = -v;
+    if (v > ORC_AREA_ONE) v = ORC_AREA_ONE;
+    return (uint32_t)(v < ORC_AREA_ONE - v ? v : ORC_AREA_ONE - v);
+}
+
+int orc_area(const orc_plan* p, const float* xy, int32_t nv, int32_t* out)
+{
+    int64_t* X = (int64_t*)malloc(sizeof(int64_t) * nv);
+    int64_t* Y = (int64_t*)malloc(sizeof(int64_t) * nv);
+    orc_snap(p, xy, nv, X, Y);
+    area_region(X, Y, nv, 0, 0, p->W, p->H, (uint32_t*)out);
+    free(X); free(Y);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Errorbounds (RasterJoinBounds.cpp:306-381, line-bound.frag:219-262) with  */
 /* boundary pixels = dilate1(touched) per polygon, inside = pixel-center     */
-/* rule, fraction f = share of the 4x4 quarter-pixel centers inside.         */
+/* rule, and the estimate ceil(min(a, 1 - a) * count) per boundary pixel     */
+/* from the exact area share a of the polygon in the pixel (area_region).    */
 /* ------------------------------------------------------------------------- */
 
 int orc_errorbounds(const orc_plan* p, const float* pxy, const int32_t* off, int32_t np,
@@ -707,27 +812,23 @@ int orc_errorbounds(const orc_plan* p, const float* pxy, const int32_t* off, int
         if (bw > 0 && bh > 0) {
             uint8_t* bm = (uint8_t*)calloc((size_t)(bw * bh), 1);
             uint8_t* in = (uint8_t*)malloc((size_t)(bw * bh));
-            uint8_t* sub = (uint8_t*)malloc((size_t)(16 * bw * bh));
+            uint32_t* area = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(bw * bh));
             mark_ctx c = { bm, p->W, p->H, 1, c0, r0, bw, bh };
             for (int i = 0, j = n - 1; i < n; j = i++)
                 touched(X[j], Y[j], X[i], Y[i], p->W, p->H, mark_dilate, &c);
             fill_region(X, Y, n, 256, c0, c1, r0, r1, in);
-            fill_region(X, Y, n, 64, 4 * c0, 4 * c1 + 3, 4 * r0, 4 * r1 + 3, sub);
+            area_region(X, Y, n, c0, r0, bw, bh, area);
             for (int64_t r = 0; r < bh; r++)
                 for (int64_t cc = 0; cc < bw; cc++) {
                     if (!bm[r * bw + cc]) continue;
                     uint32_t val = cnt[(r + r0) * p->W + (cc + c0)];
                     if (val == 0) continue;
-                    int k16 = 0;
-                    for (int sy = 0; sy < 4; sy++)
-                        for (int sx = 0; sx < 4; sx++)
-                            k16 += sub[(4 * r + sy) * (4 * bw) + 4 * cc + sx];
-                    int m = k16 < 16 - k16 ? k16 : 16 - k16;
-                    int64_t add = ((int64_t)m * val + 15) / 16;
+                    uint64_t m = area_min_share(area[r * bw + cc]);
+                    int64_t add = (int64_t)((m * val + (uint64_t)(ORC_AREA_ONE - 1)) >> 24);
                     if (in[r * bw + cc]) { fp += add; fp1 += val; }
                     else { fn += add; fn1 += val; }
                 }
-            free(bm); free(in); free(sub);
+            free(bm); free(in); free(area);
         }
         bounds[q] = fn;
         bounds[q + np] = fp;

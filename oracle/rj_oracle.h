@@ -99,6 +99,11 @@ int orc_raster_join(const orc_plan* p, const float* pxy, const int32_t* off, int
 /* Boundary pixels of one polygon (touched by an edge, dilated by r pixels). */
 int orc_boundary(const orc_plan* p, const float* xy, int32_t nv, int32_t r, uint8_t* out);
 
+/* Area of one ring inside each pixel: out[j*W+i] = signed area in units of
+ * 2^-24 of a pixel (positive for a counter-clockwise ring), exact up to one unit per edge
+ * crossing the pixel's row. */
+int orc_area(const orc_plan* p, const float* xy, int32_t nv, int32_t* out);
+
 /* Errorbounds: bounds = [fn | fp | fn1 | fp1] (4*np int64) from a count canvas. */
 int orc_errorbounds(const orc_plan* p, const float* pxy, const int32_t* off, int32_t np,
                     const uint32_t* canvas_cnt, int64_t* bounds);

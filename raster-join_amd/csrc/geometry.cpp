@@ -132,7 +132,7 @@ void rasterize_polys(const PolySet& ps, const CanvasPlan& plan, PolyRaster* out)
         // error-bounds bands span the full region width
         {
             const int wpr = (bw + 31) / 32;
-            const int rows = std::min(64, (bound_words - 2 * wpr) / (18 * wpr));
+            const int rows = std::min(64, (bound_words - 2 * wpr - kBandScan) / (34 * wpr));
             if (rows < 1) {
                 out->bounds_ok = false;
                 continue;

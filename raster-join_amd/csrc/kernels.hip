@@ -127,6 +127,31 @@ __device__ __forceinline__ void prefix_xor_row(uint32_t* w, int nw)
     }
 }
 
+// Inclusive prefix sums (wrapping) along nr rows of bw words, row stride
+// `stride`, in LDS: a power-of-two group of threads per row, each thread a
+// contiguous chunk; tot holds blockDim.x chunk totals. Every thread calls it.
+__device__ __forceinline__ void prefix_sum_rows(uint32_t* a, int nr, int bw, int stride, uint32_t* tot)
+{
+    const int tpr = 1 << (31 - __clz((int)blockDim.x / nr));
+    const int row = threadIdx.x / tpr, part = threadIdx.x % tpr;
+    const int chunk = (bw + tpr - 1) / tpr;
+    const int b = min(part * chunk, bw), e = min(b + chunk, bw);
+    uint32_t* ar = a + row * stride;
+    uint32_t s = 0;
+    if (row < nr)
+        for (int c = b; c < e; c++) s += ar[c];
+    tot[threadIdx.x] = s;
+    __syncthreads();
+    if (row < nr) {
+        uint32_t run = 0;
+        for (int q = row * tpr; q < (int)threadIdx.x; q++) run += tot[q];
+        for (int c = b; c < e; c++) {
+            run += ar[c];
+            ar[c] = run;
+        }
+    }
+}
+
 __device__ __forceinline__ int bit_at(const uint32_t* row, int col)
 {
     return (row[col >> 5] >> (col & 31)) & 1;
@@ -148,8 +173,10 @@ __global__ __launch_bounds__(kBlock) void k_band(const Band* __restrict__ bands,
     const int bw = c1 - c0 + 1;
     uint32_t* tog = lds;
     uint32_t* tch = tog + nr * wpr;
-    uint32_t* sub = tch + (nr + 2) * wpr;
-    const int nw = (MODE == BAND_BOUNDS) ? (nr * wpr + (nr + 2) * wpr + 16 * nr * wpr) : nr * wpr;
+    const int astride = 32 * wpr;
+    uint32_t* area = tch + (nr + 2) * wpr;  // BAND_BOUNDS: nr rows of per-pixel area sums
+    uint32_t* scan = area + nr * astride;    // BAND_BOUNDS: kBandScan chunk totals
+    const int nw = (MODE == BAND_BOUNDS) ? (nr * wpr + (nr + 2) * wpr + nr * astride) : nr * wpr;
     for (int k = threadIdx.x; k < nw; k += blockDim.x) lds[k] = 0;
     __syncthreads();
 
@@ -162,14 +189,18 @@ __global__ __launch_bounds__(kBlock) void k_band(const Band* __restrict__ bands,
         if (max(ay, by) < band_ylo || min(ay, by) > band_yhi) continue;
         edge_toggles(ax, ay, bx, by, kSub, r0, nr, c0, bw, wpr, tog);
         if (MODE == BAND_BOUNDS) {
-            edge_toggles(ax, ay, bx, by, kSub / 4, 4 * r0, 4 * nr, 4 * c0, 4 * bw, 4 * wpr, sub);
+            edge_area(ax, ay, bx, by, r0, nr, [&](int64_t row, int64_t c, uint32_t d) {
+                int64_t l = c - c0;
+                if (l < 0) l = 0;  // everything left of the band enters at its first column
+                if (l >= bw) return;
+                atomicAdd(&area[(row - r0) * astride + l], d);
+            });
             edge_touch(ax, ay, bx, by, r0 - 1, nr + 2, c0, bw, wpr, cv.W, cv.H, tch);
         }
     }
     __syncthreads();
     for (int r = threadIdx.x; r < nr; r += blockDim.x) prefix_xor_row(tog + r * wpr, wpr);
-    if (MODE == BAND_BOUNDS)
-        for (int r = threadIdx.x; r < 4 * nr; r += blockDim.x) prefix_xor_row(sub + r * 4 * wpr, 4 * wpr);
+    if (MODE == BAND_BOUNDS) prefix_sum_rows(area, nr, bw, astride, scan);
     __syncthreads();
 
     unsigned long long s_cnt = 0, s_hi = 0, s_lo = 0;
@@ -238,12 +269,11 @@ __global__ __launch_bounds__(kBlock) void k_band(const Band* __restrict__ bands,
                         if (col + 1 < bw) tb |= bit_at(tr, col + 1);
                     }
                     if (tb) {
-                        int k16 = 0;
-                        const int bit = (4 * col) & 31, word = (4 * col) >> 5;
-                        for (int sy = 0; sy < 4; sy++)
-                            k16 += __popc((sub[(4 * row + sy) * (4 * wpr) + word] >> bit) & 0xFu);
-                        const int m = min(k16, 16 - k16);
-                        const unsigned long long add = ((unsigned long long)m * val + 15ull) / 16ull;
+                        // min(share, 1 - share) of the pixel's area times its count, rounded up
+                        const uint32_t a = area_share(area[row * astride + col]);
+                        const uint32_t m = min(a, (uint32_t)kAreaOne - a);
+                        const unsigned long long add =
+                            ((unsigned long long)m * val + (unsigned long long)(kAreaOne - 1)) >> 24;
                         if (inside) {
                             s_fp += add;
                             s_fp1 += val;

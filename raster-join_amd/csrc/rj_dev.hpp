@@ -231,14 +231,15 @@ inline size_t acc_lds_bytes(int np, int lds_cap, bool avg)
     return (avg ? 3 : 1) * slots * 8 + (hash ? kHashSlots * 4 : 0);
 }
 
+// Threads' chunk totals of the error-bounds kernel's row prefix sums.
+constexpr int kBandScan = 256;
+
 // LDS bytes a band of `nrows` x `wpr` words needs in each mode.
 inline size_t band_lds_bytes(int mode, int nrows, int wpr)
 {
     size_t w = (size_t)nrows * wpr;
-    if (mode == BAND_BOUNDS) {
-        size_t wpr4 = (size_t)4 * wpr;
-        return 4 * (w + (size_t)(nrows + 2) * wpr + (size_t)4 * nrows * wpr4);
-    }
+    if (mode == BAND_BOUNDS)  // toggles, touched rows (+2), area sums (32 per word), scan totals
+        return 4 * (w + (size_t)(nrows + 2) * wpr + (size_t)32 * w + kBandScan);
     return 4 * w;
 }
 

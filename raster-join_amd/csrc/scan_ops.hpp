@@ -80,4 +80,86 @@ RJ_HD uint32_t bits8(const uint32_t* row, int col, int bw, int wpr)
     return m;
 }
 
+// ---------------------------------------------------------------------------
+// Exact area coverage of pixels by a ring (signed-area accumulation)
+// ---------------------------------------------------------------------------
+
+// One pixel's area in the units of the coverage sums: 1/256 of a square
+// subunit, 256 subunits a pixel side.
+constexpr int64_t kAreaOne = int64_t(1) << 24;
+
+// Area of the piece's row band left of the vertical line x = X, floored to
+// the units above: G(X) = integral over the piece's height h of
+// max(X - x(t), 0) dt, x(t) running linearly from lo to hi (mid their mean,
+// k = h / (2 (hi - lo))). IEEE double with no contraction, so host and device
+// builds floor the same value.
+RJ_HD int64_t area_left(double X, double lo, double hi, double mid, double hd, double k)
+{
+    if (X <= lo) return 0;
+    double g;
+    if (X >= hi) {
+        g = hd * (X - mid);
+    } else {
+        const double t = X - lo;
+        g = t * t * k;
+    }
+    return (int64_t)__builtin_floor(g * 256.0);
+}
+
+// Signed area contributions of edge a-b (fixed-point subunits) to the pixels
+// of rows [R0, R0+NR): in every row the edge crosses, emit(row, col, d) adds d
+// (wrapping 32-bit) at columns cmin..cmax+1 of the piece, so that a prefix sum
+// along the row gives, per pixel, the area right of the edge within the
+// row's piece (0 left of it, the full 256 x h subunits right of it), signed
+// by the edge's direction. Summed over a ring's edges the prefix is the
+// ring's winding-weighted area in each pixel, kAreaOne for a covered pixel.
+template <class F>
+RJ_HD void edge_area(int64_t ax, int64_t ay, int64_t bx, int64_t by, int R0, int NR, F&& emit)
+{
+    if (ay == by) return;
+    uint32_t sgn = 1u;
+    if (ay > by) {
+        int64_t t = ax; ax = bx; bx = t;
+        t = ay; ay = by; by = t;
+        sgn = 0xFFFFFFFFu;  // -1
+    }
+    int64_t jlo = sc_floor_div(ay, 256), jhi = sc_ceil_div(by, 256) - 1;
+    if (jlo < R0) jlo = R0;
+    if (jhi > R0 + NR - 1) jhi = R0 + NR - 1;
+    const int64_t den = by - ay, dx = bx - ax;
+    const double dden = (double)den;
+    for (int64_t row = jlo; row <= jhi; row++) {
+        const int64_t ylo = row * 256 > ay ? row * 256 : ay;
+        const int64_t yhi = row * 256 + 256 < by ? row * 256 + 256 : by;
+        const int64_t h = yhi - ylo;
+        // x at the piece's ends: num / den exactly, one rounding each
+        const int64_t n0 = ax * den + dx * (ylo - ay), n1 = ax * den + dx * (yhi - ay);
+        const int64_t nlo = n0 < n1 ? n0 : n1, nhi = n0 < n1 ? n1 : n0;
+        const double u0 = (double)n0 / dden, u1 = (double)n1 / dden;
+        const double lo = u0 < u1 ? u0 : u1, hi = u0 < u1 ? u1 : u0;
+        const double mid = 0.5 * (u0 + u1), hd = (double)h;
+        const double k = hi > lo ? hd / (2.0 * (hi - lo)) : 0.0;
+        const int64_t cmin = sc_floor_div(nlo, 256 * den), cmax = sc_floor_div(nhi, 256 * den);
+        int64_t gprev = 0, fprev = 0;  // G at the column's left side, F of the previous column
+        for (int64_t c = cmin; c <= cmax; c++) {
+            const int64_t g = area_left((double)((c + 1) * 256), lo, hi, mid, hd, k);
+            const int64_t f = g - gprev;  // area right of the edge inside column c
+            emit(row, c, (uint32_t)(f - fprev) * sgn);
+            fprev = f;
+            gprev = g;
+        }
+        emit(row, cmax + 1, (uint32_t)(65536 * h - fprev) * sgn);
+    }
+}
+
+// Area fraction of a pixel from its accumulated coverage: |cov| capped at
+// one pixel (exact for a simple ring; a self-overlapping ring's winding
+// counts once), in units of kAreaOne.
+RJ_HD uint32_t area_share(uint32_t acc)
+{
+    const int32_t v = (int32_t)acc;
+    const int64_t a = v < 0 ? -(int64_t)v : (int64_t)v;
+    return (uint32_t)(a < kAreaOne ? a : kAreaOne);
+}
+
 }  // namespace rj

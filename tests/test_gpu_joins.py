@@ -261,6 +261,28 @@ def test_errorbounds_match_emulator_and_contain_exact(handler, name, eps):
     assert np.all(c - b[3 * np_:] <= exact) and np.all(exact <= c + b[2 * np_:3 * np_])
 
 
+@pytest.mark.parametrize("W", [4000, 9000])
+def test_errorbounds_wide_bands(handler, W):
+    """Polygons spanning most of a wide canvas: error-bounds bands of three
+    rows (W=4000) and of one row each (W=9000: all 256 threads of a band on
+    one row's area prefix sums), equal to the CPU emulation."""
+    from common import star_polygons
+    xy, off = star_polygons(3, seed=77, box=1000.0, rmin=300, rmax=480, vmin=20, vmax=300)
+    rng = np.random.default_rng(78)
+    pts = rng.uniform(0, 1000, 2 * 2_000_000).astype(np.float32)
+    bbox = O.poly_bbox(xy).astype(np.float64)
+    eps = float(max(bbox[2] - bbox[0], bbox[3] - bbox[1]) / W * np.sqrt(2))
+    p = O.plan(bbox, eps)
+    assert W - 1 <= max(p.W, p.H) <= W + 1
+    oc, _, _, canvas = O.raster_join(p, xy, off, O.PointSet(pts), want_canvas=True)
+    ob = O.errorbounds(p, xy, off, canvas)
+    c, _, _, _ = run(handler, rj.ERRORBOUNDS, xy, off, pts, eps=eps)
+    b = handler.get_error_bounds(rj.ERRORBOUNDS)
+    assert np.array_equal(c, oc)
+    assert np.array_equal(b, ob)
+    assert (ob[:2 * (len(off) - 1)] > 0).all()
+
+
 # ---------------------------------------------------------------------------
 # API behaviour
 # ---------------------------------------------------------------------------

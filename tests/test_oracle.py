@@ -337,7 +337,10 @@ def test_errorbounds_contain_exact_counts(eps):
     assert (counts != exact).sum() > 0     # the raster is approximate at this eps
 
 
-def test_errorbounds_fractions_match_brute_subsamples():
+def test_errorbounds_estimate_from_area_shares():
+    """fn/fp: ceil(min(a, 1 - a) * count) over each polygon's boundary pixels
+    outside/inside it, a = the polygon's exact area share of the pixel
+    (O.area, pinned by the three tests below); fn1/fp1 the whole counts."""
     p = small_plan(40)
     xy, off = star_polygons(2, seed=41, box=1000.0, rmin=100, rmax=400)
     rng = np.random.default_rng(2)
@@ -346,17 +349,219 @@ def test_errorbounds_fractions_match_brute_subsamples():
     b = O.errorbounds(p, xy, off, canvas)
     np_ = len(off) - 1
     can = canvas.reshape(p.H, p.W).astype(np.int64)
+    one = O.AREA_ONE
     for k in range(np_):
         v = xy[2 * off[k]:2 * off[k + 1]]
         cov = O.coverage(p, v, 0)
-        sub = O.coverage(p, v, 2).reshape(p.H, 4, p.W, 4).sum(axis=(1, 3))
+        a = np.minimum(np.abs(O.area(p, v).astype(np.int64)), one)
+        m = np.minimum(a, one - a)
+        add = (m * can + one - 1) >> 24
         bnd = O.boundary(p, v, 1)
-        m = np.minimum(sub, 16 - sub)
-        add = (m * can + 15) // 16
         sel_in = (bnd == 1) & (cov == 1)
         sel_out = (bnd == 1) & (cov == 0)
+        assert (add[sel_out] > 0).any() and (add[sel_in] > 0).any()
         assert b[k] == add[sel_out].sum()
         assert b[np_ + k] == add[sel_in].sum()
         assert b[2 * np_ + k] == can[sel_out].sum()
         assert b[3 * np_ + k] == can[sel_in].sum()
         assert counts[k] == can[cov == 1].sum()
+
+
+def shoelace2(X, Y):
+    """Twice the signed area of an integer ring (exact)."""
+    return sum(int(X[i - 1]) * int(Y[i]) - int(X[i]) * int(Y[i - 1]) for i in range(len(X)))
+
+
+def row_pieces(X, Y, j):
+    """Edges whose open y-range meets row j's [256j, 256j + 256]."""
+    n = 0
+    for i in range(len(X)):
+        y0, y1 = sorted((int(Y[i - 1]), int(Y[i])))
+        if y0 != y1 and y0 < 256 * j + 256 and y1 > 256 * j:
+            n += 1
+    return n
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_area_shares_sum_to_the_ring_area(seed):
+    """Summed over the canvas, the per-pixel areas give the snapped ring's
+    shoelace area (each (edge, row) piece floors at most one unit per
+    column it spans, none right of it); covered pixels far from the boundary
+    hold exactly one pixel, outside ones exactly zero."""
+    p = small_plan(96)
+    xy, off = star_polygons(3, seed=50 + seed, box=1000.0, rmin=40, rmax=420, vmin=3, vmax=60)
+    for k in range(len(off) - 1):
+        v = xy[2 * off[k]:2 * off[k + 1]]
+        X, Y = O.snap(p, v)
+        a = O.area(p, v).astype(np.int64)
+        want = shoelace2(X, Y) * 128          # (2A / 2) sub^2 x 256 units
+        spans = sum(abs(int(X[i]) - int(X[i - 1])) // 256 + 2 for i in range(len(X)))
+        assert abs(int(a.sum()) - want) <= spans
+        assert want > 0 and a.min() > -spans  # counter-clockwise ring: positive shares
+        cov = O.coverage(p, v, 1)
+        far = O.boundary(p, v, 1) == 0
+        assert np.all(a[far & (cov == 1)] == O.AREA_ONE)
+        assert np.all(a[far & (cov == 0)] == 0)
+
+
+def clip_area(poly, x0, y0, x1, y1):
+    """Exact area (Fraction) of a simple ring inside the box, by
+    Sutherland-Hodgman clipping against the box's four sides and the
+    shoelace formula."""
+    def clip(pts, inside, cut):
+        out = []
+        for i in range(len(pts)):
+            cur, prv = pts[i], pts[i - 1]
+            if inside(cur):
+                if not inside(prv):
+                    out.append(cut(prv, cur))
+                out.append(cur)
+            elif inside(prv):
+                out.append(cut(prv, cur))
+        return out
+
+    def at_x(a, b, x):
+        return (x, a[1] + (b[1] - a[1]) * (x - a[0]) / (b[0] - a[0]))
+
+    def at_y(a, b, y):
+        return (a[0] + (b[0] - a[0]) * (y - a[1]) / (b[1] - a[1]), y)
+
+    pts = poly
+    for inside, cut in ((lambda q: q[0] >= x0, lambda a, b: at_x(a, b, x0)),
+                        (lambda q: q[0] <= x1, lambda a, b: at_x(a, b, x1)),
+                        (lambda q: q[1] >= y0, lambda a, b: at_y(a, b, y0)),
+                        (lambda q: q[1] <= y1, lambda a, b: at_y(a, b, y1))):
+        if not pts:
+            return Fraction(0)
+        pts = clip(pts, inside, cut)
+    return sum(pts[i - 1][0] * pts[i][1] - pts[i][0] * pts[i - 1][1] for i in range(len(pts))) / 2
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_area_shares_equal_exact_clipping(seed):
+    """Every pixel's area against the exact rational area of the snapped ring
+    clipped to the pixel, to within one unit (2^-24 of a pixel) per edge
+    crossing the pixel's row."""
+    p = small_plan(14)
+    xy, off = star_polygons(2, seed=70 + seed, box=1000.0, rmin=60, rmax=420, vmin=3, vmax=14)
+    for k in range(len(off) - 1):
+        v = xy[2 * off[k]:2 * off[k + 1]]
+        X, Y = O.snap(p, v)
+        ring = [(Fraction(int(x)), Fraction(int(y))) for x, y in zip(X, Y)]
+        a = O.area(p, v)
+        partial = 0
+        for j in range(p.H):
+            tol = row_pieces(X, Y, j) + 1
+            for i in range(p.W):
+                exact = clip_area(ring, 256 * i, 256 * j, 256 * i + 256, 256 * j + 256)
+                assert abs(int(a[j, i]) - exact * 256) <= tol, (k, j, i)
+                partial += 0 < exact < 65536
+        assert partial >= 10
+
+
+def ref_compute_fraction(p1, p2, lb, rt):
+    """line-bound.frag:27-187 (ComputeOutCode, computeArea, computeFraction)
+    in float32: Cohen-Sutherland clip of the segment to the pixel, then the
+    area on one side of the clipped chord as a share of the pixel. Returns
+    (accept, computed, frac)."""
+    f = np.float32
+    x0, y0, x1, y1 = f(p1[0]), f(p1[1]), f(p2[0]), f(p2[1])
+    xmin, ymin, xmax, ymax = f(lb[0]), f(lb[1]), f(rt[0]), f(rt[1])
+
+    def code(x, y):
+        c = 0
+        if x < xmin:
+            c |= 1
+        elif x > xmax:
+            c |= 2
+        if y < ymin:
+            c |= 4
+        elif y > ymax:
+            c |= 8
+        return c
+
+    o0, o1 = code(x0, y0), code(x1, y1)
+    while True:
+        if (o0 | o1) == 0:
+            break
+        if o0 & o1:
+            return False, False, 0.0
+        oo = o0 if o0 else o1
+        if oo & 8:
+            x, y = x0 + (x1 - x0) * (ymax - y0) / (y1 - y0), ymax
+        elif oo & 4:
+            x, y = x0 + (x1 - x0) * (ymin - y0) / (y1 - y0), ymin
+        elif oo & 2:
+            x, y = xmax, y0 + (y1 - y0) * (xmax - x0) / (x1 - x0)
+        else:
+            x, y = xmin, y0 + (y1 - y0) * (xmin - x0) / (x1 - x0)
+        if oo == o0:
+            x0, y0, o0 = x, y, code(x, y)
+        else:
+            x1, y1, o1 = x, y, code(x, y)
+    if x0 > x1:
+        x0, x1, y0, y1 = x1, x0, y1, y0
+
+    def tri(a, b, c):
+        return abs(f(0.5) * abs((b[0] - a[0]) * (c[1] - a[1]) - (b[1] - a[1]) * (c[0] - a[0])))
+
+    quad = None
+    if x0 == xmin:
+        if x1 == xmax:
+            quad = ((xmin, ymin), (xmax, ymin), (x1, y1), (x0, y0))
+        elif y1 == ymax:
+            t = ((xmin, ymax), (x0, y0), (x1, y1))
+        elif y1 == ymin:
+            t = ((xmin, ymin), (x0, y0), (x1, y1))
+        else:
+            return True, False, 0.5
+    elif x1 == xmax:
+        if y0 == ymax:
+            t = ((xmax, ymax), (x0, y0), (x1, y1))
+        elif y0 == ymin:
+            t = ((xmax, ymin), (x1, y1), (x0, y0))
+        else:
+            return True, False, 0.5
+    elif y0 == ymin and y1 == ymax:
+        quad = ((xmin, ymin), (xmin, ymax), (x1, y1), (x0, y0))
+    elif y1 == ymin and y0 == ymax:
+        quad = ((xmin, ymin), (xmin, ymax), (x0, y0), (x1, y1))
+    else:
+        return True, False, 0.5
+    if quad is not None:
+        area = tri(quad[0], quad[1], quad[2]) + tri(quad[0], quad[2], quad[3])
+    else:
+        area = tri(*t)
+    return True, True, float(area / ((ymax - ymin) * (xmax - xmin)))
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_area_share_of_a_one_edge_pixel_equals_the_reference_fraction(seed):
+    """Where one edge crosses a pixel from side to side, min(a, 1 - a) of the
+    polygon's area share equals line-bound.frag's folded computeFraction of
+    that edge (its float32 arithmetic, to 5e-5)."""
+    W = 64
+    p = small_plan(W)
+    rng = np.random.default_rng(90 + seed)
+    u = 1000.0 / W
+    ya, yb = rng.uniform(3 * u, (W - 3) * u, 2)
+    xa, xb = 3.4 * u, (W - 3.4) * u
+    if seed == 4:
+        xa, xb, ya, yb = ya, yb, 3.4 * u, (W - 3.4) * u   # a steep edge
+    # a simple ring whose other edges stay within 4 pixels of the canvas sides
+    quad = np.array([xa, ya, xb, yb, 999.0, 999.0, 1.0 if seed != 4 else 999.0,
+                     999.0 if seed != 4 else 1.0], np.float32)
+    a = np.abs(O.area(p, quad).astype(np.int64))
+    X, Y = O.snap(p, quad)
+    e0, e1 = (X[0] / 256.0, Y[0] / 256.0), (X[1] / 256.0, Y[1] / 256.0)
+    one = O.AREA_ONE
+    checked = 0
+    for j in range(5, W - 5):
+        for i in range(5, W - 5):
+            acc, comp, fr = ref_compute_fraction(e0, e1, (i, j), (i + 1, j + 1))
+            if not (acc and comp):
+                continue
+            share = min(a[j, i], one - a[j, i]) / one
+            assert abs(share - min(fr, 1.0 - fr)) <= 5e-5, (i, j, share, fr)
+            checked += 1
+    assert checked >= 40
