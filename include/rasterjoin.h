@@ -321,12 +321,15 @@ int rj_canvas_info(rj_ctx* ctx, int32_t* W, int32_t* H);
  *   RJ_OPT_EXACT_QUEUE:     1 (default) single-query hybrid passes queue their border
  *                           points for a separate exact pass; 0 tests them inline
  *   RJ_OPT_STREAM_ROWS:     rows per streamed chunk of host tables (0, the default:
- *                           32M rows or what two chunks of the budget left hold) */
+ *                           32M rows or what two chunks of the budget left hold)
+ *   RJ_OPT_GRID_DIRECT:     1: grid builds test every cell of a polygon's box against
+ *                           all its edges; 0 (default): per-vertex and per-edge marks
+ *                           (the same lists; grids already built are kept) */
 enum { RJ_OPT_RASTER_PATH = 1, RJ_OPT_TIMING = 2, RJ_OPT_HYBRID_DILATION = 3, RJ_OPT_CACHE = 4,
        RJ_OPT_ORGANIZE = 5, RJ_OPT_TILE_SHIFT = 6, RJ_OPT_PASS_WORKGROUPS = 7, RJ_OPT_BATCH_ROWS = 8,
        RJ_OPT_BATCH_WORKGROUPS = 9, RJ_OPT_BATCH_CHUNK_ROWS = 10, RJ_OPT_GRAPH = 11,
        RJ_OPT_INDEX_PATH = 12, RJ_OPT_EXACT_QUEUE = 13, RJ_OPT_STREAM_ROWS = 14, RJ_OPT_BATCH_HYBRID_ROWS = 15,
-       RJ_OPT_BATCH_SHARE = 16 };
+       RJ_OPT_BATCH_SHARE = 16, RJ_OPT_GRID_DIRECT = 17 };
 int rj_set_option(rj_ctx* ctx, int32_t key, int64_t value);
 
 /* Per-kernel device time of the last execute (timing 1), or of every execute

@@ -547,6 +547,7 @@ int rj_set_option(rj_ctx* ctx, int32_t key, int64_t value)
         ctx->opt_batch_chunk_rows = (int)value;
         return RJ_OK;
     case RJ_OPT_GRAPH: ctx->opt_graph = value != 0; return RJ_OK;
+    case RJ_OPT_GRID_DIRECT: ctx->opt_grid_direct = value != 0; return RJ_OK;
     case RJ_OPT_EXACT_QUEUE: ctx->opt_queue = value != 0; return RJ_OK;
     case RJ_OPT_STREAM_ROWS:
         if (value < 0 || value > ((int64_t)1 << 32)) return ctx->fail(RJ_ERR_ARG, "stream rows out of range");

@@ -189,6 +189,7 @@ struct rj_ctx {
     int opt_batch_rows = 4;                // rows per thread of the fused label-map and index kernels
     int opt_batch_hybrid_rows = 4;         // rows per thread of the fused hybrid kernel
     bool opt_batch_share = true;           // fused members on one label map share its lookups
+    bool opt_grid_direct = false;          // grid build: every cell against every edge
     int opt_batch_wg = 2048;               // persistent workgroups of a fused pass
     int opt_batch_chunk_rows = 8192;       // rows per chunk of a fused pass
     bool opt_graph = true;                 // fused passes replay a captured HIP graph

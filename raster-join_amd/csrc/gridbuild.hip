@@ -13,9 +13,16 @@
 // round as CPUJoin's float code does, and the candidate lists come out in
 // CPUJoin's order (cell-major, polygons ascending).
 //
-//   1. k_grid_cells: one workgroup per block of a polygon's bounding-box
-//      cells (its vertices in LDS); each thread tests one cell and appends
-//      (cell, polygon) for a candidate, the cell for a deficient one.
+//   1. k_grid_cells: one workgroup per rectangle of up to 64 x 32 of a
+//      polygon's bounding-box cells. The cell test is an OR of terms that
+//      each follow from one vertex or one edge: the cells holding a vertex,
+//      the cells an edge (i, i+1) meets, and the cell corners inside the
+//      polygon, whose crossing parity comes from per-row lists of the
+//      straddling edges' crossings; the deficient test likewise. So the
+//      work grows with the vertices plus the cells, not their product. An
+//      item whose float grid is too fine for that decomposition (or whose
+//      row lists overflow) tests each cell against every edge directly.
+//      Candidates append (cell, polygon), deficient cells the cell.
 //   2. radix sort of the (cell << 32 | polygon) keys; k_grid_offsets writes
 //      the candidate list and every cell's first entry.
 //   3. edge bands: k_edge_rows gives every edge its band range (and each
@@ -33,14 +40,16 @@
 namespace rj {
 namespace {
 
-constexpr int kGridLdsVerts = 4096;  // vertices a workgroup stages in LDS
-constexpr int kGridItemCells = 2048; // bounding-box cells of one work item
+constexpr int kGridLdsVerts = 2048;  // vertices a workgroup stages in LDS
+constexpr int kItemW = 64;           // cell columns of one work item
+constexpr int kItemH = 32;           // cell rows of one work item
+constexpr int kPtRowCap = 120;       // crossings kept per grid-point row of an item
+constexpr int kCenRowCap = 128;      // straddling edges kept per cell-center row of an item
 
 struct GridItem {
     int32_t poly;
-    int32_t first;  // first cell of the polygon's box, numbered (x - stx) * hy + (y - sty)
-    int32_t count;
-    int32_t pad;
+    int32_t x0, y0;  // first cell column and row
+    int32_t wh;      // cell columns | cell rows << 16
 };
 
 // Crossing number, GridIndexF::isInsidePoly (GridIndexF.cpp:383-402).
@@ -165,7 +174,7 @@ __device__ bool polygon_meets_rect(const float* v, int n, double lbx, double lby
 struct GridArgs {
     const float* xy;         // collection vertices (CSR)
     const int32_t* off;
-    const int32_t* mbr;      // per polygon: stx, enx, sty, eny
+    int32_t direct;          // 1: every item tests each cell against every edge
     int32_t xs;
     float minx, miny, width, height;
     unsigned long long* keys;  // candidates: cell << 32 | polygon
@@ -190,35 +199,215 @@ __device__ __forceinline__ void wave_append(bool flag, T val, T* out, unsigned l
     if (flag && (int64_t)pos < cap) out[pos] = val;
 }
 
+// Grid line c (column side) and r (row side) as GridIndexF::buildGrid
+// computes cell bounds (:96-99).
+__device__ __forceinline__ float grid_x(const GridArgs& a, int c) { return (float)c * a.width + a.minx; }
+__device__ __forceinline__ float grid_y(const GridArgs& a, int r) { return (float)r * a.height + a.miny; }
+
+// A cell side's bounds grown as polygon_meets_rect grows them.
+__device__ __forceinline__ void grown(float l, float h, double& gl, double& gh)
+{
+    const double lo = l, hi = h;
+    const double m = 1e-6 * (fabs(hi) + fabs(lo) + (hi - lo));
+    gl = lo - m;
+    gh = hi + m;
+}
+
+// The column (row) of a coordinate, from one division; within one of the
+// true one on a regular item.
+__device__ __forceinline__ int est_cell(float v, float mn, float step, int lo, int hi)
+{
+    const float f = floorf((v - mn) / step);
+    if (!(f >= (float)lo)) return lo;  // NaN included
+    if (f > (float)hi) return hi;
+    return (int)f;
+}
+
+// The decomposed test needs the item's float grid lines at least 1e-5 of
+// their magnitude apart (a coordinate's column estimate is then off by at
+// most one, and a grown cell's margin stays below a tenth of a cell).
+__device__ __forceinline__ bool item_regular(const GridArgs& a, int x0, int y0, int w, int h)
+{
+    const float mx = fmaxf(fabsf(grid_x(a, x0 - 4)), fabsf(grid_x(a, x0 + w + 4)));
+    const float my = fmaxf(fabsf(grid_y(a, y0 - 4)), fabsf(grid_y(a, y0 + h + 4)));
+    return a.width > 1e-5f * mx && a.height > 1e-5f * my;
+}
+
+__device__ __forceinline__ void set_bit(uint32_t* m, int r, int c) { atomicOr(&m[r * 2 + (c >> 5)], 1u << (c & 31)); }
+__device__ __forceinline__ bool get_bit(const uint32_t* m, int r, int c) { return (m[r * 2 + (c >> 5)] >> (c & 31)) & 1u; }
+
 __global__ __launch_bounds__(256) void k_grid_cells(GridArgs a, const GridItem* __restrict__ items)
 {
     __shared__ float sv[2 * kGridLdsVerts];
+    __shared__ uint32_t s_cand[kItemW * kItemH / 32];  // a vertex in the cell, or an edge (i, i+1) meets it
+    __shared__ uint32_t s_meet[kItemW * kItemH / 32];  // a vertex in the grown cell, or an edge crosses it
+    __shared__ uint32_t s_corner[(kItemH + 1) * 3];    // crossing parity of each grid point
+    __shared__ float s_px[(kItemH + 1) * kPtRowCap];   // per grid-point row: its straddling edges' crossings
+    __shared__ int32_t s_ce[kItemH * kCenRowCap];      // per cell-center row: its straddling edges
+    __shared__ int32_t s_np[kItemH + 1], s_nc[kItemH];
+    __shared__ int s_direct;
     const GridItem it = items[blockIdx.x];
     const int p = it.poly;
     const int v0 = a.off[p], n = a.off[p + 1] - v0;
     const float* v = a.xy + 2 * (int64_t)v0;
+    const int x0 = it.x0, y0 = it.y0, w = it.wh & 0xFFFF, h = it.wh >> 16;
     if (n <= kGridLdsVerts) {
         for (int k = threadIdx.x; k < 2 * n; k += blockDim.x) sv[k] = v[k];
-        __syncthreads();
         v = sv;
     }
-    const int stx = a.mbr[4 * p], sty = a.mbr[4 * p + 2], eny = a.mbr[4 * p + 3];
-    const int hy = eny - sty + 1;
+    for (int k = threadIdx.x; k < kItemW * kItemH / 32; k += blockDim.x) s_cand[k] = s_meet[k] = 0u;
+    for (int k = threadIdx.x; k < (kItemH + 1) * 3; k += blockDim.x) s_corner[k] = 0u;
+    for (int k = threadIdx.x; k <= kItemH; k += blockDim.x) {
+        s_np[k] = 0;
+        if (k < kItemH) s_nc[k] = 0;
+    }
+    if (threadIdx.x == 0) s_direct = a.direct || !item_regular(a, x0, y0, w, h);
+    __syncthreads();
+    const bool marks = !s_direct;
+    if (marks) {
+        const int cl = x0 - 4, ch = x0 + w + 3, rl = y0 - 4, rh = y0 + h + 3;
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            const int j = k == 0 ? n - 1 : k - 1;
+            const float vix = v[2 * k], viy = v[2 * k + 1], vjx = v[2 * j], vjy = v[2 * j + 1];
+            // vertex k in a cell (cell_candidate) or a grown cell (polygon_meets_rect)
+            {
+                const int ec = est_cell(vix, a.minx, a.width, cl, ch), er = est_cell(viy, a.miny, a.height, rl, rh);
+                for (int dc = -2; dc <= 2; dc++) {
+                    const int c = ec + dc;
+                    if (c < x0 || c >= x0 + w) continue;
+                    const float lx = grid_x(a, c), hx = grid_x(a, c + 1);
+                    double glx, ghx;
+                    grown(lx, hx, glx, ghx);
+                    const bool inx = vix >= lx && vix <= hx, ingx = (double)vix >= glx && (double)vix <= ghx;
+                    if (!inx && !ingx) continue;
+                    for (int dr = -2; dr <= 2; dr++) {
+                        const int r = er + dr;
+                        if (r < y0 || r >= y0 + h) continue;
+                        const float ly = grid_y(a, r), hy = grid_y(a, r + 1);
+                        double gly, ghy;
+                        grown(ly, hy, gly, ghy);
+                        if (inx && viy >= ly && viy <= hy) set_bit(s_cand, r - y0, c - x0);
+                        if (ingx && (double)viy >= gly && (double)viy <= ghy) set_bit(s_meet, r - y0, c - x0);
+                    }
+                }
+            }
+            // crossings of grid-point rows (the corner tests, GridIndexF::isInsidePoly)
+            for (int r = 0; r <= h; r++) {
+                const float ty = grid_y(a, y0 + r);
+                if ((viy > ty) != (vjy > ty)) {
+                    const float X = (vjx - vix) * (ty - viy) / (vjy - viy) + vix;
+                    const int q = atomicAdd(&s_np[r], 1);
+                    if (q < kPtRowCap) s_px[r * kPtRowCap + q] = X;
+                    else s_direct = 1;
+                }
+            }
+            // edges straddling cell-center rows (the grown cell's center test)
+            for (int r = 0; r < h; r++) {
+                double gly, ghy;
+                grown(grid_y(a, y0 + r), grid_y(a, y0 + r + 1), gly, ghy);
+                const double cy = 0.5 * (gly + ghy);
+                if (((double)viy > cy) != ((double)vjy > cy)) {
+                    const int q = atomicAdd(&s_nc[r], 1);
+                    if (q < kCenRowCap) s_ce[r * kCenRowCap + q] = k;
+                    else s_direct = 1;
+                }
+            }
+            // edge j-k against the cells of its box: edge (k-1, k) of
+            // cell_candidate (k >= 1; the closing edge is not tested there)
+            // and, for every edge, the grown cells it crosses (Liang-Barsky)
+            const int c0 = est_cell(fminf(vix, vjx), a.minx, a.width, cl, ch) - 2;
+            const int c1 = est_cell(fmaxf(vix, vjx), a.minx, a.width, cl, ch) + 2;
+            const int r0 = est_cell(fminf(viy, vjy), a.miny, a.height, rl, rh) - 2;
+            const int r1 = est_cell(fmaxf(viy, vjy), a.miny, a.height, rl, rh) + 2;
+            const double ex0 = vjx, ey0 = vjy, edx = (double)vix - ex0, edy = (double)viy - ey0;
+            for (int c = max(c0, x0); c <= min(c1, x0 + w - 1); c++) {
+                const float lx = grid_x(a, c), hx = grid_x(a, c + 1);
+                double glx, ghx;
+                grown(lx, hx, glx, ghx);
+                for (int r = max(r0, y0); r <= min(r1, y0 + h - 1); r++) {
+                    const float ly = grid_y(a, r), hy = grid_y(a, r + 1);
+                    if (k >= 1) {
+                        const int ca = region_code(vjx, vjy, lx, ly, hx, hy);
+                        const int cb = region_code(vix, viy, lx, ly, hx, hy);
+                        if (!(ca & cb) && segment_meets_cell(vjx, vjy, vix, viy, lx, ly, hx, hy))
+                            set_bit(s_cand, r - y0, c - x0);
+                    }
+                    double gly, ghy;
+                    grown(ly, hy, gly, ghy);
+                    double t0 = 0, t1 = 1;
+                    const double pp[4] = {-edx, edx, -edy, edy};
+                    const double qq[4] = {ex0 - glx, ghx - ex0, ey0 - gly, ghy - ey0};
+                    bool ok = true;
+#pragma unroll
+                    for (int m = 0; m < 4; m++) {
+                        if (!ok) break;
+                        if (pp[m] == 0) {
+                            if (qq[m] < 0) ok = false;
+                        } else {
+                            const double t = qq[m] / pp[m];
+                            if (pp[m] < 0) t0 = (t0 < t) ? t : t0;
+                            else t1 = (t < t1) ? t : t1;
+                            if (t0 > t1) ok = false;
+                        }
+                    }
+                    if (ok) set_bit(s_meet, r - y0, c - x0);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const bool dec = !s_direct;
+    if (dec) {
+        // crossing parity of every grid point of the item
+        for (int k = threadIdx.x; k < (w + 1) * (h + 1); k += blockDim.x) {
+            const int r = k / (w + 1), c = k % (w + 1);
+            const float tx = grid_x(a, x0 + c);
+            const int m = s_np[r];
+            bool in = false;
+            for (int q = 0; q < m; q++) in ^= tx < s_px[r * kPtRowCap + q];
+            if (in) atomicOr(&s_corner[r * 3 + (c >> 5)], 1u << (c & 31));
+        }
+        __syncthreads();
+    }
     // every lane takes part in the appends: the loop runs to a multiple of the block
-    const int rounds = (it.count + (int)blockDim.x - 1) / (int)blockDim.x;
-    for (int r = 0; r < rounds; r++) {
-        const int i = r * (int)blockDim.x + (int)threadIdx.x;
+    const int cells = w * h;
+    const int rounds = (cells + (int)blockDim.x - 1) / (int)blockDim.x;
+    for (int rr = 0; rr < rounds; rr++) {
+        const int i = rr * (int)blockDim.x + (int)threadIdx.x;
         bool cand = false, def = false;
         int cell = 0;
-        if (i < it.count) {
-            const int local = it.first + i;
-            const int x = stx + local / hy, y = sty + local % hy;
-            // cell bounds as GridIndexF::buildGrid computes them (:96-99)
-            const float lbx = (float)x * a.width + a.minx, lby = (float)y * a.height + a.miny;
-            const float rtx = (float)(x + 1) * a.width + a.minx, rty = (float)(y + 1) * a.height + a.miny;
+        if (i < cells) {
+            const int r = i / w, c = i % w;
+            const int x = x0 + c, y = y0 + r;
             cell = y * a.xs + x;
-            cand = cell_candidate(v, n, lbx, lby, rtx, rty);
-            if (!cand) def = polygon_meets_rect(v, n, lbx, lby, rtx, rty);
+            const float lbx = grid_x(a, x), lby = grid_y(a, y), rtx = grid_x(a, x + 1), rty = grid_y(a, y + 1);
+            if (dec) {
+                auto corner = [&](int rr2, int cc2) { return (s_corner[rr2 * 3 + (cc2 >> 5)] >> (cc2 & 31)) & 1u; };
+                cand = get_bit(s_cand, r, c) || corner(r, c) || corner(r + 1, c + 1) || corner(r + 1, c) ||
+                       corner(r, c + 1);
+                if (!cand) {
+                    def = get_bit(s_meet, r, c);
+                    if (!def) {
+                        // the grown cell's center inside (polygon_meets_rect)
+                        double glx, ghx, gly, ghy;
+                        grown(lbx, rtx, glx, ghx);
+                        grown(lby, rty, gly, ghy);
+                        const double cx = 0.5 * (glx + ghx), cy = 0.5 * (gly + ghy);
+                        const int m = s_nc[r];
+                        bool in = false;
+                        for (int q = 0; q < m; q++) {
+                            const int e = s_ce[r * kCenRowCap + q];
+                            const int ej = e == 0 ? n - 1 : e - 1;
+                            const double xi = v[2 * e], yi = v[2 * e + 1], xj = v[2 * ej], yj = v[2 * ej + 1];
+                            in ^= cx < (xj - xi) * (cy - yi) / (yj - yi) + xi;
+                        }
+                        def = in;
+                    }
+                }
+            } else {
+                cand = cell_candidate(v, n, lbx, lby, rtx, rty);
+                if (!cand) def = polygon_meets_rect(v, n, lbx, lby, rtx, rty);
+            }
         }
         wave_append<unsigned long long>(cand, ((unsigned long long)(uint32_t)cell << 32) | (uint32_t)p, a.keys,
                                         &a.count[0], a.cap);
@@ -327,7 +516,6 @@ int rj_ctx::grid_device(rj_polyset* P, int res, Grid* G)
     int rc;
 
     // each polygon's box of cells (GridIndexF.cpp:82-91) and the work items
-    std::vector<int32_t> mbr(4 * (size_t)np);
     std::vector<GridItem> items;
     int64_t total_cells = 0;
     for (int p = 0; p < np; p++) {
@@ -340,27 +528,28 @@ int rj_ctx::grid_device(rj_polyset* P, int res, Grid* G)
         if (eny >= res) eny = res - 1;
         if (stx < 0) stx = 0;
         if (sty < 0) sty = 0;
-        mbr[4 * (size_t)p] = stx; mbr[4 * (size_t)p + 1] = enx;
-        mbr[4 * (size_t)p + 2] = sty; mbr[4 * (size_t)p + 3] = eny;
         const int64_t cells = (enx >= stx && eny >= sty) ? (int64_t)(enx - stx + 1) * (eny - sty + 1) : 0;
-        for (int64_t c = 0; c < cells; c += kGridItemCells)
-            items.push_back({p, (int32_t)c, (int32_t)std::min<int64_t>(kGridItemCells, cells - c), 0});
+        if (cells)
+            for (int x = stx; x <= enx; x += kItemW)
+                for (int y = sty; y <= eny; y += kItemH) {
+                    const int w = std::min(kItemW, enx - x + 1), h = std::min(kItemH, eny - y + 1);
+                    items.push_back({p, x, y, w | (h << 16)});
+                }
         total_cells += cells;
     }
     if (total_cells >= ((int64_t)1 << 31) || ncells >= ((int64_t)1 << 31))
         return fail(RJ_ERR_LIMIT, "grid too fine for this collection");
 
-    DevBuf d_mbr, d_items, d_keys, d_keys2, d_defi, d_count, d_tmp;
+    DevBuf d_items, d_keys, d_keys2, d_defi, d_count, d_tmp;
     const int64_t cap = std::max<int64_t>(1, total_cells);
-    if ((rc = upload(d_mbr, mbr.data(), mbr.size() * 4, "grid boxes")) ||
-        (rc = upload(d_items, items.data(), std::max<size_t>(1, items.size()) * sizeof(GridItem), "grid items")) ||
+    if ((rc = upload(d_items, items.data(), std::max<size_t>(1, items.size()) * sizeof(GridItem), "grid items")) ||
         (rc = alloc(d_keys, (size_t)cap * 8, "grid keys")) || (rc = alloc(d_defi, (size_t)cap * 4, "grid cells")) ||
         (rc = alloc(d_count, 16, "grid counters")))
         return rc;
     unsigned long long* cnt = d_count.as<unsigned long long>();
     if ((rc = launch("grid_clear", [&] { launch_fill_u32(stream, reinterpret_cast<uint32_t*>(cnt), 4, 0); })))
         return rc;
-    GridArgs a{P->d_xy.as<float>(), P->d_off.as<int32_t>(), d_mbr.as<int32_t>(), res, g.minx, g.miny, g.width,
+    GridArgs a{P->d_xy.as<float>(), P->d_off.as<int32_t>(), opt_grid_direct ? 1 : 0, res, g.minx, g.miny, g.width,
                g.height, d_keys.as<unsigned long long>(), d_defi.as<int32_t>(), cnt, cap};
     if (!items.empty() &&
         (rc = launch("grid_cells", [&] {

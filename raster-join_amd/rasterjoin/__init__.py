@@ -30,6 +30,7 @@ OPT_RASTER_PATH, OPT_TIMING, OPT_HYBRID_DILATION, OPT_CACHE, OPT_ORGANIZE = 1, 2
 OPT_TILE_SHIFT, OPT_PASS_WORKGROUPS, OPT_BATCH_ROWS, OPT_BATCH_WORKGROUPS, OPT_BATCH_CHUNK_ROWS, OPT_GRAPH = \
     6, 7, 8, 9, 10, 11
 OPT_INDEX_PATH, OPT_EXACT_QUEUE, OPT_STREAM_ROWS, OPT_BATCH_HYBRID_ROWS, OPT_BATCH_SHARE = 12, 13, 14, 15, 16
+OPT_GRID_DIRECT = 17
 # how a query ran
 PATH_SINGLE, PATH_FUSED = 0, 1
 ERR_ARG, ERR_STATE, ERR_NOMEM, ERR_DEVICE, ERR_LIMIT, ERR_UNSUPPORTED = -1, -2, -3, -4, -5, -6

@@ -66,3 +66,27 @@ def test_device_grid_equals_oracle(handler, name, res):
         lx, ly, hx, hy = minx + x * w, miny + y * h, minx + (x + 1) * w, miny + (y + 1) * h
         near = np.nonzero((bx[:, 0] <= hx + w) & (bx[:, 2] >= lx - w) & (bx[:, 1] <= hy + h) & (bx[:, 3] >= ly - h))[0]
         assert len(set(near.tolist()) - set(cand[cell_off[c]:cell_off[c + 1]].tolist())) > 0, c
+
+
+@pytest.mark.parametrize("name,res,shift", [("tracts2200", 1024, 0.0), ("voronoi260", 2048, 0.0), ("stars", 64, 0.0),
+                                            ("voronoi10", 1024, 0.0), ("voronoi260", 256, 1.0e6),
+                                            ("voronoi20", 1024, 3.0e7)])
+def test_device_grid_marks_equal_per_cell_test(handler, name, res, shift):
+    """The grid build's per-vertex and per-edge marks (default) give the same
+    candidate lists and deficient cells as testing every cell of a polygon's
+    box against all its edges (RJ_OPT_GRID_DIRECT), also with coordinates far
+    from the origin: at 3e7 the float grid is too fine for the marks and the
+    items fall back to the per-cell test."""
+    xy, off = rj.poly_set(name) if name in rj.POLY_SETS else polys(name)
+    xy = (xy.astype(np.float64) + shift).astype(np.float32)
+    out = []
+    for direct in (0, 1):
+        handler.set_option(rj.OPT_GRID_DIRECT, direct)
+        ps = rj.PolySet(handler, xy, off)
+        out.append(ps.grid(res))
+    handler.set_option(rj.OPT_GRID_DIRECT, 0)
+    (c0, k0, d0), (c1, k1, d1) = out
+    assert np.array_equal(c0, c1) and np.array_equal(k0, k1) and np.array_equal(d0, d1)
+    if shift == 0.0:
+        ocell, ocand = O.Grid(xy, off, res).export()
+        assert np.array_equal(c0, ocell) and np.array_equal(k0, ocand)
