@@ -69,11 +69,63 @@ unsigned long long* tiles_aux(const DevBuf& tiles, const DevCanvas& c, int tshif
 // buffers, tables, collections
 // ---------------------------------------------------------------------------
 
+size_t rj::size_class(size_t b)
+{
+    if (b <= 256) return 256;
+    if (b <= ((size_t)1 << 20)) {
+        size_t c = 512;
+        while (c < b) c <<= 1;
+        return c;
+    }
+    const int lg = 63 - __builtin_clzll((unsigned long long)(b - 1));  // 2^lg < b <= 2^(lg+1)
+    const size_t step = (size_t)1 << (lg - 3);
+    return (b + step - 1) / step * step;
+}
+
+namespace {
+constexpr size_t kPoolBlockMax = (size_t)1 << 30;  // larger blocks are freed, not pooled
+constexpr int64_t kPoolUnlimited = (int64_t)4 << 30;  // pool bound of a context without a budget
+}  // namespace
+
+void* rj::Budget::take(size_t cls)
+{
+    auto it = pool.find(cls);
+    if (it == pool.end()) return nullptr;
+    void* p = it->second;
+    pool.erase(it);
+    cached -= (int64_t)cls;
+    return p;
+}
+
+bool rj::Budget::put(void* p, size_t cls)
+{
+    if (cls > kPoolBlockMax) return false;
+    const int64_t room = limit > 0 ? limit - used : kPoolUnlimited;
+    if (cached + (int64_t)cls > std::min<int64_t>(room, kPoolUnlimited)) return false;
+    pool.emplace(cls, p);
+    cached += (int64_t)cls;
+    return true;
+}
+
+void rj::Budget::trim(int64_t keep)
+{
+    while (cached > keep && !pool.empty()) {
+        auto it = std::prev(pool.end());  // largest first
+        (void)hipFree(it->second);
+        cached -= (int64_t)it->first;
+        pool.erase(it);
+    }
+}
+
 void rj::DevBuf::release()
 {
     if (p) {
-        (void)hipFree(p);
-        if (budget) budget->used -= (int64_t)bytes;
+        if (budget) {
+            budget->used -= (int64_t)bytes;
+            if (!budget->put(p, bytes)) (void)hipFree(p);
+        } else {
+            (void)hipFree(p);
+        }
     }
     p = nullptr;
     bytes = 0;
@@ -189,17 +241,32 @@ int rj_ctx::alloc(DevBuf& b, size_t bytes, const char* what)
     if (bytes == 0) bytes = 8;
     if (b.p && b.bytes >= bytes) return RJ_OK;
     b.release();
-    if (budget->limit > 0 && budget->used + (int64_t)bytes > budget->limit) make_room((int64_t)bytes);
-    if (budget->limit > 0 && budget->used + (int64_t)bytes > budget->limit)
-        return fail(RJ_ERR_NOMEM, std::string("memory budget exhausted allocating ") + what + " (" +
-                                      std::to_string(bytes) + " bytes, " + std::to_string(budget->used) + " of " +
-                                      std::to_string(budget->limit) + " in use)");
-    hipError_t e = hipMalloc(&b.p, bytes);
-    if (e != hipSuccess) {
-        b.p = nullptr;
-        (void)hipGetLastError();
-        return hip_fail(e, what);
+    bytes = size_class(bytes);
+    void* p = budget->take(bytes);
+    if (!p) {
+        Budget& B = *budget;
+        auto fits = [&] { return B.limit <= 0 || B.used + (int64_t)bytes <= B.limit; };
+        if (!fits()) make_room((int64_t)bytes);  // evicted structures go to the pool
+        p = B.take(bytes);
+        if (!p) {
+            if (!fits())
+                return fail(RJ_ERR_NOMEM, std::string("memory budget exhausted allocating ") + what + " (" +
+                                              std::to_string(bytes) + " bytes, " + std::to_string(B.used) + " of " +
+                                              std::to_string(B.limit) + " in use)");
+            if (B.limit > 0) B.trim(B.limit - B.used - (int64_t)bytes);
+            hipError_t e = hipMalloc(&p, bytes);
+            if (e != hipSuccess && B.cached > 0) {
+                (void)hipGetLastError();
+                B.trim(0);
+                e = hipMalloc(&p, bytes);
+            }
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                return hip_fail(e, what);
+            }
+        }
     }
+    b.p = p;
     b.bytes = bytes;
     b.budget = budget;
     budget->used += (int64_t)bytes;

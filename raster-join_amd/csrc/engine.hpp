@@ -21,10 +21,32 @@
 
 namespace rj {
 
+// Device memory of a context: the budget (the reference's gpuMemMB,
+// GLFunction.cpp:177-192) and a pool of freed buffers by size class, which
+// alloc reuses before calling hipMalloc. A first query on a new collection
+// allocates a few dozen buffers, and hipMalloc / hipFree take 40-130 us each.
+// Pooled blocks count against the budget and are freed when a buffer would
+// not fit otherwise. Every buffer of a context is used on the context's
+// stream (or behind a synchronisation of its copy stream), so a block
+// released while queued work still reads it is only written again by work
+// queued after that work.
 struct Budget {
-    int64_t limit = 0;  // bytes; <= 0: unlimited
-    int64_t used = 0;
+    int64_t limit = 0;   // bytes; <= 0: unlimited
+    int64_t used = 0;    // bytes of live buffers
+    int64_t cached = 0;  // bytes of pooled blocks
+    std::multimap<size_t, void*> pool;
+    Budget() = default;
+    Budget(const Budget&) = delete;
+    Budget& operator=(const Budget&) = delete;
+    ~Budget() { trim(0); }
+    void* take(size_t cls);          // a pooled block of size class cls, or null
+    bool put(void* p, size_t cls);   // false: the block is not pooled (the caller frees it)
+    void trim(int64_t keep);         // free pooled blocks until at most `keep` bytes stay
 };
+
+// Allocation size classes: powers of two up to 1 MiB, then eight classes per
+// power of two (at most 12.5 % above the request).
+size_t size_class(size_t bytes);
 
 struct DevBuf {
     void* p = nullptr;
