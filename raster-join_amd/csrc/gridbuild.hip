@@ -26,8 +26,9 @@
 //   2. radix sort of the (cell << 32 | polygon) keys; k_grid_offsets writes
 //      the candidate list and every cell's first entry.
 //   3. edge bands: k_edge_rows gives every edge its band range (and each
-//      polygon its first and last band), then one thread per (polygon, band)
-//      counts and fills the band's edges in edge order.
+//      polygon its first and last band), then one thread per edge counts
+//      itself into each of its bands and, after a scan, writes itself into
+//      them (atomic cursors: a band's edges in no particular order).
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -430,10 +431,24 @@ __global__ void k_grid_offsets(const unsigned long long* __restrict__ keys, int6
     for (int64_t c = prev + 1; c <= cur; c++) cell_off[c] = (int32_t)k;
 }
 
+// The polygon of vertex k: the last p with off[p] <= k.
+__device__ __forceinline__ int vertex_poly(const int32_t* __restrict__ off, int np, int64_t k)
+{
+    int a = 0, b = np - 1;
+    while (a < b) {
+        const int m = (a + b + 1) >> 1;
+        if (off[m] <= k) a = m;
+        else b = m - 1;
+    }
+    return a;
+}
+
 // Band range of every edge (vertex k and its predecessor), as the host's
 // band_row: the edge toggles the crossing test of a point at ty iff
 // min(y) <= ty < max(y), whose bands run from band_row(min) to band_row(the
-// largest float below max). Each polygon gets its first and last band.
+// largest float below max). Each polygon gets its first and last band: the
+// lanes of a wave's first and last polygon combine before their atomics
+// (consecutive vertices mostly share a polygon).
 __global__ void k_edge_rows(const float* __restrict__ xy, const int32_t* __restrict__ prev,
                             const int32_t* __restrict__ off, int np, int64_t nv, float miny, float band_h, int rows,
                             int2* __restrict__ er, int32_t* __restrict__ lo, int32_t* __restrict__ hi,
@@ -441,57 +456,83 @@ __global__ void k_edge_rows(const float* __restrict__ xy, const int32_t* __restr
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long span = 0;
+    int poly = -1, r0 = INT_MAX, r1 = INT_MIN;
+    bool has = false;
     if (k < nv) {
         const float yi = xy[2 * k + 1], yj = xy[2 * (int64_t)prev[k] + 1];
         const float y0 = fminf(yi, yj), y1 = fmaxf(yi, yj);
-        int r0 = 0, r1 = -1;
+        int b0 = 0, b1 = -1;
         if (y0 < y1) {
-            r0 = band_row(y0, miny, band_h, rows);
-            r1 = band_row(nextafterf(y1, -INFINITY), miny, band_h, rows);
+            b0 = band_row(y0, miny, band_h, rows);
+            b1 = band_row(nextafterf(y1, -INFINITY), miny, band_h, rows);
         }
-        er[k] = make_int2(r0, r1);
-        if (r0 <= r1) {
-            int a = 0, b = np - 1;  // the polygon of vertex k: the last with off[p] <= k
-            while (a < b) {
-                const int m = (a + b + 1) >> 1;
-                if (off[m] <= k) a = m;
-                else b = m - 1;
-            }
-            atomicMin(&lo[a], r0);
-            atomicMax(&hi[a], r1);
-            span = (unsigned long long)(r1 - r0 + 1);
+        er[k] = make_int2(b0, b1);
+        poly = vertex_poly(off, np, k);
+        if (b0 <= b1) {
+            has = true;
+            r0 = b0;
+            r1 = b1;
+            span = (unsigned long long)(b1 - b0 + 1);
         }
     }
-    for (int d = 32; d > 0; d >>= 1) span += __shfl_xor(span, d, 64);
-    if (__lane_id() == 0 && span) atomicAdd(total, span);
+    const int pa = __shfl(poly, 0, 64), pb = __shfl(poly, 63, 64);
+    int la = (has && poly == pa) ? r0 : INT_MAX, ha = (has && poly == pa) ? r1 : INT_MIN;
+    int lb = (has && poly == pb && pb != pa) ? r0 : INT_MAX, hb = (has && poly == pb && pb != pa) ? r1 : INT_MIN;
+    for (int d = 32; d > 0; d >>= 1) {
+        la = min(la, __shfl_xor(la, d, 64));
+        ha = max(ha, __shfl_xor(ha, d, 64));
+        lb = min(lb, __shfl_xor(lb, d, 64));
+        hb = max(hb, __shfl_xor(hb, d, 64));
+        span += __shfl_xor(span, d, 64);
+    }
+    const int lane = (int)__lane_id();
+    if (lane == 0 && pa >= 0 && la <= ha) {
+        atomicMin(&lo[pa], la);
+        atomicMax(&hi[pa], ha);
+    }
+    if (lane == 63 && pb >= 0 && lb <= hb) {
+        atomicMin(&lo[pb], lb);
+        atomicMax(&hi[pb], hb);
+    }
+    if (has && poly != pa && poly != pb) {
+        atomicMin(&lo[poly], r0);
+        atomicMax(&hi[poly], r1);
+    }
+    if (lane == 0 && span) atomicAdd(total, span);
 }
 
-// One thread per band of one polygon: the number of its edges (FILL = false)
-// or the edges themselves in edge order, (vi, vj) as float4 (FILL = true).
+// One thread per edge (vertex k and its predecessor): +1 to the edge count of
+// every band of its polygon its rows cover (FILL = false; counts at
+// slot[band + 1]), or the edge as (vi, vj) into the next free entry of each
+// such band (FILL = true; slot[band] is the band's cursor, from zero). The
+// entries of a band are taken in no particular order: the crossing parity of
+// the exact test does not depend on it.
 template <bool FILL>
-__global__ void k_band_edges(const float* __restrict__ xy, const int32_t* __restrict__ off,
+__global__ void k_band_edges(const float* __restrict__ xy, const int32_t* __restrict__ off, int np, int64_t nv,
                              const int32_t* __restrict__ prev, const int2* __restrict__ er,
-                             const int32_t* __restrict__ bpoly, const int32_t* __restrict__ blo,
-                             const int32_t* __restrict__ pband, int64_t nbands, int32_t* __restrict__ band_off,
+                             const int32_t* __restrict__ blo, const int32_t* __restrict__ pband,
+                             const int32_t* __restrict__ band_off, int32_t* __restrict__ slot,
                              float4* __restrict__ edges)
 {
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nbands) return;
-    const int p = bpoly[b];
-    const int y = blo[p] + (int)(b - pband[p]);
-    int64_t w = FILL ? band_off[b] : 0;
-    int cnt = 0;
-    for (int k = off[p]; k < off[p + 1]; k++) {
-        const int2 r = er[k];
-        if (y < r.x || y > r.y) continue;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nv) return;
+    const int2 r = er[k];
+    if (r.x > r.y) return;
+    const int p = vertex_poly(off, np, k);
+    const int64_t base = (int64_t)pband[p] - blo[p];
+    float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (FILL) {
+        const int j = prev[k];
+        e = make_float4(xy[2 * k], xy[2 * k + 1], xy[2 * (int64_t)j], xy[2 * (int64_t)j + 1]);
+    }
+    for (int b = r.x; b <= r.y; b++) {
         if (FILL) {
-            const int j = prev[k];
-            edges[w++] = make_float4(xy[2 * (int64_t)k], xy[2 * (int64_t)k + 1], xy[2 * (int64_t)j], xy[2 * (int64_t)j + 1]);
+            const int q = atomicAdd(&slot[base + b], 1);
+            edges[band_off[base + b] + q] = e;
         } else {
-            cnt++;
+            atomicAdd(&slot[base + b + 1], 1);
         }
     }
-    if (!FILL) band_off[b + 1] = cnt;
 }
 
 }  // namespace
@@ -612,7 +653,7 @@ int rj_ctx::grid_device(rj_polyset* P, int res, Grid* G)
 
     // edge bands: the grid rows, split k ways for about 4 edges per band
     const int64_t nv = ps.off[np];
-    DevBuf d_er, d_lohi, d_bpoly, d_scan, d_bcnt;
+    DevBuf d_er, d_lohi, d_scan, d_bcnt;
     if ((rc = alloc(d_er, (size_t)std::max<int64_t>(1, nv) * 8, "edge rows")) ||
         (rc = alloc(d_lohi, (size_t)np * 8 + 16, "band ranges")))
         return rc;
@@ -666,9 +707,6 @@ int rj_ctx::grid_device(rj_polyset* P, int res, Grid* G)
     std::vector<int32_t> pband(np + 1, 0);
     for (int p = 0; p < np; p++) pband[p + 1] = pband[p] + (hi[p] - lo[p] + 1);
     const int64_t nbands = pband[np];
-    std::vector<int32_t> bpoly((size_t)std::max<int64_t>(1, nbands));
-    for (int p = 0; p < np; p++)
-        for (int b = pband[p]; b < pband[p + 1]; b++) bpoly[b] = p;
     G->np = np;
     if ((rc = alloc(G->d_bands, ((size_t)3 * np + 1 + nbands + 1) * 4, "grid row bands"))) return rc;
     int32_t* bands = G->d_bands.as<int32_t>();
@@ -676,20 +714,19 @@ int rj_ctx::grid_device(rj_polyset* P, int res, Grid* G)
     if (e == hipSuccess) e = hipMemcpyAsync(bands + np, hi.data(), (size_t)np * 4, hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = hipMemcpyAsync(bands + 2 * np, pband.data(), (size_t)(np + 1) * 4, hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) return hip_fail(e, "grid row bands");
-    if ((rc = upload(d_bpoly, bpoly.data(), bpoly.size() * 4, "band polygons"))) return rc;
     int32_t* band_off = bands + 3 * np + 1;
     const int32_t* blo = bands;
     const int32_t* pb = bands + 2 * np;
-    const int32_t* bp = d_bpoly.as<int32_t>();
-    const unsigned nbk = (unsigned)((std::max<int64_t>(1, nbands) + 255) / 256);
+    const unsigned nvk = (unsigned)((std::max<int64_t>(1, nv) + 255) / 256);
     if ((rc = alloc(d_bcnt, (size_t)(std::max<int64_t>(1, nbands) + 1) * 4, "band counts"))) return rc;
-    int32_t* bcnt = d_bcnt.as<int32_t>();  // bcnt[b + 1]: edges of band b
+    int32_t* bcnt = d_bcnt.as<int32_t>();  // bcnt[b + 1]: edges of band b; then band b's cursor at bcnt[b]
     if ((rc = launch("band_count", [&] {
              launch_fill_u32(stream, (uint32_t*)band_off, 1, 0);
+             launch_fill_u32(stream, (uint32_t*)bcnt, std::max<int64_t>(1, nbands) + 1, 0);
              if (nbands)
-                 hipLaunchKernelGGL(k_band_edges<false>, dim3(nbk), dim3(256), 0, stream, P->d_xy.as<float>(),
-                                    P->d_off.as<int32_t>(), P->d_prev.as<int32_t>(), er, bp, blo, pb, nbands, bcnt,
-                                    nullptr);
+                 hipLaunchKernelGGL(k_band_edges<false>, dim3(nvk), dim3(256), 0, stream, P->d_xy.as<float>(),
+                                    P->d_off.as<int32_t>(), np, nv, P->d_prev.as<int32_t>(), er, blo, pb, band_off,
+                                    bcnt, nullptr);
          })))
         return rc;
     int32_t nedges = 0;
@@ -705,9 +742,10 @@ int rj_ctx::grid_device(rj_polyset* P, int res, Grid* G)
     if ((rc = alloc(G->d_edges, (size_t)std::max<int32_t>(1, nedges) * 16, "grid band edges"))) return rc;
     float4* ed = G->d_edges.as<float4>();
     if (nbands && (rc = launch("band_fill", [&] {
-                       hipLaunchKernelGGL(k_band_edges<true>, dim3(nbk), dim3(256), 0, stream, P->d_xy.as<float>(),
-                                          P->d_off.as<int32_t>(), P->d_prev.as<int32_t>(), er, bp, blo, pb, nbands,
-                                          band_off, ed);
+                       launch_fill_u32(stream, (uint32_t*)bcnt, nbands, 0);
+                       hipLaunchKernelGGL(k_band_edges<true>, dim3(nvk), dim3(256), 0, stream, P->d_xy.as<float>(),
+                                          P->d_off.as<int32_t>(), np, nv, P->d_prev.as<int32_t>(), er, blo, pb,
+                                          band_off, bcnt, ed);
                    })))
         return rc;
     // the temporaries above are freed when this returns: wait for the kernels
