@@ -389,27 +389,29 @@ int rj_ctx::make_polyset(const float* xy, const int32_t* off, int32_t np, const 
     PolySet& ps = P->ps;
     ps.np = np;
     ps.off.assign(off, off + np + 1);
-    ps.xy.assign(xy, xy + 2 * (size_t)off[np]);
-    float mn[2] = {xy[0], xy[1]}, mx[2] = {xy[0], xy[1]};
-    for (int64_t k = 0; k < off[np]; k++) {
-        for (int d = 0; d < 2; d++) {
-            float v = xy[2 * k + d];
-            if (!std::isfinite(v)) return fail(RJ_ERR_ARG, "non-finite polygon vertex");
-            mn[d] = std::min(mn[d], v);
-            mx[d] = std::max(mx[d], v);
-        }
-    }
+    const int64_t nv = off[np];
+    if (!ps.derive(xy)) return fail(RJ_ERR_ARG, "non-finite polygon vertex");
     if (bbox) {
         for (int d = 0; d < 4; d++) ps.bbox[d] = bbox[d];
     } else {
+        float mn[2] = {ps.pbox[0], ps.pbox[1]}, mx[2] = {ps.pbox[2], ps.pbox[3]};
+        for (int p = 1; p < np; p++) {
+            mn[0] = std::min(mn[0], ps.pbox[4 * p]);
+            mn[1] = std::min(mn[1], ps.pbox[4 * p + 1]);
+            mx[0] = std::max(mx[0], ps.pbox[4 * p + 2]);
+            mx[1] = std::max(mx[1], ps.pbox[4 * p + 3]);
+        }
         ps.bbox[0] = mn[0]; ps.bbox[1] = mn[1]; ps.bbox[2] = mx[0]; ps.bbox[3] = mx[1];
     }
     for (int d = 0; d < 4; d++) ps.bboxf[d] = (float)ps.bbox[d];
     if (!(ps.bbox[2] > ps.bbox[0]) || !(ps.bbox[3] > ps.bbox[1])) return fail(RJ_ERR_ARG, "degenerate bbox");
-    ps.derive();
-    int rc = upload(P->d_xy, ps.xy.data(), ps.xy.size() * 4, "polygons");
+    // the caller's vertices straight to the device (the copy is done when
+    // this returns); the edge list is derived there
+    int rc = upload(P->d_xy, xy, (size_t)nv * 8, "polygons");
     if (!rc) rc = upload(P->d_off, ps.off.data(), ps.off.size() * 4, "polygon offsets");
-    if (!rc) rc = upload(P->d_prev, ps.prev.data(), ps.prev.size() * 4, "edge list");
+    if (!rc) rc = alloc(P->d_prev, (size_t)nv * 4, "edge list");
+    if (!rc)
+        rc = launch("prev", [&] { launch_prev(stream, P->d_off.as<int32_t>(), np, nv, P->d_prev.as<int32_t>()); });
     if (!rc) {
         hipError_t e = hipStreamSynchronize(stream);
         if (e != hipSuccess) rc = hip_fail(e, "polygon upload");

@@ -2,7 +2,9 @@
 // canvas plan, fixed-point snapping and scan-conversion bands, and the pixel
 // rectangles of the grid cells whose candidate lists are incomplete.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <thread>
 #include <cstring>
 
 #include "rj_impl.hpp"
@@ -68,22 +70,45 @@ static int64_t floor_div(int64_t a, int64_t b)
     return q;
 }
 
-void PolySet::derive()
+bool PolySet::derive(const float* xy)
 {
     pbox.assign(4 * (size_t)np, 0.0f);
-    prev.resize(off.empty() ? 0 : off[np]);
-    for (int p = 0; p < np; p++) {
-        const int v0 = off[p], v1 = off[p + 1];
-        float mnx = xy[2 * v0], mxx = mnx, mny = xy[2 * v0 + 1], mxy = mny;
-        for (int k = v0; k < v1; k++) {
-            prev[k] = (k == v0) ? v1 - 1 : k - 1;
-            mnx = std::min(mnx, xy[2 * k]);
-            mxx = std::max(mxx, xy[2 * k]);
-            mny = std::min(mny, xy[2 * k + 1]);
-            mxy = std::max(mxy, xy[2 * k + 1]);
+    const int64_t nv = off.empty() ? 0 : off[np];
+    std::atomic<bool> finite{true};
+    auto run = [&](int pa, int pb) {
+        bool ok = true;
+        for (int p = pa; p < pb; p++) {
+            const int v0 = off[p], v1 = off[p + 1];
+            float mnx = xy[2 * v0], mxx = mnx, mny = xy[2 * v0 + 1], mxy = mny;
+            for (int k = v0; k < v1; k++) {
+                const float x = xy[2 * k], y = xy[2 * k + 1];
+                ok = ok && std::isfinite(x) && std::isfinite(y);
+                mnx = std::min(mnx, x);
+                mxx = std::max(mxx, x);
+                mny = std::min(mny, y);
+                mxy = std::max(mxy, y);
+            }
+            pbox[4 * p] = mnx; pbox[4 * p + 1] = mny; pbox[4 * p + 2] = mxx; pbox[4 * p + 3] = mxy;
         }
-        pbox[4 * p] = mnx; pbox[4 * p + 1] = mny; pbox[4 * p + 2] = mxx; pbox[4 * p + 3] = mxy;
+        if (!ok) finite = false;
+    };
+    // large collections on a few threads, split by vertex count
+    const int nth = nv >= (1 << 18) ? 8 : 1;
+    if (nth == 1) {
+        run(0, np);
+    } else {
+        std::vector<std::thread> th;
+        int pa = 0;
+        for (int t = 0; t < nth; t++) {
+            int pb = pa;
+            const int64_t goal = nv * (t + 1) / nth;
+            while (pb < np && (t == nth - 1 || off[pb] < goal)) pb++;
+            th.emplace_back(run, pa, pb);
+            pa = pb;
+        }
+        for (auto& x : th) x.join();
     }
+    return finite;
 }
 
 int64_t snap_x(const CanvasPlan& plan, float x) { return std::llrint(((double)x - plan.minx) * plan.fxs); }

@@ -43,6 +43,20 @@ __global__ void k_snap(const float* __restrict__ xy, int64_t nv, double minx, do
     Y[k] = (int32_t)rint(((double)xy[2 * k + 1] - miny) * fys);
 }
 
+// The edge list of a collection: the predecessor of every vertex in its ring.
+__global__ void k_prev(const int32_t* __restrict__ off, int np, int64_t nv, int32_t* __restrict__ prev)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nv) return;
+    int a = 0, b = np - 1;  // the polygon of vertex k: the last p with off[p] <= k
+    while (a < b) {
+        const int m = (a + b + 1) >> 1;
+        if (off[m] <= k) a = m;
+        else b = m - 1;
+    }
+    prev[k] = (k == off[a]) ? off[a + 1] - 1 : (int32_t)(k - 1);
+}
+
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -517,6 +531,12 @@ void launch_snap(hipStream_t s, const float* xy, int64_t nv, double minx, double
     if (nv <= 0) return;
     hipLaunchKernelGGL(k_snap, dim3((unsigned)((nv + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, xy, nv, minx,
                        miny, fxs, fys, X, Y);
+}
+
+void launch_prev(hipStream_t s, const int32_t* off, int np, int64_t nv, int32_t* prev)
+{
+    if (nv <= 0) return;
+    hipLaunchKernelGGL(k_prev, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, off, np, nv, prev);
 }
 
 void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v)

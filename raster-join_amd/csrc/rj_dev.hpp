@@ -245,6 +245,8 @@ inline size_t band_lds_bytes(int mode, int nrows, int wpr)
 
 // ---- launchers (kernels.hip) ----
 void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);
+// prev[k]: the vertex before k in its polygon's ring (the last for the first)
+void launch_prev(hipStream_t s, const int32_t* off, int np, int64_t nv, int32_t* prev);
 void launch_snap(hipStream_t s, const float* xy, int64_t nv, double minx, double miny, double fxs, double fys,
                  int32_t* X, int32_t* Y);
 void launch_band(hipStream_t s, int mode, const Band* bands, int nbands, const DevPolys& P,

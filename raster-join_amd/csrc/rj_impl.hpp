@@ -12,15 +12,17 @@ namespace rj {
 
 // A polygon collection in CSR form (PolyHandler::polys / pindexes,
 // db/PolyHandler.cpp:49-75) with its bbox.
+// A collection on the host: offsets and bounds (the vertices and the edge
+// list live on the device only).
 struct PolySet {
     int32_t np = 0;
-    std::vector<float> xy;
     std::vector<int32_t> off;
     double bbox[4] = {0, 0, 0, 0};  // minx, miny, maxx, maxy (double of the float bounds)
     float bboxf[4] = {0, 0, 0, 0};
     std::vector<float> pbox;        // per-polygon float bounds: minx, miny, maxx, maxy
-    std::vector<int32_t> prev;      // predecessor vertex of every vertex (edge list)
-    void derive();                  // fills pbox and prev from xy/off
+    // pbox from the vertices xy (2 * off[np] floats); false when a vertex is
+    // not finite
+    bool derive(const float* xy);
 };
 
 // Canvas geometry for a bbox and accuracy (GLFunction.cpp:26-30,
