@@ -122,7 +122,9 @@ void rasterize_polys(const PolySet& ps, const CanvasPlan& plan, PolyRaster* out)
     out->lds_sum = out->lds_bounds = 0;
     out->bounds_ok = true;
     const int sum_words = 12288;    // LDS words per band, sum / label kernels
-    const int bound_words = 16384;  // LDS words per band, error-bounds kernel
+    // LDS words per band of the error-bounds kernel: bands aim at 8192 words
+    // (eight workgroups per CU), a wide region takes up to 16384 per row
+    const int bound_words = 16384, bound_target = 4096;
     for (int p = 0; p < ps.np; p++) {
         const int64_t mnx = snap_x(plan, ps.pbox[4 * p]), mny = snap_y(plan, ps.pbox[4 * p + 1]);
         const int64_t mxx = snap_x(plan, ps.pbox[4 * p + 2]), mxy = snap_y(plan, ps.pbox[4 * p + 3]);
@@ -157,7 +159,8 @@ void rasterize_polys(const PolySet& ps, const CanvasPlan& plan, PolyRaster* out)
         // error-bounds bands span the full region width
         {
             const int wpr = (bw + 31) / 32;
-            const int rows = std::min(64, (bound_words - 2 * wpr - kBandScan) / (34 * wpr));
+            const int fit = (bound_words - 2 * wpr - kBandScan) / (34 * wpr);
+            const int rows = std::min({64, fit, std::max(1, (bound_target - 2 * wpr - kBandScan) / (34 * wpr))});
             if (rows < 1) {
                 out->bounds_ok = false;
                 continue;

@@ -166,6 +166,19 @@ __device__ __forceinline__ void prefix_sum_rows(uint32_t* a, int nr, int bw, int
     }
 }
 
+// A row of bits grown by one column each way (bit k set when k-1, k or k+1
+// was), in place; bits past the row's last word stay clear.
+__device__ __forceinline__ void hdilate_row(uint32_t* w, int nw)
+{
+    uint32_t prev = 0u, cur = w[0];
+    for (int k = 0; k < nw; k++) {
+        const uint32_t next = k + 1 < nw ? w[k + 1] : 0u;
+        w[k] = cur | (cur << 1) | (prev >> 31) | (cur >> 1) | (next << 31);
+        prev = cur;
+        cur = next;
+    }
+}
+
 __device__ __forceinline__ int bit_at(const uint32_t* row, int col)
 {
     return (row[col >> 5] >> (col & 31)) & 1;
@@ -214,7 +227,11 @@ __global__ __launch_bounds__(kBlock) void k_band(const Band* __restrict__ bands,
     }
     __syncthreads();
     for (int r = threadIdx.x; r < nr; r += blockDim.x) prefix_xor_row(tog + r * wpr, wpr);
-    if (MODE == BAND_BOUNDS) prefix_sum_rows(area, nr, bw, astride, scan);
+    if (MODE == BAND_BOUNDS) {
+        // touched rows grown by one column; the rows above and below join at use
+        for (int r = threadIdx.x; r < nr + 2; r += blockDim.x) hdilate_row(tch + r * wpr, wpr);
+        prefix_sum_rows(area, nr, bw, astride, scan);
+    }
     __syncthreads();
 
     unsigned long long s_cnt = 0, s_hi = 0, s_lo = 0;
@@ -275,14 +292,10 @@ __global__ __launch_bounds__(kBlock) void k_band(const Band* __restrict__ bands,
                 const uint32_t val = cnt[pix];
                 if (inside) s_cnt += val;
                 if (val != 0) {
-                    int tb = 0;
-                    for (int dr = 0; dr < 3 && !tb; dr++) {
-                        const uint32_t* tr = tch + (row + dr) * wpr;
-                        tb |= bit_at(tr, col);
-                        if (col > 0) tb |= bit_at(tr, col - 1);
-                        if (col + 1 < bw) tb |= bit_at(tr, col + 1);
-                    }
-                    if (tb) {
+                    // boundary pixel: a touched pixel within one row and column
+                    const int w = col >> 5;
+                    const uint32_t d = tch[row * wpr + w] | tch[(row + 1) * wpr + w] | tch[(row + 2) * wpr + w];
+                    if ((d >> (col & 31)) & 1u) {
                         // min(share, 1 - share) of the pixel's area times its count, rounded up
                         const uint32_t a = area_share(area[row * astride + col]);
                         const uint32_t m = min(a, (uint32_t)kAreaOne - a);
