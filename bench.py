@@ -271,27 +271,36 @@ def headline(h, pts, cols, xy, off, args):
 
 
 def canvas_section(h, n, W, H, args):
-    """The canvas path on the headline table (RasterJoinBounds and the
-    splat + polygon pass of RasterJoin.cpp:91-194): errorbounds count() and
-    the raster avg(fare) forced onto the canvas, eps = 10 m; polygon
-    structures cached. The splat's algorithmic bytes per launch: x,y of every
-    point (8 B; with sums the fare too, 4 B), plus each canvas pixel written
-    once (4 B count; with sums also 8 + 8 B)."""
+    """Errorbounds and the canvas path on the headline table, eps = 10 m,
+    polygon structures cached: errorbounds count() on its label map (one
+    pass sums the polygons and counts the points of boundary pixels per
+    pixel; RasterJoinBounds.cpp:202-381), the same on the canvas (splat +
+    bounds pass, as the reference draws points then lines), and the raster
+    avg(fare) forced onto the canvas (the splat + polygon pass of
+    RasterJoin.cpp:91-194). Algorithmic bytes per launch of the label pass:
+    x,y of every point (8 B) plus the 16-bit label map read once; of the
+    splat: x,y (8 B; with sums the fare too, 4 B) plus each canvas pixel
+    written once (4 B count; with sums also 8 + 8 B)."""
     out = {}
     steps = max(6, args.steps // 2)
-    for name, fn, agg, path in (("errorbounds_count", rj.ERRORBOUNDS, rj.COUNT, 0),
-                                ("raster_avg_canvas", rj.RASTER, rj.AVG, 1)):
+    LW = (W + 31) // 32 * 32
+    for name, fn, agg, path, kname in (("errorbounds_count", rj.ERRORBOUNDS, rj.COUNT, 0, "points_bounds"),
+                                       ("errorbounds_count_canvas", rj.ERRORBOUNDS, rj.COUNT, 1, "splat"),
+                                       ("raster_avg_canvas", rj.RASTER, rj.AVG, 1, "splat")):
         h.set_aggregation(agg, 1 if agg == rj.AVG else -1)
         h.set_option(rj.OPT_RASTER_PATH, path)
         h.set_option(rj.OPT_TIMING, 1)
         dt, ktot, per = timed_steps(h, fn, steps, 2)
-        k_ms = ktot["splat"][0] / ktot["splat"][1]
-        alg = n * (8 if agg == rj.COUNT else 12) + W * H * (4 if agg == rj.COUNT else 20)
+        k_ms = ktot[kname][0] / ktot[kname][1]
+        if kname == "splat":
+            alg = n * (8 if agg == rj.COUNT else 12) + W * H * (4 if agg == rj.COUNT else 20)
+        else:
+            alg = n * 8 + LW * H * 2
         ach = alg / (k_ms * 1e-3) / 1e9
         out[name] = {"ms_per_join": round(dt / steps * 1e3, 3), **per,
                      "G_points_per_s": round(n * steps / dt / 1e9, 2),
                      "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in ktot.items()},
-                     "roofline": {"kernel": "splat", "bound": "hbm", "achieved": round(ach, 1),
+                     "roofline": {"kernel": kname, "bound": "hbm", "achieved": round(ach, 1),
                                   "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                                   "kernel_ms": round(k_ms, 4), "alg_bytes": int(alg)}}
         log(f"canvas path {name}: {out[name]}")

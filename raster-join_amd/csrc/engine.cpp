@@ -135,7 +135,7 @@ void rj::DevBuf::release()
 size_t rj::Canvas::device_bytes() const
 {
     return d_pc.bytes + d_XY.bytes + d_lab.bytes + d_tiles.bytes + d_hlab.bytes + d_htiles.bytes + d_mask.bytes +
-           d_rects.bytes;
+           d_rects.bytes + d_blab.bytes + d_btiles.bytes + d_bmask.bytes + d_bb.bytes;
 }
 
 size_t rj_polyset::device_bytes() const
@@ -453,13 +453,15 @@ int rj_ctx::canvas_for(rj_polyset* P, double eps_m, double gres, Canvas** out)
     rasterize_polys(P->ps, plan, &C->pr);
     cur_setup += now_ms() - t0;
     t0 = now_ms();
-    // regions and bands: one pinned staging buffer, one copy
+    // regions and bands: one pinned staging buffer, one copy (the
+    // error-bounds bands follow on the first error-bounds join, bounds_for)
+    C->bb_built = false;
+    C->bands_bounds = nullptr;
     const PolyRaster& pr = C->pr;
-    const size_t parts[3] = {pr.region.size() * 4, pr.bands_sum.size() * sizeof(Band),
-                             pr.bands_bounds.size() * sizeof(Band)};
-    const void* srcs[3] = {pr.region.data(), pr.bands_sum.data(), pr.bands_bounds.data()};
-    size_t offs[3], total = 0;
-    for (int k = 0; k < 3; k++) {
+    const size_t parts[2] = {pr.region.size() * 4, pr.bands_sum.size() * sizeof(Band)};
+    const void* srcs[2] = {pr.region.data(), pr.bands_sum.data()};
+    size_t offs[2], total = 0;
+    for (int k = 0; k < 2; k++) {
         offs[k] = total;
         total += al256(parts[k]);
     }
@@ -468,7 +470,7 @@ int rj_ctx::canvas_for(rj_polyset* P, double eps_m, double gres, Canvas** out)
         stage_busy = false;
     }
     if ((rc = pinned(&h_stage, &h_stage_bytes, total, "pinned staging"))) return rc;
-    for (int k = 0; k < 3; k++)
+    for (int k = 0; k < 2; k++)
         if (parts[k]) std::memcpy((char*)h_stage + offs[k], srcs[k], parts[k]);
     if ((rc = alloc(C->d_pc, total, "polygon regions"))) {
         P->canvases.erase(plan.cell);
@@ -482,7 +484,6 @@ int rj_ctx::canvas_for(rj_polyset* P, double eps_m, double gres, Canvas** out)
     char* base = C->d_pc.as<char>();
     C->region = (int32_t*)(base + offs[0]);
     C->bands_sum = (Band*)(base + offs[1]);
-    C->bands_bounds = (Band*)(base + offs[2]);
     const int64_t nv = P->ps.off[P->ps.np];
     if ((rc = alloc(C->d_XY, (size_t)nv * 8, "snapped vertices"))) {
         P->canvases.erase(plan.cell);
@@ -574,6 +575,55 @@ int rj_ctx::raster_labels(rj_polyset* P, Canvas* C, bool* fresh)
     C->lab_tshift = tshift;
     C->lab_state = 0;  // confirmed when the counters are read
     *fresh = true;
+    return RJ_OK;
+}
+
+// The error-bounds bands of a canvas, planned and uploaded on its first
+// error-bounds join.
+int rj_ctx::bounds_for(rj_polyset* P, Canvas* C)
+{
+    if (C->bb_built) return RJ_OK;
+    bounds_bands(P->ps, &C->pr);
+    int rc = upload(C->d_bb, C->pr.bands_bounds.data(), C->pr.bands_bounds.size() * sizeof(Band), "error-bounds bands");
+    if (rc) return rc;
+    C->bands_bounds = C->d_bb.as<Band>();
+    C->bb_built = true;
+    return RJ_OK;
+}
+
+// The error-bounds label map: the raster label map (with its overlap
+// counters) whose boundary pixels, every pixel within one of a pixel a
+// snapped edge touches (the boundary rule of the error bounds), carry the
+// halo flag (kLabelNoneHalo outside every polygon), then its tile map. The
+// point pass counts the points of flagged pixels per pixel for the bounds
+// kernel and sums the rest per polygon like a raster pass.
+int rj_ctx::bounds_labels(rj_polyset* P, Canvas* C)
+{
+    const int tshift = raster_tshift();
+    if (C->blab_built && cache && C->blab_tshift == tshift) return RJ_OK;
+    if (P->ps.np >= (int)kHaloFlag32) return fail(RJ_ERR_LIMIT, "too many polygons for an error-bounds label map");
+    const DevCanvas cv = dev_canvas(C->plan);
+    const size_t npx = (size_t)cv.LW * cv.H;
+    const size_t wpr = ((size_t)cv.W + 31) / 32;
+    const int l16 = P->ps.np < (int)(kHaloFlag16 - 16) ? 1 : 0;  // 16-bit ids keep the flag bit free
+    int rc = alloc(C->d_blab, npx * (l16 ? 2 : 4) + 16, "error-bounds label map");
+    if (!rc) rc = alloc(C->d_bmask, wpr * cv.H * 4, "boundary mask");
+    if (rc) return rc;
+    if ((rc = build_labels(P, C, C->d_blab.p, l16, C->d_btiles, tshift, true))) return rc;
+    uint32_t* mask = C->d_bmask.as<uint32_t>();
+    void* lab = C->d_blab.p;
+    const int32_t* prev = P->d_prev.as<int32_t>();
+    const int64_t nv = P->ps.off[P->ps.np];
+    if ((rc = launch("fill_mask", [&] { launch_fill_u32(stream, mask, wpr * cv.H, 0); }))) return rc;
+    if ((rc = launch("touch", [&] { launch_touch(stream, C->X, C->Y, prev, nv, cv, mask); }))) return rc;
+    if ((rc = launch("halo_flags", [&] { launch_halo_flags(stream, mask, cv, lab, l16); }))) return rc;
+    uint32_t* t = C->d_btiles.as<uint32_t>();
+    if ((rc = launch("tile_labels", [&] { launch_tile_labels(stream, lab, l16, cv, tshift, t, nullptr); })))
+        return rc;
+    C->blab_built = true;
+    C->blab16 = l16;
+    C->blab_tshift = tshift;
+    C->blab_state = 0;  // confirmed when the counters are read
     return RJ_OK;
 }
 
@@ -706,7 +756,10 @@ int64_t rj_ctx::estimate_bytes(const QuerySpec& s) const
         if (!C) b += (int64_t)s.P->ps.off[np] * 8 + (int64_t)np * 1024;
         if (s.exec == RJ_RASTER && (!C || !C->lab_built)) b += npx * lw + npx / 64;
         if (s.exec == RJ_HYBRID && (!C || C->hyb_res != s.index_res)) b += npx * lw + npx / 64 + npx / 8;
-        if (s.exec == RJ_ERRORBOUNDS) b += (int64_t)plan.W * plan.H * 4;
+        if (s.exec == RJ_ERRORBOUNDS) {
+            b += (int64_t)plan.W * plan.H * 4;  // count canvas
+            if (!C || !C->blab_built) b += npx * lw + npx / 64 + npx / 8;
+        }
     }
     return b;
 }
@@ -1111,6 +1164,7 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
     Canvas* C = nullptr;
     Grid* G = nullptr;
     const unsigned long long* check_aux = nullptr;  // overlap counters of a fresh label map
+    int* check_state = nullptr;                     // ... and the overlap state they settle
     const uint32_t* check_queue = nullptr;          // border-queue count of a hybrid pass
     out->path = RJ_PATH_SINGLE;
     last_exact = -1;
@@ -1133,7 +1187,22 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
         return launch("band_sum", [&] {
             launch_band(stream, BAND_SUM, C->bands_sum, (int)C->pr.bands_sum.size(), dp, cv, d_cnt.as<uint32_t>(),
                         d_chi.as<long long>(), d_clo.as<unsigned long long>(), res, nullptr, nullptr, 0, nullptr,
-                        C->pr.lds_sum, avg ? 1 : 0);
+                        C->pr.lds_sum, avg ? kBandSums : 0);
+        });
+    };
+
+    // the error-bounds kernel over the count canvas; counts: also the
+    // polygons' counts from the canvas (else a label pass wrote them)
+    auto bounds_pass = [&](bool counts) -> int {
+        const DevCanvas cv = dev_canvas(C->plan);
+        const DevPolys dp = dev_polys(P, C);
+        // narrow bands with their small LDS (more workgroups per CU), then the wide ones
+        const int ns = C->pr.nbounds_small, nall = (int)C->pr.bands_bounds.size();
+        return launch("band_bounds", [&] {
+            launch_band(stream, BAND_BOUNDS, C->bands_bounds, ns, dp, cv, d_cnt.as<uint32_t>(), nullptr, nullptr, res,
+                        bnd, nullptr, 0, nullptr, C->pr.lds_bounds_small, counts ? 0 : kBandNoCounts);
+            launch_band(stream, BAND_BOUNDS, C->bands_bounds + ns, nall - ns, dp, cv, d_cnt.as<uint32_t>(), nullptr,
+                        nullptr, res, bnd, nullptr, 0, nullptr, C->pr.lds_bounds, counts ? 0 : kBandNoCounts);
         });
     };
 
@@ -1202,22 +1271,44 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
             pa.tiles = C->d_tiles.as<uint32_t>();
             pa.tiles_w = cv.LW >> pa.tshift;
             if ((rc = point_pass("points_label", PT_LABEL, pa, t, rb0, re0))) return rc;
-            if (C->lab_state == 0) check_aux = tiles_aux(C->d_tiles, cv, C->lab_tshift);
+            if (C->lab_state == 0) {
+                check_aux = tiles_aux(C->d_tiles, cv, C->lab_tshift);
+                check_state = &C->lab_state;
+            }
         } else {
             if ((rc = canvas_join())) return rc;
         }
     } else {
         if ((rc = canvas_for(P, canvas_eps(s), s.ground_res, &C))) return rc;
+        if ((rc = bounds_for(P, C))) return rc;
         if (!C->pr.bounds_ok) return fail(RJ_ERR_UNSUPPORTED, "a polygon is too wide for the error-bounds kernel");
-        const DevCanvas cv = dev_canvas(C->plan);
-        const DevPolys dp = dev_polys(P, C);
-        if ((rc = canvas_pass(false))) return rc;
-        if ((rc = launch("band_bounds", [&] {
-                 launch_band(stream, BAND_BOUNDS, C->bands_bounds, (int)C->pr.bands_bounds.size(), dp, cv,
-                             d_cnt.as<uint32_t>(), nullptr, nullptr, res, bnd, nullptr, 0, nullptr,
-                             C->pr.lds_bounds, 0);
-             })))
-            return rc;
+        const bool known_overlap = cache && ((C->blab_built && C->blab_state == 2) || (C->lab_built && C->lab_state == 2));
+        if (raster_path == 2 && known_overlap)
+            return fail(RJ_ERR_UNSUPPORTED, "polygons overlap: the label-map path needs disjoint polygons");
+        if (raster_path != 1 && !known_overlap) {
+            // the error-bounds label map: one pass sums the polygons' counts
+            // and counts the points of boundary pixels per pixel
+            if ((rc = bounds_labels(P, C))) return rc;
+            if ((rc = ensure_canvas_buffers(C->plan, false))) return rc;
+            const DevCanvas cv = dev_canvas(C->plan);
+            pa.cv = cv;
+            pa.P = dev_polys(P, C);
+            pa.label = C->d_blab.p;
+            pa.label16 = C->blab16;
+            pa.tshift = C->blab_tshift;
+            pa.tiles = C->d_btiles.as<uint32_t>();
+            pa.tiles_w = cv.LW >> pa.tshift;
+            pa.cnt = d_cnt.as<uint32_t>();
+            if ((rc = point_pass("points_bounds", PT_LABEL_BOUNDS, pa, t, rb0, re0))) return rc;
+            if ((rc = bounds_pass(false))) return rc;
+            if (C->blab_state == 0) {
+                check_aux = tiles_aux(C->d_btiles, cv, C->blab_tshift);
+                check_state = &C->blab_state;
+            }
+        } else {
+            if ((rc = canvas_pass(false))) return rc;
+            if ((rc = bounds_pass(true))) return rc;
+        }
     }
 
     // one pinned copy of [results | bounds] (+ the overlap counters of a
@@ -1247,13 +1338,18 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
         if (e != hipSuccess) return hip_fail(e, "execute");
     }
     if (check_aux) {
-        C->lab_state = (h_res[res_words] != h_res[res_words + 1]) ? 2 : 1;
-        if (C->lab_state == 2) {
+        *check_state = (h_res[res_words] != h_res[res_words + 1]) ? 2 : 1;
+        if (*check_state == 2) {
             // the fresh label map has pixels in two polygons: redo on the canvas
             if (raster_path == 2)
                 return fail(RJ_ERR_UNSUPPORTED, "polygons overlap: the label-map path needs disjoint polygons");
             if ((rc = clear_results())) return rc;
-            if ((rc = canvas_join())) return rc;
+            if (fn == RJ_ERRORBOUNDS) {
+                if ((rc = canvas_pass(false))) return rc;
+                if ((rc = bounds_pass(true))) return rc;
+            } else if ((rc = canvas_join())) {
+                return rc;
+            }
             e = hipMemcpyAsync(h_res, res_base, res_words * 8, hipMemcpyDeviceToHost, stream);
             if (e == hipSuccess) e = hipStreamSynchronize(stream);
             if (e != hipSuccess) return hip_fail(e, "execute");

@@ -84,6 +84,16 @@ struct Canvas {
     double hyb_border = 0.125;  // share of a pass's rows last queued for the exact test
     float hyb_delta = 0;    // halo width in pixels
     DevBuf d_hlab, d_htiles, d_mask, d_rects;
+    // error-bounds label map: the raster label map with every boundary pixel
+    // (touched by an edge, dilated by one) flagged, for errorbounds without a
+    // canvas splat; blab_state as lab_state
+    bool blab_built = false;
+    int blab_state = 0;
+    int blab16 = 0;
+    int blab_tshift = kTileShiftDefault;
+    DevBuf d_blab, d_btiles, d_bmask;
+    bool bb_built = false;  // error-bounds bands planned and uploaded (bands_bounds)
+    DevBuf d_bb;
     uint64_t last_use = 0;
     size_t device_bytes() const;
 };
@@ -333,6 +343,8 @@ struct rj_ctx {
     // builds grid G of P at resolution res on the device (gridbuild.hip)
     int grid_device(rj_polyset* P, int res, rj::Grid* G);
     int raster_labels(rj_polyset* P, rj::Canvas* C, bool* fresh);
+    int bounds_labels(rj_polyset* P, rj::Canvas* C);
+    int bounds_for(rj_polyset* P, rj::Canvas* C);
     int hybrid_labels(rj_polyset* P, rj::Canvas* C, rj::Grid* G);
     int build_labels(rj_polyset* P, rj::Canvas* C, void* lab, int label16, rj::DevBuf& tiles, int tshift,
                      bool count);

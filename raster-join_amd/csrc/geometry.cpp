@@ -119,12 +119,10 @@ void rasterize_polys(const PolySet& ps, const CanvasPlan& plan, PolyRaster* out)
     out->region.assign(4 * (size_t)ps.np, 0);
     out->bands_sum.clear();
     out->bands_bounds.clear();
-    out->lds_sum = out->lds_bounds = 0;
+    out->lds_sum = out->lds_bounds = out->lds_bounds_small = 0;
+    out->nbounds_small = 0;
     out->bounds_ok = true;
     const int sum_words = 12288;    // LDS words per band, sum / label kernels
-    // LDS words per band of the error-bounds kernel: bands aim at 8192 words
-    // (eight workgroups per CU), a wide region takes up to 16384 per row
-    const int bound_words = 16384, bound_target = 4096;
     for (int p = 0; p < ps.np; p++) {
         const int64_t mnx = snap_x(plan, ps.pbox[4 * p]), mny = snap_y(plan, ps.pbox[4 * p + 1]);
         const int64_t mxx = snap_x(plan, ps.pbox[4 * p + 2]), mxy = snap_y(plan, ps.pbox[4 * p + 3]);
@@ -156,28 +154,54 @@ void rasterize_polys(const PolySet& ps, const CanvasPlan& plan, PolyRaster* out)
                 out->lds_sum = std::max(out->lds_sum, band_lds_bytes(BAND_SUM, b.nrows, wpr));
             }
         }
-        // error-bounds bands span the full region width
-        {
-            const int wpr = (bw + 31) / 32;
-            const int fit = (bound_words - 2 * wpr - kBandScan) / (34 * wpr);
-            const int rows = std::min({64, fit, std::max(1, (bound_target - 2 * wpr - kBandScan) / (34 * wpr))});
-            if (rows < 1) {
-                out->bounds_ok = false;
-                continue;
-            }
-            for (int64_t r = r0; r <= r1; r += rows) {
-                Band b;
-                b.poly = p;
-                b.r0 = (int32_t)r;
-                b.nrows = (int32_t)std::min<int64_t>(rows, r1 - r + 1);
-                b.wpr = wpr;
-                b.c0 = (int32_t)c0;
-                b.c1 = (int32_t)c1;
+    }
+}
+
+void bounds_bands(const PolySet& ps, PolyRaster* out)
+{
+    out->bands_bounds.clear();
+    out->lds_bounds = out->lds_bounds_small = 0;
+    out->nbounds_small = 0;
+    out->bounds_ok = true;
+    // LDS words per band: bands aim at kBoundsSmallWords, so that eight
+    // workgroups share a CU (at most 64 rows; 1.19 -> 0.53 ms for 260
+    // polygons at 7072^2, although a wide region then takes bands of few
+    // rows, each walking all of its polygon's edges). A region wider than
+    // one row of that takes up to 16384 words, in a launch of its own.
+    const int bound_words = 16384;
+    std::vector<Band> wide;
+    for (int p = 0; p < ps.np; p++) {
+        const int32_t* reg = &out->region[4 * (size_t)p];
+        const int64_t c0 = reg[0], c1 = reg[1], r0 = reg[2], r1 = reg[3];
+        if (c1 < c0 || r1 < r0) continue;
+        const int bw = (int)(c1 - c0 + 1);
+        const int wpr = (bw + 31) / 32;
+        const int fit = (bound_words - 2 * wpr - kBandScan) / (34 * wpr);
+        const int rows = std::min({64, fit, std::max(1, (kBoundsSmallWords - 2 * wpr - kBandScan) / (34 * wpr))});
+        if (rows < 1) {
+            out->bounds_ok = false;
+            continue;
+        }
+        for (int64_t r = r0; r <= r1; r += rows) {
+            Band b;
+            b.poly = p;
+            b.r0 = (int32_t)r;
+            b.nrows = (int32_t)std::min<int64_t>(rows, r1 - r + 1);
+            b.wpr = wpr;
+            b.c0 = (int32_t)c0;
+            b.c1 = (int32_t)c1;
+            const size_t lds = band_lds_bytes(BAND_BOUNDS, b.nrows, wpr);
+            if (lds <= (size_t)kBoundsSmallWords * 4) {
                 out->bands_bounds.push_back(b);
-                out->lds_bounds = std::max(out->lds_bounds, band_lds_bytes(BAND_BOUNDS, b.nrows, wpr));
+                out->lds_bounds_small = std::max(out->lds_bounds_small, lds);
+            } else {
+                wide.push_back(b);
+                out->lds_bounds = std::max(out->lds_bounds, lds);
             }
         }
     }
+    out->nbounds_small = (int)out->bands_bounds.size();
+    out->bands_bounds.insert(out->bands_bounds.end(), wide.begin(), wide.end());
 }
 
 // ---------------------------------------------------------------------------

@@ -190,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_band(const Band* __restrict__ bands,
                                                  const unsigned long long* __restrict__ chi,
                                                  const unsigned long long* __restrict__ clo, DevResult res,
                                                  unsigned long long* __restrict__ bounds, void* label,
-                                                 int label16, unsigned long long* aux, int with_sums)
+                                                 int label16, unsigned long long* aux, int opt)
 {
     extern __shared__ uint32_t lds[];
     const Band bd = bands[blockIdx.x];
@@ -271,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void k_band(const Band* __restrict__ bands,
             if (MODE == BAND_SUM) {
                 if (inside) {
                     s_cnt += cnt[pix];
-                    if (with_sums) {
+                    if (opt & kBandSums) {
                         s_hi += chi[pix];
                         s_lo += clo[pix];
                     }
@@ -320,8 +320,10 @@ __global__ __launch_bounds__(kBlock) void k_band(const Band* __restrict__ bands,
         // total coverage; equals the labeled pixel count iff no pixel is in two polygons
         block_add4(s_cnt, 0, 0, 0, aux, nullptr, nullptr, nullptr);
     } else if (MODE == BAND_BOUNDS) {
-        block_add4(s_cnt, 0, 0, 0, res.cnt + p, nullptr, nullptr, nullptr);
-        __syncthreads();
+        if (!(opt & kBandNoCounts)) {
+            block_add4(s_cnt, 0, 0, 0, res.cnt + p, nullptr, nullptr, nullptr);
+            __syncthreads();
+        }
         block_add4(s_fn, s_fp, s_fn1, s_fp1, bounds + p, bounds + P.np + p, bounds + 2 * P.np + p,
                    bounds + 3 * P.np + p);
     }
@@ -493,6 +495,7 @@ __global__ __launch_bounds__(kBlock) void k_points(PointArgs a)
     Run run{-1, 0, 0, 0};
     point_rows<MODE, AVG, R, true, CMP>(a, b0, max(a.begin, b0), b1, A, run);
     if (MODE == PT_HYBRID_QUEUE) border_finish(a, run);
+    if (MODE == PT_LABEL_BOUNDS && run.bc) atomicAdd(&a.cnt[run.bp], run.bc);
     if (run.ck >= 0) run_flush<MODE, AVG>(a, A, run.ck, run.cc, run.ch, run.cl);
     if (MODE == PT_SPLAT) splat_flush(A, a.cnt, a.chi, a.clo, AVG);
     else acc_flush(A, AVG);
@@ -574,23 +577,23 @@ static void chunking(int64_t n, int cap, int* blocks, int64_t* per_block)
 void launch_band(hipStream_t s, int mode, const Band* bands, int nbands, const DevPolys& P,
                  const DevCanvas& c, const uint32_t* cnt, const long long* chi,
                  const unsigned long long* clo, DevResult res, long long* bounds, void* label,
-                 int label16, unsigned long long* aux, size_t lds_bytes, int with_sums)
+                 int label16, unsigned long long* aux, size_t lds_bytes, int opt)
 {
     if (nbands <= 0) return;
     const unsigned long long* hi = (const unsigned long long*)chi;
     unsigned long long* bd = (unsigned long long*)bounds;
     if (mode == BAND_SUM)
         hipLaunchKernelGGL(k_band<BAND_SUM>, dim3(nbands), dim3(kBlock), lds_bytes, s, bands, P, c, cnt,
-                           hi, clo, res, bd, label, label16, aux, with_sums);
+                           hi, clo, res, bd, label, label16, aux, opt);
     else if (mode == BAND_LABEL)
         hipLaunchKernelGGL(k_band<BAND_LABEL>, dim3(nbands), dim3(kBlock), lds_bytes, s, bands, P, c,
-                           cnt, hi, clo, res, bd, label, label16, aux, with_sums);
+                           cnt, hi, clo, res, bd, label, label16, aux, opt);
     else if (mode == BAND_LABEL_ATOMIC)
         hipLaunchKernelGGL(k_band<BAND_LABEL_ATOMIC>, dim3(nbands), dim3(kBlock), lds_bytes, s, bands, P, c,
-                           cnt, hi, clo, res, bd, label, label16, aux, with_sums);
+                           cnt, hi, clo, res, bd, label, label16, aux, opt);
     else
         hipLaunchKernelGGL(k_band<BAND_BOUNDS>, dim3(nbands), dim3(kBlock), lds_bytes, s, bands, P, c,
-                           cnt, hi, clo, res, bd, label, label16, aux, with_sums);
+                           cnt, hi, clo, res, bd, label, label16, aux, opt);
 }
 
 void launch_touch(hipStream_t s, const int32_t* X, const int32_t* Y, const int32_t* prev,
@@ -752,6 +755,7 @@ void launch_points(hipStream_t s, int mode, PointArgs a)
     switch (mode) {
     case PT_INDEX: if (avg) RJ_LAUNCH_PTS(PT_INDEX, true); else RJ_LAUNCH_PTS(PT_INDEX, false); break;
     case PT_LABEL: if (avg) RJ_LAUNCH_PTS(PT_LABEL, true); else RJ_LAUNCH_PTS(PT_LABEL, false); break;
+    case PT_LABEL_BOUNDS: RJ_LAUNCH_PTS(PT_LABEL_BOUNDS, false); break;  // counts only
     case PT_LABEL_HYBRID:
         if (avg) RJ_LAUNCH_PTS(PT_LABEL_HYBRID, true); else RJ_LAUNCH_PTS(PT_LABEL_HYBRID, false);
         break;

@@ -463,6 +463,8 @@ struct Run {
     uint32_t cc;
     unsigned long long ch, cl;
     int qh, qn;
+    int bp;        // PT_LABEL_BOUNDS: the lane's run of one boundary pixel
+    uint32_t bc;   // and its points (0: none)
 };
 
 template <int MODE, bool AVG, bool LDS = true>
@@ -923,7 +925,8 @@ __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, in
             px[r] = nx[r];
             py[r] = ny[r];
         }
-        constexpr bool late = MODE == PT_LABEL || MODE == PT_LABEL_HYBRID || MODE == PT_HYBRID_QUEUE;
+        constexpr bool late =
+            MODE == PT_LABEL || MODE == PT_LABEL_HYBRID || MODE == PT_HYBRID_QUEUE || MODE == PT_LABEL_BOUNDS;
         if (!late) load_xy<R>(a.xy, i0 + stride, hi, a.end, nx, ny);
         // interior: the workgroup's whole row group lies in [lo, hi) (and so
         // in the table): no per-row bounds
@@ -1012,6 +1015,24 @@ __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, in
                         const bool f = lab[r] == kLabelNoneHalo || (lab[r] < kLabelMaxId && (lab[r] & fb));
                         lab[r] = lab[r] == kLabelNoneHalo ? kLabelNone : (f ? lab[r] & ~fb : lab[r]);
                         if (f && halo_near(a, px[r], py[r], ixs[r], iys[r])) lab[r] = kLabelBorder;
+                    }
+                }
+                if (MODE == PT_LABEL_BOUNDS) {
+                    // flagged (boundary) pixels: their points are also counted
+                    // per pixel, in a run per lane
+                    const uint32_t fb = a.label16 ? kHaloFlag16 : kHaloFlag32;
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const bool bd = lab[r] == kLabelNoneHalo || (lab[r] < kLabelMaxId && (lab[r] & fb));
+                        lab[r] = lab[r] == kLabelNoneHalo ? kLabelNone : (bd ? lab[r] & ~fb : lab[r]);
+                        if (bd) {
+                            if (pix[r] != run.bp || run.bc == 0) {
+                                if (run.bc) atomicAdd(&a.cnt[run.bp], run.bc);
+                                run.bp = pix[r];
+                                run.bc = 0;
+                            }
+                            run.bc++;
+                        }
                     }
                 }
 #pragma unroll

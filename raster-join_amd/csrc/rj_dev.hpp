@@ -116,9 +116,18 @@ struct Band {
 };
 
 enum BandMode { BAND_SUM = 0, BAND_LABEL = 1, BAND_BOUNDS = 2, BAND_LABEL_ATOMIC = 3 };
+// Options of a band launch: BAND_SUM adds the canvas sums too (kBandSums);
+// BAND_BOUNDS leaves the polygons' counts to a label pass that wrote them
+// (kBandNoCounts).
+constexpr int kBandSums = 1, kBandNoCounts = 2;
+// Error-bounds bands of at most this many LDS words run in a launch of their own.
+constexpr int kBoundsSmallWords = 4096;
 // PT_HYBRID_QUEUE: the hybrid pass that appends its border points to a queue
-// for k_exact instead of testing them itself.
-enum PointMode { PT_INDEX = 0, PT_LABEL = 1, PT_LABEL_HYBRID = 2, PT_SPLAT = 3, PT_HYBRID_QUEUE = 4 };
+// for k_exact instead of testing them itself. PT_LABEL_BOUNDS: a count pass
+// over the error-bounds label map that also counts the points of its flagged
+// (boundary) pixels per pixel into the count canvas.
+enum PointMode { PT_INDEX = 0, PT_LABEL = 1, PT_LABEL_HYBRID = 2, PT_SPLAT = 3, PT_HYBRID_QUEUE = 4,
+                 PT_LABEL_BOUNDS = 5 };
 
 // Workgroups of a point pass when not set: spatially ordered tables take one
 // workgroup per kPassRowsPerWorkgroup rows within these bounds (short row
@@ -158,7 +167,7 @@ struct PointArgs {
     uint32_t qcap;
     DevResult res;
     int lds_np;           // LDS sums indexed by polygon when np <= lds_np, else hashed
-    uint32_t* cnt;        // canvas (PT_SPLAT)
+    uint32_t* cnt;        // count canvas (PT_SPLAT; PT_LABEL_BOUNDS: boundary pixels only)
     unsigned long long* chi;
     unsigned long long* clo;
     int cap;              // workgroups of the pass (0: from wg_rows)
@@ -252,7 +261,7 @@ void launch_snap(hipStream_t s, const float* xy, int64_t nv, double minx, double
 void launch_band(hipStream_t s, int mode, const Band* bands, int nbands, const DevPolys& P,
                  const DevCanvas& c, const uint32_t* cnt, const long long* chi,
                  const unsigned long long* clo, DevResult res, long long* bounds, void* label,
-                 int label16, unsigned long long* aux, size_t lds_bytes, int with_sums);
+                 int label16, unsigned long long* aux, size_t lds_bytes, int opt);
 // Tile map (tiles of 2^tshift pixels square, tshift 3..5); `labeled` (may be
 // null) receives the count of labeled pixels.
 void launch_tile_labels(hipStream_t s, const void* label, int label16, const DevCanvas& c, int tshift,

@@ -45,11 +45,18 @@ DevCanvas dev_canvas(const CanvasPlan& p);
 struct PolyRaster {
     std::vector<int32_t> region;      // c0, c1, r0, r1 per polygon (inclusive)
     std::vector<Band> bands_sum;      // BAND_SUM / BAND_LABEL
-    std::vector<Band> bands_bounds;   // BAND_BOUNDS (full region width per band)
-    size_t lds_sum = 0, lds_bounds = 0;
+    // BAND_BOUNDS (full region width per band): the bands of at most
+    // kBoundsSmallWords LDS words first (nbounds_small of them, launched with
+    // lds_bounds_small), then the wider ones (launched with lds_bounds)
+    std::vector<Band> bands_bounds;
+    int nbounds_small = 0;
+    size_t lds_sum = 0, lds_bounds = 0, lds_bounds_small = 0;
     bool bounds_ok = true;            // every region fits a BAND_BOUNDS band
 };
+// Regions and sum / label bands of every polygon on a canvas.
 void rasterize_polys(const PolySet& ps, const CanvasPlan& plan, PolyRaster* out);
+// The error-bounds bands (and bounds_ok) from the regions rasterize_polys set.
+void bounds_bands(const PolySet& ps, PolyRaster* out);
 // Snap one coordinate to the fixed-point canvas grid (round half to even).
 int64_t snap_x(const CanvasPlan& plan, float x);
 int64_t snap_y(const CanvasPlan& plan, float y);
