@@ -293,6 +293,8 @@ int rj_canvas_info(rj_ctx* ctx, int32_t* W, int32_t* H);
 
 /* Tuning and path selection (no reference counterpart).
  *   RJ_OPT_RASTER_PATH:     0 auto, 1 canvas splat + polygon scan, 2 polygon label map
+ *                           (raster and errorbounds joins; auto takes the label map
+ *                           unless the polygons overlap on the canvas)
  *   RJ_OPT_TIMING:          1 records device time per kernel of each execute with HIP events;
  *                           2 only the kernels that stream the table (point passes, fused
  *                           passes), totals kept over executes until the option is set again

@@ -85,11 +85,17 @@ def test_raster_100M_eps10_paths_additivity_bounds():
         h.bind_table(table, rb, re)
         parts.append(fixed(h, rj.RASTER))
     assert additive(parts, full)
-    # error bounds contain the exact (hybrid) count; errorbounds counts = raster counts
+    # error bounds contain the exact (hybrid) count; errorbounds counts = raster
+    # counts; the label-map and canvas forms of errorbounds agree bit for bit
     h.bind_table(table)
     h.set_aggregation(rj.COUNT, -1)
+    h.set_option(rj.OPT_RASTER_PATH, 1)
+    c1, _ = h.execute_function(rj.ERRORBOUNDS)
+    b1 = h.get_error_bounds()
+    h.set_option(rj.OPT_RASTER_PATH, 0)
     c, _ = h.execute_function(rj.ERRORBOUNDS)
     b = h.get_error_bounds()
+    assert np.array_equal(c, c1) and np.array_equal(b, b1)
     assert np.array_equal(c, full[0])
     exact, _ = h.execute_function(rj.HYBRID)
     assert np.all(c - b[3 * np_:] <= exact) and np.all(exact <= c + b[2 * np_:3 * np_])
