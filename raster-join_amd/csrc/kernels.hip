@@ -289,25 +289,25 @@ __global__ __launch_bounds__(kBlock) void k_band(const Band* __restrict__ bands,
                     if (old != kLabelNone) atomicExch(&l32[lrow + col], kLabelBorder);
                 }
             } else {
-                const uint32_t val = cnt[pix];
+                // boundary pixel: a touched pixel within one row and column
+                const int w = col >> 5;
+                const uint32_t d = tch[row * wpr + w] | tch[(row + 1) * wpr + w] | tch[(row + 2) * wpr + w];
+                const bool bd = (d >> (col & 31)) & 1u;
+                // with the counts from a label pass only boundary pixels are read
+                const uint32_t val = (bd || (inside && !(opt & kBandNoCounts))) ? cnt[pix] : 0u;
                 if (inside) s_cnt += val;
-                if (val != 0) {
-                    // boundary pixel: a touched pixel within one row and column
-                    const int w = col >> 5;
-                    const uint32_t d = tch[row * wpr + w] | tch[(row + 1) * wpr + w] | tch[(row + 2) * wpr + w];
-                    if ((d >> (col & 31)) & 1u) {
-                        // min(share, 1 - share) of the pixel's area times its count, rounded up
-                        const uint32_t a = area_share(area[row * astride + col]);
-                        const uint32_t m = min(a, (uint32_t)kAreaOne - a);
-                        const unsigned long long add =
-                            ((unsigned long long)m * val + (unsigned long long)(kAreaOne - 1)) >> 24;
-                        if (inside) {
-                            s_fp += add;
-                            s_fp1 += val;
-                        } else {
-                            s_fn += add;
-                            s_fn1 += val;
-                        }
+                if (bd && val != 0) {
+                    // min(share, 1 - share) of the pixel's area times its count, rounded up
+                    const uint32_t a = area_share(area[row * astride + col]);
+                    const uint32_t m = min(a, (uint32_t)kAreaOne - a);
+                    const unsigned long long add =
+                        ((unsigned long long)m * val + (unsigned long long)(kAreaOne - 1)) >> 24;
+                    if (inside) {
+                        s_fp += add;
+                        s_fp1 += val;
+                    } else {
+                        s_fn += add;
+                        s_fn1 += val;
                     }
                 }
             }
