@@ -53,6 +53,7 @@ __device__ __forceinline__ void group_keys(const PointArgs& a, int64_t r0, int64
 {
     const int64_t stride = (int64_t)blockDim.x * R;
     const int64_t start = r0 - r0 % R;
+    const bool all_in = rows_inside(a.cols, a.cv);
     float nx[R], ny[R];
     load_xy<R>(a.xy, start + (int64_t)threadIdx.x * R, r1, a.end, nx, ny);
     for (int64_t base = start; base < r1; base += stride) {
@@ -65,11 +66,10 @@ __device__ __forceinline__ void group_keys(const PointArgs& a, int64_t r0, int64
         }
         if (i0 >= r1) continue;
         int pix[R], ixs[R], iys[R];
+        bool ok[R];
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-            pix[r] = pixel_xy(a.cv, px[r], py[r], ixs[r], iys[r]);
-            if (i0 + r < r0 || i0 + r >= r1) pix[r] = -1;
-        }
+        for (int r = 0; r < R; r++) ok[r] = i0 + r >= r0 && i0 + r < r1;
+        row_pixels<R>(a.cv, all_in, px, py, ok, pix, ixs, iys);
         uint32_t lab[R];
         row_labels<HYB, R>(a, px, py, pix, ixs, iys, lab, [&] { load_xy<R>(a.xy, i0 + stride, r1, a.end, nx, ny); });
 #pragma unroll

@@ -56,22 +56,53 @@ int table_fill(rj_ctx* ctx, rj_table* t, int64_t n, const float* xy, int32_t nco
     return RJ_OK;
 }
 
-// Bounds of the finite points (0..1 when there are none).
+// Bounds of the finite points (0..1 when there are none) and whether some
+// point is not finite; large tables on 8 host threads.
 void table_bbox(rj_table* t, int64_t n, const float* xy)
 {
+    struct Part {
+        float mn[2], mx[2];
+        bool any = false, bad = false;
+    };
+    const int nth = n >= (1 << 22) ? 8 : 1;
+    std::vector<Part> part(nth);
+    auto run = [&](int k) {
+        Part& P = part[k];
+        for (int64_t i = n * k / nth; i < n * (k + 1) / nth; i++) {
+            const float x = xy[2 * i], y = xy[2 * i + 1];
+            if (!std::isfinite(x) || !std::isfinite(y)) {
+                P.bad = true;
+                continue;
+            }
+            if (!P.any) {
+                P.mn[0] = P.mx[0] = x; P.mn[1] = P.mx[1] = y;
+                P.any = true;
+            }
+            P.mn[0] = std::min(P.mn[0], x); P.mx[0] = std::max(P.mx[0], x);
+            P.mn[1] = std::min(P.mn[1], y); P.mx[1] = std::max(P.mx[1], y);
+        }
+    };
+    if (nth == 1) {
+        run(0);
+    } else {
+        std::vector<std::thread> th;
+        for (int k = 0; k < nth; k++) th.emplace_back(run, k);
+        for (auto& x : th) x.join();
+    }
     float mn[2] = {0, 0}, mx[2] = {1, 1};
-    bool first = true;
-    for (int64_t i = 0; i < n; i++) {
-        float x = xy[2 * i], y = xy[2 * i + 1];
-        if (!std::isfinite(x) || !std::isfinite(y)) continue;
+    bool first = true, bad = false;
+    for (const Part& P : part) {
+        bad = bad || P.bad;
+        if (!P.any) continue;
         if (first) {
-            mn[0] = mx[0] = x; mn[1] = mx[1] = y;
+            mn[0] = P.mn[0]; mn[1] = P.mn[1]; mx[0] = P.mx[0]; mx[1] = P.mx[1];
             first = false;
         }
-        mn[0] = std::min(mn[0], x); mx[0] = std::max(mx[0], x);
-        mn[1] = std::min(mn[1], y); mx[1] = std::max(mx[1], y);
+        mn[0] = std::min(mn[0], P.mn[0]); mx[0] = std::max(mx[0], P.mx[0]);
+        mn[1] = std::min(mn[1], P.mn[1]); mx[1] = std::max(mx[1], P.mx[1]);
     }
     t->bbox[0] = mn[0]; t->bbox[1] = mn[1]; t->bbox[2] = mx[0]; t->bbox[3] = mx[1];
+    t->nonfinite = bad;
 }
 
 // A host-resident table: the rows are copied into one host allocation

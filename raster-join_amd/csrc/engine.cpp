@@ -776,6 +776,8 @@ DevCols rj_ctx::dev_cols(const rj_table* t) const
         c.col[k] = t->cols[k];
         c.type[k] = t->types[k];
     }
+    for (int k = 0; k < 4; k++) c.box[k] = t->bbox[k];
+    c.box_ok = (t->n > 0 && !t->nonfinite) ? 1 : 0;
     return c;
 }
 

@@ -160,7 +160,8 @@ struct rj_table {
     int32_t types[RJ_MAX_COLUMNS] = {0};
     void* xy = nullptr;
     void* cols[RJ_MAX_COLUMNS] = {nullptr};
-    float bbox[4] = {0, 0, 0, 0};
+    float bbox[4] = {0, 0, 0, 0};  // bounds of the finite locations
+    bool nonfinite = false;         // some location is not finite
     bool organized = false;
     // time buckets: rows are ordered by (bucket, Morton key); bucket b holds
     // rows [bucket_off[b], bucket_off[b+1]). bucket(t) = (t - t0) / span,

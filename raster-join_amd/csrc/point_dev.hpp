@@ -172,6 +172,49 @@ __device__ __forceinline__ int pixel_xy(const DevCanvas& c, float x, float y, in
     return iy * c.W + ix;
 }
 
+// The pixel formula for a point known to be inside the canvas: the same
+// operations as pixel_xy without its membership test.
+__device__ __forceinline__ int pixel_in(const DevCanvas& c, float x, float y, int& ix, int& iy)
+{
+    const float fx = (x - c.minx) * c.sx;
+    const float fy = (y - c.miny) * c.sy;
+    ix = min((int)fx, c.W - 1);
+    iy = min((int)fy, c.H - 1);
+    return iy * c.W + ix;
+}
+
+// Every location of the table lies inside the canvas (its bounds do, by the
+// comparisons of pixel_xy, and none is NaN or infinite): a pass then takes
+// pixel_in for every row (wave-uniform).
+__device__ __forceinline__ bool rows_inside(const DevCols& t, const DevCanvas& c)
+{
+    return t.box_ok && t.box[0] >= c.minx && t.box[2] < c.maxx && t.box[1] >= c.miny && t.box[3] < c.maxy;
+}
+
+// Pixels of R rows (rows not ok: -1).
+template <int R>
+__device__ __forceinline__ void row_pixels(const DevCanvas& c, bool inside, const float (&px)[R],
+                                           const float (&py)[R], const bool (&ok)[R], int (&pix)[R], int (&ixs)[R],
+                                           int (&iys)[R])
+{
+    if (inside) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            int ix, iy;
+            const int p = pixel_in(c, px[r], py[r], ix, iy);
+            pix[r] = ok[r] ? p : -1;
+            ixs[r] = ok[r] ? ix : 0;
+            iys[r] = ok[r] ? iy : 0;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            pix[r] = pixel_xy(c, px[r], py[r], ixs[r], iys[r]);
+            if (!ok[r]) pix[r] = -1;
+        }
+    }
+}
+
 // CPUJoin getXIndex/getYIndex (GridIndexF.cpp:39-47) with the in-grid test of
 // getRegion (:163); (int) truncation done only on in-range values. row
 // receives the cell's grid row.
@@ -796,6 +839,7 @@ __device__ __forceinline__ void rows_compact(const PointArgs& a, int64_t start, 
     const int agg_t = AVG ? a.cols.type[q.slot_col[q.agg_slot < 0 ? 0 : q.agg_slot]] : RJ_ATTR_UINT;
     const SlotPlan SP = make_plan(q, a.cols, AVG);
     const int64_t stride = (int64_t)blockDim.x * R;
+    const bool all_in = rows_inside(a.cols, a.cv);
     int head = 0, cnt = 0;  // the wave's queue: entries head .. head + cnt - 1 (mod kRowQueue)
     // the first n (<= K * 64) rows of the queue, K per lane (rows lane,
     // lane + 64, ...): their label-map gathers are in flight together
@@ -813,7 +857,15 @@ __device__ __forceinline__ void rows_compact(const PointArgs& a, int64_t start, 
             py[k] = in ? s_y[w][e] : 0.0f;
             v[k] = (AVG && in) ? s_v[w][e] : 0u;
             ixs[k] = iys[k] = 0;
-            pix[k] = in ? pixel_xy(a.cv, px[k], py[k], ixs[k], iys[k]) : -1;
+            if (all_in) {
+                int ix, iy;
+                const int p = pixel_in(a.cv, px[k], py[k], ix, iy);
+                pix[k] = in ? p : -1;
+                ixs[k] = in ? ix : 0;
+                iys[k] = in ? iy : 0;
+            } else {
+                pix[k] = in ? pixel_xy(a.cv, px[k], py[k], ixs[k], iys[k]) : -1;
+            }
         }
         head = (head + n) & kMask;
         cnt -= n;
@@ -915,6 +967,7 @@ __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, in
     const DevQuery& q = a.q;
     const int agg_t = AVG ? a.cols.type[q.slot_col[q.agg_slot < 0 ? 0 : q.agg_slot]] : RJ_ATTR_UINT;
     const int64_t stride = (int64_t)blockDim.x * R;
+    const bool all_in = MODE != PT_INDEX && rows_inside(a.cols, a.cv);
     float nx[R], ny[R];
     load_xy<R>(a.xy, start + (int64_t)threadIdx.x * R, hi, a.end, nx, ny);
     for (int64_t base = start; base < hi; base += stride) {
@@ -969,11 +1022,7 @@ __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, in
                 key[r] = ok[r] ? exact_first<AVG, LDS, 4>(a.g, a.P, A, px[r], py[r], agg_t, vraw[r]) : -1;
         } else {
             int pix[R], ixs[R], iys[R];
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                pix[r] = pixel_xy(a.cv, px[r], py[r], ixs[r], iys[r]);
-                if (!ok[r]) pix[r] = -1;
-            }
+            row_pixels<R>(a.cv, all_in, px, py, ok, pix, ixs, iys);
             if (MODE == PT_SPLAT) {
 #pragma unroll
                 for (int r = 0; r < R; r++) key[r] = pix[r];

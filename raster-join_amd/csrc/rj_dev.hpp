@@ -26,6 +26,10 @@ constexpr int kLabelRowAlign = 32;               // label-map rows are padded to
 struct DevCols {
     const void* col[RJ_MAX_COLUMNS];
     int32_t type[RJ_MAX_COLUMNS];
+    // bounds of the table's locations (minx, miny, maxx, maxy); box_ok: every
+    // location is finite and inside them
+    float box[4];
+    int32_t box_ok;
 };
 
 constexpr int kMaxSlots = 5;  // distinct columns one query may read (GLFunction.cpp:250-253)
