@@ -54,9 +54,13 @@ __device__ __forceinline__ void slot_filter(int type, uint32_t lo, uint32_t hi, 
 // [-2^-25, 0) the float difference v - floor(v) rounds to 1.0: such a v
 // counts as 0 (hi = floor(v) + 1, lo = 0). Otherwise the difference is at
 // most 1 - 2^-24, so lo <= 2^32 - 256 fits 32 bits.
+// The integer value is set first and the float branch overwrites it: hipcc
+// (ROCm 7.2, -O3) compiled the form if (Float) .. else if (Int) .. else
+// hi = value so that u8 columns (type 4) left hi unset.
 __device__ __forceinline__ void agg_bits(int type, uint32_t a, unsigned long long& hi, unsigned long long& lo)
 {
     lo = 0;
+    hi = type == RJ_ATTR_INT ? (unsigned long long)(long long)(int32_t)a : (unsigned long long)a;
     if (type == RJ_ATTR_FLOAT) {
         const float f = __uint_as_float(a);
         const float fl = floorf(f);
@@ -74,10 +78,6 @@ __device__ __forceinline__ void agg_bits(int type, uint32_t a, unsigned long lon
             return;
         }
         lo = (uint32_t)(fr * 4294967296.0f);
-    } else if (type == RJ_ATTR_INT) {
-        hi = (unsigned long long)(long long)(int32_t)a;
-    } else {
-        hi = a;
     }
 }
 

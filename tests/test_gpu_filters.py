@@ -137,3 +137,45 @@ def test_float_sums_of_special_values(handler, data):
         assert r.status == 0
         assert np.array_equal(r.counts, c)
         assert np.array_equal(r.hi, hi) and np.array_equal(r.lo, lo)
+
+
+@pytest.mark.parametrize("col", [0, 2, 3])
+def test_integer_sums_of_every_type(handler, data, col):
+    """sum over the Uint, Int and u8 columns (the integer extremes among the
+    values): the exact sums equal the oracle's on both index paths, the hybrid
+    join, the raster join and a fused batch of all four, with and without a
+    filter on the summed column. (u8 sums were garbage before: the device
+    compiler dropped the integer branch of the aggregate decode for type 4.)"""
+    pts, cols = data
+    xy, off = polys("voronoi20")
+    g = O.Grid(xy, off, 64)
+    p = O.plan(O.poly_bbox(xy), 20.0)
+    h = handler
+    h.set_polygons(xy, off)
+    h.set_points(pts, cols, TYPES)
+    h.set_aggregation(rj.SUM, col)
+    h.set_poly_index_resolution(64)
+    h.set_accuracy_distance(20.0)
+    filt = {0: (0, rj.LT, 0x80000000), 2: (2, rj.GTE, -77), 3: (3, rj.LTE, 200)}[col]
+    table = rj.Table(h, pts, cols, TYPES)
+    ps = rj.PolySet(h, xy, off)
+    for cons in ([], [filt]):
+        pset = O.PointSet(pts, cols, TYPES, cons, agg_type=rj.SUM, agg_col=col)
+        c, hi, lo, _ = O.join_exact(g, pset)
+        rc, rhi, rlo = O.raster_join(p, xy, off, pset)
+        h.set_query_constraints(cons)
+        for fn, path in ((rj.INDEX, 0), (rj.INDEX, 1), (rj.HYBRID, 0), (rj.RASTER, 0)):
+            h.set_option(rj.OPT_INDEX_PATH, path)
+            got, _ = h.execute_function(fn)
+            ghi, glo = h.sums_fixed()
+            want = (rc, rhi, rlo) if fn == rj.RASTER else (c, hi, lo)
+            assert np.array_equal(got, want[0]), (fn, path, cons)
+            assert np.array_equal(ghi, want[1]) and np.array_equal(glo, want[2]), (fn, path, cons)
+        h.set_option(rj.OPT_INDEX_PATH, 0)
+        fns = (rj.INDEX, rj.HYBRID, rj.RASTER, rj.RASTER)
+        qs = [rj.make_query(ps, fn, rj.SUM, col, cons, eps=20.0, index_res=64) for fn in fns]
+        for fn, r in zip(fns, h.execute_batch(table, qs, [ps.np] * len(qs))):
+            want = (rc, rhi, rlo) if fn == rj.RASTER else (c, hi, lo)
+            assert r.status == 0
+            assert np.array_equal(r.counts, want[0]), (fn, cons)
+            assert np.array_equal(r.hi, want[1]) and np.array_equal(r.lo, want[2]), (fn, cons)
