@@ -177,6 +177,54 @@ __device__ __forceinline__ void member_query(const PointArgs& a, int64_t r0, int
     acc_flush(A, AVG);
 }
 
+// The chunk passes every constraint of the member (its rows run as a query
+// without constraints).
+__device__ __forceinline__ bool chunk_plain(const PointArgs& a, int64_t r0, int64_t r1)
+{
+    return a.q.plain || (r0 >= a.in_b && r1 <= a.in_e);
+}
+
+// The aggregate column a member reads (-1: count only).
+__device__ __forceinline__ int agg_column(const PointArgs& a)
+{
+    return a.q.agg_type == RJ_AGG_COUNT ? -1 : a.q.slot_col[a.q.agg_slot];
+}
+
+// One pass of the chunk for the members in `share` (label-map members of one
+// group with per-polygon sums in LDS, for all of which the chunk passes every
+// constraint: their sums over it are the same), with the arguments of `lead`;
+// the sums go to every member's results, counts only to count members.
+template <bool AVG, int R>
+__device__ __forceinline__ void shared_query(ConstBlock* CB, int lead, unsigned long long share, int64_t r0,
+                                             int64_t r1, const uint16_t* skey, unsigned long long* sacc,
+                                             int lds_cap)
+{
+    const PointArgs& a = *(const PointArgs*)(&CB->q[lead]);
+    Acc A;
+    __syncthreads();  // the previous query's sums have been flushed
+    acc_init(A, sacc, a.P.np, lds_cap, AVG, a.res);
+    Run run{-1, 0, 0, 0};
+    member_rows<PT_LABEL, AVG, R>(a, r0, r1, skey, A, run);
+    if (run.ck >= 0) run_flush<PT_LABEL, AVG>(a, A, run.ck, run.cc, run.ch, run.cl);
+    __syncthreads();
+    for (int k = threadIdx.x; k < A.lds_np; k += blockDim.x) {
+        const unsigned long long c = A.cnt[k];
+        if (!c) continue;
+        unsigned long long mm = share;
+        while (mm) {
+            const int j = __ffsll((long long)mm) - 1;
+            mm &= mm - 1;
+            const PointArgs& aj = *(const PointArgs*)(&CB->q[j]);
+            const DevResult& g = aj.res;
+            atomicAdd(&g.cnt[k], c);
+            if (AVG && aj.q.agg_type != RJ_AGG_COUNT) {
+                atomicAdd((unsigned long long*)&g.hi[k], A.hi[k]);
+                atomicAdd(&g.lo[k], A.lo[k]);
+            }
+        }
+    }
+}
+
 template <int R, int CLS>
 __global__ __launch_bounds__(kBlock) void k_batch(const BatchBlock* __restrict__ B, uint32_t* __restrict__ counters)
 {
@@ -224,6 +272,36 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchBlock* __restrict__
                     __syncthreads();  // the previous group's members have read their keys
                     group_keys<CLS == BC_HYBRID, R>(*(const PointArgs*)(&CB->q[q]), r0, r1, skey);
                     unsigned long long mm = gm;
+                    const PointArgs& aq = *(const PointArgs*)(&CB->q[q]);
+                    if (CLS == BC_LABEL && aq.P.np <= lds_cap) {
+                        // members the chunk passes every constraint of share one pass: with
+                        // the aggregate column of the first such avg member (count members
+                        // take its counts), else counts only
+                        unsigned long long plain = 0;
+                        int col = -1, lead = -1;
+                        for (unsigned long long t = gm; t; t &= t - 1) {
+                            const int j = __ffsll((long long)t) - 1;
+                            const PointArgs& aj = *(const PointArgs*)(&CB->q[j]);
+                            if (!chunk_plain(aj, r0, r1)) continue;
+                            plain |= 1ull << j;
+                            if (col < 0 && agg_column(aj) >= 0) {
+                                col = agg_column(aj);
+                                lead = j;
+                            }
+                        }
+                        unsigned long long share = 0;
+                        for (unsigned long long t = plain; t; t &= t - 1) {
+                            const int j = __ffsll((long long)t) - 1;
+                            const int cj = agg_column(*(const PointArgs*)(&CB->q[j]));
+                            if (cj < 0 || cj == col) share |= 1ull << j;
+                        }
+                        if (lead < 0 && share) lead = __ffsll((long long)share) - 1;
+                        if (__popcll(share) >= 2) {
+                            mm &= ~share;
+                            if (col >= 0) shared_query<true, R>(CB, lead, share, r0, r1, skey, sacc, lds_cap);
+                            else shared_query<false, R>(CB, lead, share, r0, r1, skey, sacc, lds_cap);
+                        }
+                    }
                     while (mm) {
                         const int j = __ffsll((long long)mm) - 1;
                         mm &= mm - 1;

@@ -331,3 +331,39 @@ def test_groups_on_one_label_map_equal_single(handler, data, share):
     for i, (q, k, b) in enumerate(zip(qs, nps, batch)):
         assert b.status == 0 and b.path == rj.PATH_FUSED, i
         assert same(h.execute_query(tb, q, k), b), i
+
+
+@pytest.mark.parametrize("share", [1, 0])
+def test_plain_members_of_a_group_equal_single(handler, data, share):
+    """Members of one label-map group whose constraints a chunk passes in full
+    (no constraints, or a window covering the chunk) take one pass of it
+    together: counts go to every member, sums only to the members that
+    aggregate the same column. Count, avg and sum over two columns, with and
+    without windows and a filter: results equal each query run alone."""
+    h = handler
+    tb = bucketed(h, data)
+    ps = polysets(h, ("voronoi260",))[0]
+    rng = np.random.default_rng(31)
+    aggs = [(rj.COUNT, -1), (rj.AVG, 1), (rj.SUM, 1), (rj.AVG, 2), (rj.SUM, 2), (rj.COUNT, -1)]
+    qs = []
+    for i in range(24):
+        agg, col = aggs[i % len(aggs)]
+        if i % 4 == 0:
+            cons = []
+        elif i % 4 == 1:
+            cons = window(0.1, YEAR // 2)
+        elif i % 4 == 2:
+            length = int(np.exp(rng.uniform(np.log(86400), np.log(YEAR))))
+            cons = window(rng.uniform(0, 1 - length / YEAR), length)
+        else:
+            cons = [(2, rj.LTE, int(rng.integers(1, 7)))]
+        qs.append(rj.make_query(ps, rj.RASTER, agg, col, cons, eps=20.0))
+    nps = [ps.np] * len(qs)
+    h.set_option(rj.OPT_BATCH_SHARE, share)
+    try:
+        batch = h.execute_batch(tb, qs, nps)
+    finally:
+        h.set_option(rj.OPT_BATCH_SHARE, 1)
+    for i, (q, b) in enumerate(zip(qs, batch)):
+        assert b.status == 0 and b.path == rj.PATH_FUSED, i
+        assert same(h.execute_query(tb, q, ps.np), b), i
