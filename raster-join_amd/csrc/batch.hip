@@ -87,7 +87,7 @@ __device__ __forceinline__ void member_rows(const PointArgs& a, int64_t r0, int6
                                             const Acc& A, Run& run)
 {
     const DevQuery& q = a.q;
-    const int agg_t = AVG ? a.cols.type[q.slot_col[q.agg_slot < 0 ? 0 : q.agg_slot]] : RJ_ATTR_UINT;
+    const int agg_t = AVG ? agg_kind(a.cols, q) : kAggUnsigned;
     const int64_t stride = (int64_t)blockDim.x * R;
     // a chunk inside the rows that pass every constraint skips the filters
     const bool plain = q.plain || (r0 >= a.in_b && r1 <= a.in_e);
@@ -343,7 +343,7 @@ __device__ __forceinline__ void exact_tile(const PointArgs& a, uint32_t i0, uint
     __syncthreads();  // the previous item's sums have been flushed
     acc_init(A, sacc, a.P.np, lds_cap, AVG, a.res);
     const DevQuery& q = a.q;
-    const int agg_t = AVG ? a.cols.type[q.slot_col[q.agg_slot < 0 ? 0 : q.agg_slot]] : RJ_ATTR_UINT;
+    const int agg_t = AVG ? agg_kind(a.cols, q) : kAggUnsigned;
     for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         const float4 e = a.queue[i];
         const uint32_t v = __float_as_uint(e.z);

@@ -16,6 +16,8 @@
 
 namespace rj {
 size_t organize_scratch_bytes(int64_t n);
+int float_columns_in_range(hipStream_t s, int64_t n, void* const* cols, const int32_t* types, int ncols,
+                           uint32_t* flags, uint32_t* mask);
 int organize_rows(hipStream_t s, int64_t n, float* xy, void** cols, const int32_t* types, int ncols,
                   const float bbox[4], const uint32_t* time, uint32_t t0, uint32_t span, uint32_t nb,
                   int64_t* off_dev, void* scratch);
@@ -53,6 +55,16 @@ int table_fill(rj_ctx* ctx, rj_table* t, int64_t n, const float* xy, int32_t nco
     }
     if (e != hipSuccess) return ctx->hip_fail(e, "point table");
     table_bbox(t, n, xy);
+    bool any_float = false;
+    for (int c = 0; c < ncols; c++) any_float = any_float || types[c] == RJ_ATTR_FLOAT;
+    if (any_float) {
+        uint32_t* flags = nullptr;
+        e = hipMalloc((void**)&flags, (size_t)ncols * 4);
+        if (e != hipSuccess) return ctx->hip_fail(e, "column ranges");
+        const int r = float_columns_in_range(ctx->stream, n, t->cols, t->types, ncols, flags, &t->float_in_range);
+        (void)hipFree(flags);
+        if (r) return ctx->hip_fail((hipError_t)r, "column ranges");
+    }
     return RJ_OK;
 }
 

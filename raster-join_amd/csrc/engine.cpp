@@ -778,6 +778,7 @@ DevCols rj_ctx::dev_cols(const rj_table* t) const
     }
     for (int k = 0; k < 4; k++) c.box[k] = t->bbox[k];
     c.box_ok = (t->n > 0 && !t->nonfinite) ? 1 : 0;
+    c.in_range = t->float_in_range;
     return c;
 }
 

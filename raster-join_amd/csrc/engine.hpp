@@ -162,6 +162,7 @@ struct rj_table {
     void* cols[RJ_MAX_COLUMNS] = {nullptr};
     float bbox[4] = {0, 0, 0, 0};  // bounds of the finite locations
     bool nonfinite = false;         // some location is not finite
+    uint32_t float_in_range = 0;    // bit c: float column c is finite and in [-2^31, 2^31)
     bool organized = false;
     // time buckets: rows are ordered by (bucket, Morton key); bucket b holds
     // rows [bucket_off[b], bucket_off[b+1]). bucket(t) = (t - t0) / span,

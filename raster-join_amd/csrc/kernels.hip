@@ -514,7 +514,7 @@ __global__ __launch_bounds__(kBlock) void k_exact(PointArgs a)
     Acc A;
     acc_init(A, sacc, a.P.np, a.lds_np, AVG, a.res);
     const DevQuery& q = a.q;
-    const int agg_t = AVG ? a.cols.type[q.slot_col[q.agg_slot < 0 ? 0 : q.agg_slot]] : RJ_ATTR_UINT;
+    const int agg_t = AVG ? agg_kind(a.cols, q) : kAggUnsigned;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float4 e = a.queue[i];
         const uint32_t v = __float_as_uint(e.z);

@@ -85,7 +85,48 @@ int bits_for(uint32_t v)
     return b;
 }
 
+// flag = 1 when some value of the float column is not finite or lies outside
+// [-2^31, 2^31).
+__global__ void k_float_range(const float* __restrict__ v, int64_t n, uint32_t* flag)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    bool bad = false;
+    const int64_t n4 = n / 4;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+        const float4 f = reinterpret_cast<const float4*>(v)[i];
+        bad |= !(f.x >= -2147483648.0f && f.x < 2147483648.0f) | !(f.y >= -2147483648.0f && f.y < 2147483648.0f) |
+               !(f.z >= -2147483648.0f && f.z < 2147483648.0f) | !(f.w >= -2147483648.0f && f.w < 2147483648.0f);
+    }
+    for (int64_t i = 4 * n4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        bad |= !(v[i] >= -2147483648.0f && v[i] < 2147483648.0f);
+    if (__ballot(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
 }  // namespace
+
+// Bit c of *mask: float column c is finite and in [-2^31, 2^31) on every row
+// (its aggregate decode then needs no range tests). flags: ncols words of
+// device memory. Returns a hipError_t.
+int float_columns_in_range(hipStream_t s, int64_t n, void* const* cols, const int32_t* types, int ncols,
+                           uint32_t* flags, uint32_t* mask)
+{
+    *mask = 0;
+    hipError_t e = hipMemsetAsync(flags, 0, (size_t)ncols * 4, s);
+    for (int c = 0; c < ncols && e == hipSuccess; c++) {
+        if (types[c] != RJ_ATTR_FLOAT || n == 0) continue;
+        const int64_t want = (n / 4 + 255) / 256;
+        const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, 4096));
+        k_float_range<<<blocks, 256, 0, s>>>(static_cast<const float*>(cols[c]), n, flags + c);
+        e = hipGetLastError();
+    }
+    uint32_t h[RJ_MAX_COLUMNS] = {0};
+    if (e == hipSuccess && ncols > 0) e = hipMemcpyAsync(h, flags, (size_t)ncols * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return (int)e;
+    for (int c = 0; c < ncols; c++)
+        if (types[c] == RJ_ATTR_FLOAT && (n == 0 || !h[c])) *mask |= 1u << c;
+    return 0;
+}
 
 // Sorts the rows of xy (+ columns) by Morton key in place, or, with a time
 // column, by (time bucket, Morton key) and writes the nb+1 bucket offsets to

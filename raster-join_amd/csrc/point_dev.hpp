@@ -49,19 +49,42 @@ __device__ __forceinline__ void slot_filter(int type, uint32_t lo, uint32_t hi, 
     }
 }
 
+// How a pass decodes its aggregate column (wave-uniform): unsigned (Uint,
+// u8), signed (Int), float, or float known finite and in [-2^31, 2^31) on
+// every row of the table (no range tests).
+constexpr int kAggUnsigned = 0, kAggSigned = 1, kAggFloat = 2, kAggFloatInRange = 3;
+
+__device__ __forceinline__ int agg_kind(const DevCols& C, const DevQuery& q)
+{
+    const int col = q.slot_col[q.agg_slot < 0 ? 0 : q.agg_slot];
+    const int t = C.type[col];
+    if (t == RJ_ATTR_FLOAT) return ((C.in_range >> col) & 1u) ? kAggFloatInRange : kAggFloat;
+    return t == RJ_ATTR_INT ? kAggSigned : kAggUnsigned;
+}
+
 // Exact fixed-point aggregate value of a 32-bit pattern: hi = floor(v),
 // lo = frac(v) * 2^32 (integer columns: hi = value, lo = 0). For v in
 // [-2^-25, 0) the float difference v - floor(v) rounds to 1.0: such a v
 // counts as 0 (hi = floor(v) + 1, lo = 0). Otherwise the difference is at
-// most 1 - 2^-24, so lo <= 2^32 - 256 fits 32 bits.
-// The integer value is set first and the float branch overwrites it: hipcc
+// most 1 - 2^-24, so lo <= 2^32 - 256 fits 32 bits. Non-finite values and
+// |v| >= 2^62 count nothing; floor(v) outside 32 bits takes the 64-bit
+// conversion. For a column in range, floor(v) fits 32 bits and the
+// decode is the same without branches.
+// The integer value is set first and the float branches overwrite it: hipcc
 // (ROCm 7.2, -O3) compiled the form if (Float) .. else if (Int) .. else
-// hi = value so that u8 columns (type 4) left hi unset.
-__device__ __forceinline__ void agg_bits(int type, uint32_t a, unsigned long long& hi, unsigned long long& lo)
+// hi = value so that u8 columns left hi unset.
+__device__ __forceinline__ void agg_bits(int kind, uint32_t a, unsigned long long& hi, unsigned long long& lo)
 {
     lo = 0;
-    hi = type == RJ_ATTR_INT ? (unsigned long long)(long long)(int32_t)a : (unsigned long long)a;
-    if (type == RJ_ATTR_FLOAT) {
+    hi = kind == kAggSigned ? (unsigned long long)(long long)(int32_t)a : (unsigned long long)a;
+    if (kind == kAggFloatInRange) {
+        const float f = __uint_as_float(a);
+        const float fl = floorf(f);
+        const float fr = f - fl;
+        const bool one = fr >= 1.0f;
+        hi = (unsigned long long)(long long)((int32_t)fl + (one ? 1 : 0));
+        lo = one ? 0ull : (unsigned long long)(uint32_t)(fr * 4294967296.0f);
+    } else if (kind == kAggFloat) {
         const float f = __uint_as_float(a);
         const float fl = floorf(f);
         if (fabsf(fl) < 2147483648.0f) {
@@ -69,7 +92,7 @@ __device__ __forceinline__ void agg_bits(int type, uint32_t a, unsigned long lon
         } else if (fabsf(f) < 4.611686018427387904e18f) {
             hi = (unsigned long long)(long long)fl;
         } else {
-            hi = 0;  // non-finite or |v| >= 2^62 contributes nothing
+            hi = 0;
             return;
         }
         const float fr = f - fl;
@@ -836,7 +859,7 @@ __device__ __forceinline__ void rows_compact(const PointArgs& a, int64_t start, 
     const int lane = (int)__lane_id();
     const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const DevQuery& q = a.q;
-    const int agg_t = AVG ? a.cols.type[q.slot_col[q.agg_slot < 0 ? 0 : q.agg_slot]] : RJ_ATTR_UINT;
+    const int agg_t = AVG ? agg_kind(a.cols, q) : kAggUnsigned;
     const SlotPlan SP = make_plan(q, a.cols, AVG);
     const int64_t stride = (int64_t)blockDim.x * R;
     const bool all_in = rows_inside(a.cols, a.cv);
@@ -965,7 +988,7 @@ __device__ __forceinline__ void point_rows(const PointArgs& a, int64_t start, in
         return;
     }
     const DevQuery& q = a.q;
-    const int agg_t = AVG ? a.cols.type[q.slot_col[q.agg_slot < 0 ? 0 : q.agg_slot]] : RJ_ATTR_UINT;
+    const int agg_t = AVG ? agg_kind(a.cols, q) : kAggUnsigned;
     const int64_t stride = (int64_t)blockDim.x * R;
     const bool all_in = MODE != PT_INDEX && rows_inside(a.cols, a.cv);
     float nx[R], ny[R];

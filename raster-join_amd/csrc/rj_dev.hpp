@@ -30,6 +30,7 @@ struct DevCols {
     // location is finite and inside them
     float box[4];
     int32_t box_ok;
+    uint32_t in_range;  // bit c: float column c is finite and in [-2^31, 2^31) on every row
 };
 
 constexpr int kMaxSlots = 5;  // distinct columns one query may read (GLFunction.cpp:250-253)

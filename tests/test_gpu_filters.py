@@ -179,3 +179,62 @@ def test_integer_sums_of_every_type(handler, data, col):
             assert r.status == 0
             assert np.array_equal(r.counts, want[0]), (fn, cons)
             assert np.array_equal(r.hi, want[1]) and np.array_equal(r.lo, want[2]), (fn, cons)
+
+
+def test_float_sums_of_a_column_in_range(handler, data):
+    """A float column whose every value is finite and in [-2^31, 2^31) takes
+    the decode without range tests. Its edge values: tiny negatives that
+    count as 0 (-1e-45, -2^-26, -2^-25), signed zeros, denormals, -2^31
+    itself and the largest float below 2^31, fractions. Sums (hi, lo) equal
+    the oracle's on both index paths, the hybrid and raster joins and a fused
+    batch, and equal the same column's sums with one NaN added (the decode
+    with range tests)."""
+    pts, cols = data
+    rng = np.random.default_rng(9)
+    edge = np.array([-1e-45, -2.0 ** -26, -2.0 ** -25, -2.0 ** -24, 0.0, -0.0, 1e-45, 1.1754944e-38,
+                     -1.1754944e-38, -2147483648.0, 2147483520.0, -2147483520.0, 0.5, -0.5, 7.25,
+                     -7.75, 1e9 + 0.5, -1e9], np.float32)
+    f = rng.normal(0, 1e4, N).astype(np.float32)
+    pick = rng.random(N) < 0.4
+    f[pick] = rng.choice(edge, pick.sum())
+    with_nan = f.copy()
+    with_nan[int(np.flatnonzero(f == 0.0)[0])] = np.nan  # a row that adds 0 either way
+    xy, off = polys("voronoi20")
+    g = O.Grid(xy, off, 64)
+    p = O.plan(O.poly_bbox(xy), 20.0)
+    h = handler
+    ps = rj.PolySet(h, xy, off)
+    types = (rj.FLOAT, rj.UINT8)
+    got = {}
+    for name, col in (("in_range", f), ("nan", with_nan)):
+        tcols = (col, cols[3])
+        pset = O.PointSet(pts, tcols, types, [], agg_type=rj.AVG, agg_col=0)
+        c, hi, lo, _ = O.join_exact(g, pset)
+        rc, rhi, rlo = O.raster_join(p, xy, off, pset)
+        h.set_polygons(xy, off)
+        h.set_points(pts, tcols, types)
+        h.set_aggregation(rj.AVG, 0)
+        h.set_poly_index_resolution(64)
+        h.set_accuracy_distance(20.0)
+        h.set_query_constraints([])
+        for fn, path in ((rj.INDEX, 0), (rj.INDEX, 1), (rj.HYBRID, 0), (rj.RASTER, 0)):
+            h.set_option(rj.OPT_INDEX_PATH, path)
+            cnt, _ = h.execute_function(fn)
+            ghi, glo = h.sums_fixed()
+            want = (rc, rhi, rlo) if fn == rj.RASTER else (c, hi, lo)
+            assert np.array_equal(cnt, want[0]), (name, fn, path)
+            assert np.array_equal(ghi, want[1]) and np.array_equal(glo, want[2]), (name, fn, path)
+            got[(name, fn, path)] = (ghi, glo)
+        h.set_option(rj.OPT_INDEX_PATH, 0)
+        table = rj.Table(h, pts, tcols, types)
+        fns = (rj.INDEX, rj.HYBRID, rj.RASTER)
+        qs = [rj.make_query(ps, fn, rj.AVG, 0, [(1, rj.LTE, 200)] if k else [], eps=20.0, index_res=64)
+              for fn in fns for k in (0, 1)]
+        for q, r in zip(qs, h.execute_batch(table, qs, [ps.np] * len(qs))):
+            assert r.status == 0
+            assert np.array_equal(r.hi, h.execute_query(table, q, ps.np).hi)
+        table.close()
+    # the NaN replaced a zero and adds nothing: both decodes give the same sums
+    for fn, path in ((rj.INDEX, 0), (rj.HYBRID, 0), (rj.RASTER, 0)):
+        a, b = got[("in_range", fn, path)], got[("nan", fn, path)]
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (fn, path)
