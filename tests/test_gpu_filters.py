@@ -179,6 +179,17 @@ def test_integer_sums_of_every_type(handler, data, col):
             assert r.status == 0
             assert np.array_equal(r.counts, want[0]), (fn, cons)
             assert np.array_equal(r.hi, want[1]) and np.array_equal(r.lo, want[2]), (fn, cons)
+        # the same queries on a host (streamed) table and through the entry
+        # point without allocations
+        host = rj.Table(h, pts, cols, TYPES, host=True)
+        for fn, q in zip(fns[:3], qs[:3]):
+            want = (rc, rhi, rlo) if fn == rj.RASTER else (c, hi, lo)
+            alone = h.execute_query(table, q, ps.np)  # also prepares the structures
+            for r in (alone, h.execute_query(host, q, ps.np), h.execute_query_noalloc(table, q, ps.np)):
+                assert r.status == 0
+                assert np.array_equal(r.counts, want[0]), (fn, cons)
+                assert np.array_equal(r.hi, want[1]) and np.array_equal(r.lo, want[2]), (fn, cons)
+        host.close()
 
 
 def test_float_sums_of_a_column_in_range(handler, data):
