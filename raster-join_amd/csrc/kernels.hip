@@ -404,7 +404,7 @@ __global__ void k_tile_labels(const void* __restrict__ label, int label16, int L
                               uint32_t* __restrict__ tiles, unsigned long long* __restrict__ labeled)
 {
     constexpr int T = 1 << TS;
-    constexpr int KT = TS >= 5 ? 1 : (32 >> TS) / 2;  // tile rows per workgroup (16 pixel rows, at least one tile)
+    constexpr int KT = TS == 3 ? 4 : 1;  // tile rows per workgroup (8-px tiles: 1, 2 and 8 measured slower)
     const int per = label16 ? 8 : 4;  // labels per vector
     const int tpt = T / per;          // threads per tile
     const int nvec = LW / per;
@@ -630,7 +630,7 @@ void launch_tile_labels(hipStream_t s, const void* label, int label16, const Dev
 {
     const int TW = c.LW >> tshift, TH = (c.H + (1 << tshift) - 1) >> tshift;
     const int nvec = c.LW / (label16 ? 8 : 4);
-    const int kt = tshift >= 5 ? 1 : (32 >> tshift) / 2;  // tile rows per workgroup, as the kernel
+    const int kt = tshift == 3 ? 4 : 1;  // tile rows per workgroup, as the kernel
     const dim3 grid((unsigned)((nvec + kBlock - 1) / kBlock), (unsigned)((TH + kt - 1) / kt));
     if (tshift == 3)
         hipLaunchKernelGGL(k_tile_labels<3>, grid, dim3(kBlock), 0, s, label, label16, c.LW, c.H, TW, TH, tiles,
