@@ -89,30 +89,7 @@ __device__ __forceinline__ void edge_touch(int64_t ax, int64_t ay, int64_t bx, i
                                            int TNR, int C0, int BW, int WPR, int W, int H,
                                            uint32_t* buf)
 {
-    if (ay > by) {
-        int64_t t = ax; ax = bx; bx = t;
-        t = ay; ay = by; by = t;
-    }
-    int64_t jlo = ceil_div(ay, kSub) - 1, jhi = floor_div(by, kSub);
-    if (jlo < 0) jlo = 0;
-    if (jhi > H - 1) jhi = H - 1;
-    if (jlo < TR0) jlo = TR0;
-    if (jhi > TR0 + TNR - 1) jhi = TR0 + TNR - 1;
-    for (int64_t j = jlo; j <= jhi; j++) {
-        int64_t xlo, xhi;
-        if (ay == by) {
-            xlo = min(ax, bx);
-            xhi = max(ax, bx);
-        } else {
-            int64_t ylo = max(ay, j * kSub), yhi = min(by, (j + 1) * kSub);
-            int64_t den = by - ay;
-            int64_t n0 = ax * den + (bx - ax) * (ylo - ay);
-            int64_t n1 = ax * den + (bx - ax) * (yhi - ay);
-            int64_t f0 = floor_div(n0, den), f1 = floor_div(n1, den);
-            int64_t g0 = ceil_div(n0, den), g1 = ceil_div(n1, den);
-            xlo = min(f0, f1);
-            xhi = max(g0, g1);
-        }
+    edge_touch_rows(ax, ay, bx, by, TR0, TNR, H, [&](int64_t j, int64_t xlo, int64_t xhi) {
         int64_t ilo = ceil_div(xlo, kSub) - 1, ihi = floor_div(xhi, kSub);
         if (ilo < 0) ilo = 0;
         if (ihi > W - 1) ihi = W - 1;
@@ -122,7 +99,7 @@ __device__ __forceinline__ void edge_touch(int64_t ax, int64_t ay, int64_t bx, i
             int64_t l = i - C0;
             atomicOr(&buf[(j - TR0) * WPR + (l >> 5)], 1u << (l & 31));
         }
-    }
+    });
 }
 
 __device__ __forceinline__ void prefix_xor_row(uint32_t* w, int nw)
@@ -209,20 +186,53 @@ __global__ __launch_bounds__(kBlock) void k_band(const Band* __restrict__ bands,
 
     const int e0 = P.off[p], e1 = P.off[p + 1], n = e1 - e0;
     const int64_t band_ylo = (int64_t)(r0 - 1) * kSub, band_yhi = (int64_t)(r0 + nr + 1) * kSub;
-    for (int e = threadIdx.x; e < n; e += blockDim.x) {
-        const int i = e0 + e;
-        const int j = (e == 0) ? e1 - 1 : i - 1;
-        const int64_t ax = P.X[i], ay = P.Y[i], bx = P.X[j], by = P.Y[j];
-        if (max(ay, by) < band_ylo || min(ay, by) > band_yhi) continue;
-        edge_toggles(ax, ay, bx, by, kSub, r0, nr, c0, bw, wpr, tog);
-        if (MODE == BAND_BOUNDS) {
-            edge_area(ax, ay, bx, by, r0, nr, [&](int64_t row, int64_t c, uint32_t d) {
-                int64_t l = c - c0;
-                if (l < 0) l = 0;  // everything left of the band enters at its first column
-                if (l >= bw) return;
-                atomicAdd(&area[(row - r0) * astride + l], d);
-            });
-            edge_touch(ax, ay, bx, by, r0 - 1, nr + 2, c0, bw, wpr, cv.W, cv.H, tch);
+    if constexpr (MODE == BAND_BOUNDS) {
+        // (edge, row) items: the edges that meet the band are compacted in
+        // LDS and every one of their rows r0-1 .. r0+nr goes to a lane of its
+        // own (a band holds few edges; one lane per edge ran their rows one
+        // after the other). The list and its count use the scan totals'
+        // words, free until the prefix sums.
+        constexpr int kChunk = kBandScan - 1;
+        int* s_in = reinterpret_cast<int*>(scan);
+        int* s_nin = s_in + kChunk;
+        const int nrows = nr + 2;
+        for (int base = 0; base < n; base += kChunk) {
+            if (threadIdx.x == 0) *s_nin = 0;
+            __syncthreads();
+            const int e = base + (int)threadIdx.x;
+            if ((int)threadIdx.x < kChunk && e < n) {
+                const int i = e0 + e;
+                const int j = (e == 0) ? e1 - 1 : i - 1;
+                const int64_t ay = P.Y[i], by = P.Y[j];
+                if (!(max(ay, by) < band_ylo || min(ay, by) > band_yhi)) s_in[atomicAdd(s_nin, 1)] = e;
+            }
+            __syncthreads();
+            const int nitems = *s_nin * nrows;
+            for (int it = threadIdx.x; it < nitems; it += blockDim.x) {
+                const int ei = s_in[it / nrows], row = r0 - 1 + it % nrows;
+                const int i = e0 + ei;
+                const int j = (ei == 0) ? e1 - 1 : i - 1;
+                const int64_t ax = P.X[i], ay = P.Y[i], bx = P.X[j], by = P.Y[j];
+                if (row >= r0 && row < r0 + nr) {
+                    edge_toggles(ax, ay, bx, by, kSub, row, 1, c0, bw, wpr, tog + (row - r0) * wpr);
+                    edge_area(ax, ay, bx, by, row, 1, [&](int64_t rw, int64_t c, uint32_t d) {
+                        int64_t l = c - c0;
+                        if (l < 0) l = 0;  // everything left of the band enters at its first column
+                        if (l >= bw) return;
+                        atomicAdd(&area[(rw - r0) * astride + l], d);
+                    });
+                }
+                edge_touch(ax, ay, bx, by, row, 1, c0, bw, wpr, cv.W, cv.H, tch + (row - (r0 - 1)) * wpr);
+            }
+            __syncthreads();  // s_in takes the next edges
+        }
+    } else {
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            const int i = e0 + e;
+            const int j = (e == 0) ? e1 - 1 : i - 1;
+            const int64_t ax = P.X[i], ay = P.Y[i], bx = P.X[j], by = P.Y[j];
+            if (max(ay, by) < band_ylo || min(ay, by) > band_yhi) continue;
+            edge_toggles(ax, ay, bx, by, kSub, r0, nr, c0, bw, wpr, tog);
         }
     }
     __syncthreads();

@@ -126,20 +126,51 @@ RJ_HD void edge_area(int64_t ax, int64_t ay, int64_t bx, int64_t by, int R0, int
     int64_t jlo = sc_floor_div(ay, 256), jhi = sc_ceil_div(by, 256) - 1;
     if (jlo < R0) jlo = R0;
     if (jhi > R0 + NR - 1) jhi = R0 + NR - 1;
-    const int64_t den = by - ay, dx = bx - ax;
+    if (jlo > jhi) return;
+    const int64_t den = by - ay, dx = bx - ax, D = 256 * den;
     const double dden = (double)den;
+    // x at a row boundary y: num / den, as a double (one rounding) and as the
+    // pixel column floor(num / (256 den)) with its remainder. Between interior
+    // boundaries num grows by dx * 256: the column steps instead of dividing.
+    auto at = [&](int64_t y, int64_t& n, double& u, int64_t& c, int64_t& r) {
+        n = ax * den + dx * (y - ay);
+        u = (double)n / dden;
+        c = sc_floor_div(n, D);
+        r = n - c * D;
+    };
+    int64_t yl = jlo * 256 > ay ? jlo * 256 : ay;
+    int64_t n0, c0, r0;
+    double u0;
+    at(yl, n0, u0, c0, r0);
+    const int64_t s = dx * 256;
+    int64_t sq = 0, sr = 0;
+    if (jlo < jhi) {
+        sq = sc_floor_div(s, D);
+        sr = s - sq * D;
+    }
     for (int64_t row = jlo; row <= jhi; row++) {
-        const int64_t ylo = row * 256 > ay ? row * 256 : ay;
-        const int64_t yhi = row * 256 + 256 < by ? row * 256 + 256 : by;
-        const int64_t h = yhi - ylo;
+        int64_t yh, n1, c1, r1;
+        double u1;
+        if (row < jhi && row > jlo) {
+            yh = row * 256 + 256;
+            n1 = n0 + s;
+            u1 = (double)n1 / dden;
+            c1 = c0 + sq;
+            r1 = r0 + sr;
+            if (r1 >= D) {
+                r1 -= D;
+                c1++;
+            }
+        } else {
+            yh = row * 256 + 256 < by ? row * 256 + 256 : by;
+            at(yh, n1, u1, c1, r1);
+        }
+        const int64_t h = yh - yl;
         // x at the piece's ends: num / den exactly, one rounding each
-        const int64_t n0 = ax * den + dx * (ylo - ay), n1 = ax * den + dx * (yhi - ay);
-        const int64_t nlo = n0 < n1 ? n0 : n1, nhi = n0 < n1 ? n1 : n0;
-        const double u0 = (double)n0 / dden, u1 = (double)n1 / dden;
         const double lo = u0 < u1 ? u0 : u1, hi = u0 < u1 ? u1 : u0;
         const double mid = 0.5 * (u0 + u1), hd = (double)h;
         const double k = hi > lo ? hd / (2.0 * (hi - lo)) : 0.0;
-        const int64_t cmin = sc_floor_div(nlo, 256 * den), cmax = sc_floor_div(nhi, 256 * den);
+        const int64_t cmin = n0 < n1 ? c0 : c1, cmax = n0 < n1 ? c1 : c0;
         int64_t gprev = 0, fprev = 0;  // G at the column's left side, F of the previous column
         for (int64_t c = cmin; c <= cmax; c++) {
             const int64_t g = area_left((double)((c + 1) * 256), lo, hi, mid, hd, k);
@@ -149,6 +180,67 @@ RJ_HD void edge_area(int64_t ax, int64_t ay, int64_t bx, int64_t by, int R0, int
             gprev = g;
         }
         emit(row, cmax + 1, (uint32_t)(65536 * h - fprev) * sgn);
+        yl = yh;
+        n0 = n1;
+        u0 = u1;
+        c0 = c1;
+        r0 = r1;
+    }
+}
+
+// The x-extent of edge a-b in every pixel row whose closed square row it
+// meets (rows [R0, R0+NR) within [0, H)): emit(j, xlo, xhi) with xlo the
+// floor of the smallest and xhi the ceiling of the largest x of the edge's
+// piece in the row (subunits). Rows share their boundaries; between interior
+// boundaries the quotient steps instead of dividing.
+template <class F>
+RJ_HD void edge_touch_rows(int64_t ax, int64_t ay, int64_t bx, int64_t by, int R0, int NR, int H, F&& emit)
+{
+    if (ay > by) {
+        int64_t t = ax; ax = bx; bx = t;
+        t = ay; ay = by; by = t;
+    }
+    int64_t jlo = sc_ceil_div(ay, 256) - 1, jhi = sc_floor_div(by, 256);
+    if (jlo < 0) jlo = 0;
+    if (jhi > H - 1) jhi = H - 1;
+    if (jlo < R0) jlo = R0;
+    if (jhi > R0 + NR - 1) jhi = R0 + NR - 1;
+    if (jlo > jhi) return;
+    if (ay == by) {
+        const int64_t lo = ax < bx ? ax : bx, hi = ax < bx ? bx : ax;
+        for (int64_t j = jlo; j <= jhi; j++) emit(j, lo, hi);
+        return;
+    }
+    const int64_t den = by - ay, dx = bx - ax;
+    auto at = [&](int64_t y, int64_t& q, int64_t& r) {
+        const int64_t n = ax * den + dx * (y - ay);
+        q = sc_floor_div(n, den);
+        r = n - q * den;
+    };
+    int64_t ql, rl;  // x at the row's lower boundary: floor and remainder
+    at(jlo * 256 > ay ? jlo * 256 : ay, ql, rl);
+    const int64_t s = dx * 256;
+    int64_t sq = 0, sr = 0;
+    if (jlo < jhi) {
+        sq = sc_floor_div(s, den);
+        sr = s - sq * den;
+    }
+    for (int64_t j = jlo; j <= jhi; j++) {
+        int64_t qu, ru;
+        if (j < jhi && j > jlo) {
+            qu = ql + sq;
+            ru = rl + sr;
+            if (ru >= den) {
+                ru -= den;
+                qu++;
+            }
+        } else {
+            at((j + 1) * 256 < by ? (j + 1) * 256 : by, qu, ru);
+        }
+        const int64_t g0 = ql + (rl != 0 ? 1 : 0), g1 = qu + (ru != 0 ? 1 : 0);
+        emit(j, ql < qu ? ql : qu, g0 > g1 ? g0 : g1);
+        ql = qu;
+        rl = ru;
     }
 }
 
