@@ -268,6 +268,38 @@ __global__ __launch_bounds__(kBlock) void k_band(const Band* __restrict__ bands,
         block_add4(s_cnt, 0, 0, 0, aux, nullptr, nullptr, nullptr);
         return;
     }
+    if (MODE == BAND_BOUNDS && (opt & kBandNoCounts)) {
+        // only boundary pixels count: a thread per 32-pixel word of a row,
+        // over the word's boundary bits alone
+        for (int idx = threadIdx.x; idx < nr * wpr; idx += blockDim.x) {
+            const int row = idx / wpr, w = idx - row * wpr;
+            uint32_t d = tch[row * wpr + w] | tch[(row + 1) * wpr + w] | tch[(row + 2) * wpr + w];
+            if (w == wpr - 1 && (bw & 31)) d &= (1u << (bw & 31)) - 1u;  // columns past the band
+            const uint32_t in_w = tog[row * wpr + w];
+            const int64_t prow = (int64_t)(r0 + row) * cv.W + c0;
+            while (d) {
+                const int b = __ffs(d) - 1;
+                d &= d - 1;
+                const int col = 32 * w + b;
+                const uint32_t val = cnt[prow + col];
+                if (val == 0) continue;
+                const uint32_t a = area_share(area[row * astride + col]);
+                const uint32_t m = min(a, (uint32_t)kAreaOne - a);
+                const unsigned long long add =
+                    ((unsigned long long)m * val + (unsigned long long)(kAreaOne - 1)) >> 24;
+                if ((in_w >> b) & 1u) {
+                    s_fp += add;
+                    s_fp1 += val;
+                } else {
+                    s_fn += add;
+                    s_fn1 += val;
+                }
+            }
+        }
+        block_add4(s_fn, s_fp, s_fn1, s_fp1, bounds + p, bounds + P.np + p, bounds + 2 * P.np + p,
+                   bounds + 3 * P.np + p);
+        return;
+    }
     // Threads cover 2^lg columns of (256 >> lg) rows at a time: no division.
     const int lg = bw >= kBlock ? 8 : (32 - __clz(bw - 1));
     const int colstep = 1 << lg;
