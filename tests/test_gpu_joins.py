@@ -283,6 +283,32 @@ def test_errorbounds_wide_bands(handler, W):
     assert (ob[:2 * (len(off) - 1)] > 0).all()
 
 
+def test_errorbounds_band_with_many_edges(handler):
+    """A comb whose 800 teeth edges all cross the same few pixel rows: the
+    error-bounds bands there hold more edges than one pass of a workgroup
+    lists (255), so their (edge, row) items come in several rounds; equal to
+    the CPU emulation."""
+    teeth = 400
+    xs = np.linspace(1000.0, 0.0, 2 * teeth + 1)
+    top = np.stack([xs, 500.0 + 4.0 * (np.arange(2 * teeth + 1) % 2)], axis=1)
+    comb = np.concatenate([np.array([[0.0, 0.0], [1000.0, 0.0]]), top]).astype(np.float32).ravel()
+    square = np.array([200, 600, 800, 600, 800, 900, 200, 900], np.float32)
+    xy = np.concatenate([comb, square])
+    off = np.array([0, len(comb) // 2, len(comb) // 2 + 4], np.int32)
+    rng = np.random.default_rng(79)
+    pts = rng.uniform(0, 1000, 2 * 1_000_000).astype(np.float32)
+    bbox = O.poly_bbox(xy).astype(np.float64)
+    eps = float(max(bbox[2] - bbox[0], bbox[3] - bbox[1]) / 1000 * np.sqrt(2))
+    p = O.plan(bbox, eps)
+    oc, _, _, canvas = O.raster_join(p, xy, off, O.PointSet(pts), want_canvas=True)
+    ob = O.errorbounds(p, xy, off, canvas)
+    c, _, _, _ = run(handler, rj.ERRORBOUNDS, xy, off, pts, eps=eps)
+    b = handler.get_error_bounds(rj.ERRORBOUNDS)
+    assert np.array_equal(c, oc)
+    assert np.array_equal(b, ob)
+    assert ob[0] > 0 and ob[2] > 0  # the comb's teeth make boundary pixels on both sides
+
+
 # ---------------------------------------------------------------------------
 # API behaviour
 # ---------------------------------------------------------------------------
