@@ -283,13 +283,22 @@ int rj_ctx::upload(DevBuf& b, const void* src, size_t bytes, const char* what)
     return RJ_OK;
 }
 
+// The device address of pinned host memory from pinned() (nullptr if the
+// runtime gives none: callers then use a copy-engine transfer).
+static void* host_dev(void* h)
+{
+    void* d = nullptr;
+    return hipHostGetDevicePointer(&d, h, 0) == hipSuccess ? d : nullptr;
+}
+
 int rj_ctx::pinned(void** p, size_t* cur, size_t bytes, const char* what)
 {
     if (*p && *cur >= bytes) return RJ_OK;
     if (*p) (void)hipHostFree(*p);
     *p = nullptr;
     *cur = 0;
-    hipError_t e = hipHostMalloc(p, std::max<size_t>(bytes, 64));
+    // mapped and coherent: kernels read and write it directly (k_copy_words)
+    hipError_t e = hipHostMalloc(p, std::max<size_t>(bytes, 64), hipHostMallocMapped | hipHostMallocCoherent);
     if (e != hipSuccess) {
         *p = nullptr;
         return hip_fail(e, what);
@@ -476,7 +485,17 @@ int rj_ctx::canvas_for(rj_polyset* P, double eps_m, double gres, Canvas** out)
         P->canvases.erase(plan.cell);
         return rc;
     }
-    hipError_t e = hipMemcpyAsync(C->d_pc.p, h_stage, total, hipMemcpyHostToDevice, stream);
+    // parts are 256-byte aligned, so total is whole 64-bit words
+    hipError_t e = hipSuccess;
+    if (void* ds = host_dev(h_stage)) {
+        if ((rc = launch("upload", [&] {
+                 launch_copy_words(stream, CopyWords{(const unsigned long long*)ds, C->d_pc.as<unsigned long long>(),
+                                                     total / 8});
+             })))
+            return rc;
+    } else {
+        e = hipMemcpyAsync(C->d_pc.p, h_stage, total, hipMemcpyHostToDevice, stream);
+    }
     if (e != hipSuccess) return hip_fail(e, "polygon regions");
     if (!stage_ev) (void)hipEventCreateWithFlags(&stage_ev, hipEventDisableTiming);
     (void)hipEventRecord(stage_ev, stream);
@@ -1319,9 +1338,18 @@ int rj_ctx::run_single(rj_table* t, const QuerySpec& s, QueryOut* out)
     // after the results: never both)
     const size_t res_bytes = (res_words + 3) * 8;
     if ((rc = pinned((void**)&h_res, &h_res_bytes, res_bytes, "pinned results"))) return rc;
-    hipError_t e = hipMemcpyAsync(h_res, res_base, (res_words + (check_queue ? 1 : 0)) * 8, hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess && check_aux)
-        e = hipMemcpyAsync(h_res + res_words, check_aux, 16, hipMemcpyDeviceToHost, stream);
+    hipError_t e = hipSuccess;
+    const size_t main_words = res_words + (check_queue ? 1 : 0);
+    if (auto* dr = (unsigned long long*)host_dev(h_res)) {
+        // one kernel writes the results (and counters) into pinned memory
+        const CopyWords aux = check_aux ? CopyWords{check_aux, dr + res_words, 2} : CopyWords{nullptr, nullptr, 0};
+        if ((rc = launch("read_results", [&] { launch_copy_words(stream, CopyWords{res_base, dr, main_words}, aux); })))
+            return rc;
+    } else {
+        e = hipMemcpyAsync(h_res, res_base, main_words * 8, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess && check_aux)
+            e = hipMemcpyAsync(h_res + res_words, check_aux, 16, hipMemcpyDeviceToHost, stream);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "execute");
     const uint32_t queued = check_queue ? (uint32_t)h_res[res_words] : 0;

@@ -64,6 +64,19 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v)
     for (; i < n; i += stride) p[i] = v;
 }
 
+// Up to two ranges of 64-bit words copied by one kernel; either side may be
+// pinned host memory (its device address), so small uploads and readbacks
+// run on the compute queue in stream order instead of as copy-engine
+// transfers that the next kernel has to wait for.
+__global__ void k_copy_words(CopyWords r0, CopyWords r1)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < r0.n + r1.n; i += stride) {
+        if (i < r0.n) r0.dst[i] = r0.src[i];
+        else r1.dst[i - r0.n] = r1.src[i - r0.n];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // scan conversion of a band of one polygon
 // ---------------------------------------------------------------------------
@@ -595,6 +608,14 @@ void launch_prev(hipStream_t s, const int32_t* off, int np, int64_t nv, int32_t*
 {
     if (nv <= 0) return;
     hipLaunchKernelGGL(k_prev, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, off, np, nv, prev);
+}
+
+void launch_copy_words(hipStream_t s, CopyWords r0, CopyWords r1)
+{
+    const size_t n = r0.n + r1.n;
+    if (n == 0) return;
+    int blocks = (int)std::min<size_t>((n + kBlock - 1) / kBlock, 64);
+    hipLaunchKernelGGL(k_copy_words, dim3(blocks), dim3(kBlock), 0, s, r0, r1);
 }
 
 void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v)

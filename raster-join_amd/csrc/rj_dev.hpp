@@ -259,6 +259,13 @@ inline size_t band_lds_bytes(int mode, int nrows, int wpr)
 
 // ---- launchers (kernels.hip) ----
 void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);
+// n 64-bit words from src to dst (device addresses; pinned host memory included)
+struct CopyWords {
+    const unsigned long long* src;
+    unsigned long long* dst;
+    size_t n;
+};
+void launch_copy_words(hipStream_t s, CopyWords r0, CopyWords r1 = CopyWords{nullptr, nullptr, 0});
 // prev[k]: the vertex before k in its polygon's ring (the last for the first)
 void launch_prev(hipStream_t s, const int32_t* off, int np, int64_t nv, int32_t* prev);
 void launch_snap(hipStream_t s, const float* xy, int64_t nv, double minx, double miny, double fxs, double fys,
